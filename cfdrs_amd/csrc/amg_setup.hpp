@@ -1,0 +1,40 @@
+// Host-side AMG setup of libcfdla (see amg_setup.cpp).
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+
+namespace cfdla {
+
+struct CoarseSplit {
+  std::vector<int32_t> coarse_points;
+  std::vector<int32_t> fine_to_coarse;  // -1 = no coarse point
+  HostCsr strength;
+};
+
+struct HostLevel {
+  HostCsr A, P, R;  // P, R empty on the coarsest level
+  std::vector<int32_t> fine_to_coarse;
+};
+
+struct HostHierarchy {
+  std::vector<HostLevel> levels;
+};
+
+HostCsr csr_transpose(const HostCsr &a);
+HostCsr csr_multiply(const HostCsr &a, const HostCsr &b);
+HostCsr strong_connections(const HostCsr &a, double theta);
+CoarseSplit greedy_aggregates(const HostCsr &a, int max_size);
+CoarseSplit ruge_stueben_split(const HostCsr &a, double theta);
+CoarseSplit hybrid_split(const HostCsr &a, double theta);
+CoarseSplit three_phase_aggregates(const HostCsr &a, double theta);
+HostCsr piecewise_constant_p(const std::vector<int32_t> &f2c, int64_t n, int64_t nc);
+HostCsr classical_p(const HostCsr &a, const std::vector<int32_t> &cpts, const HostCsr &s);
+HostCsr smooth_prolongator(const HostCsr &a, const HostCsr &pt, double omega);
+std::vector<double> dense_inverse(const HostCsr &a);
+
+void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHierarchy &h);
+void recompute_hierarchy(HostHierarchy &h, const double *fine_values);
+
+}  // namespace cfdla

@@ -1,0 +1,594 @@
+// extern "C" entry points of libcfdla (declared in include/cfdla.h).
+// Every entry point converts C++ exceptions into a status code and a
+// thread-local message; nothing throws across the boundary.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "solver.hpp"
+
+using namespace cfdla;
+
+struct cfdla_matrix_s {
+  HostCsr host;
+  DevCsr dev;
+  bool on_device = false;     // device copy made on first device use
+  std::vector<int64_t> perm;  // stored position -> caller position (empty: identity)
+  int symmetric = -1;         // -1 unknown
+  std::unique_ptr<cfdla_matrix_s> transpose;
+  std::unique_ptr<Precond> transpose_prec;
+};
+
+struct cfdla_precond_s {
+  std::unique_ptr<Precond> p;
+  cfdla_matrix_s *m = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+cfdla_status guarded(F &&f) {
+  try {
+    f();
+    g_error.clear();
+    return CFDLA_OK;
+  } catch (const Error &e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_error = "host allocation failed";
+    return CFDLA_ERR_OOM;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return CFDLA_ERR_INVALID_ARG;
+  }
+}
+
+void require_ptr(const void *p, const char *what) {
+  CFDLA_REQUIRE(p != nullptr, CFDLA_ERR_INVALID_ARG, std::string(what) + " must not be NULL");
+}
+
+// Validate caller CSR, sort each row by column and reject duplicates.
+void ingest_csr(int64_t nrows, int64_t ncols, const int64_t *rp, const int64_t *ci, const double *v, HostCsr &h,
+                std::vector<int64_t> &perm) {
+  CFDLA_REQUIRE(nrows >= 0 && ncols >= 0, CFDLA_ERR_INVALID_ARG, "matrix dimensions must be non-negative");
+  require_ptr(rp, "row_ptr");
+  CFDLA_REQUIRE(rp[0] == 0, CFDLA_ERR_INVALID_ARG, "row_ptr[0] must be 0");
+  for (int64_t i = 0; i < nrows; ++i)
+    CFDLA_REQUIRE(rp[i + 1] >= rp[i], CFDLA_ERR_INVALID_ARG, "row_ptr must be non-decreasing");
+  const int64_t nnz = rp[nrows];
+  CFDLA_REQUIRE(nnz < INT32_MAX && nrows < INT32_MAX && ncols < INT32_MAX, CFDLA_ERR_INVALID_ARG,
+                "matrix exceeds 32-bit device indexing");
+  if (nnz > 0) {
+    require_ptr(ci, "col_indices");
+    require_ptr(v, "values");
+  }
+  h.n = nrows;
+  h.m = ncols;
+  h.rp.assign(rp, rp + nrows + 1);
+  h.ci.resize((size_t)nnz);
+  h.v.resize((size_t)nnz);
+  perm.clear();
+  bool sorted = true;
+  for (int64_t i = 0; i < nrows && sorted; ++i)
+    for (int64_t k = rp[i] + 1; k < rp[i + 1]; ++k)
+      if (ci[k] <= ci[k - 1]) { sorted = false; break; }
+  std::vector<int64_t> order;
+  if (!sorted) {
+    order.resize((size_t)nnz);
+    std::iota(order.begin(), order.end(), 0);
+    for (int64_t i = 0; i < nrows; ++i)
+      std::stable_sort(order.begin() + rp[i], order.begin() + rp[i + 1],
+                       [&](int64_t a, int64_t b) { return ci[a] < ci[b]; });
+  }
+  for (int64_t i = 0; i < nrows; ++i)
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+      int64_t src = sorted ? k : order[k];
+      int64_t c = ci[src];
+      CFDLA_REQUIRE(c >= 0 && c < ncols, CFDLA_ERR_INVALID_ARG, "column index out of range");
+      CFDLA_REQUIRE(k == rp[i] || c != h.ci[k - 1], CFDLA_ERR_INVALID_ARG, "duplicate column index in a row");
+      h.ci[k] = (int32_t)c;
+      h.v[k] = v[src];
+    }
+  if (!sorted) perm = std::move(order);
+}
+
+DevCsr &devmat(cfdla_matrix_s *m) {
+  if (!m->on_device) {
+    m->dev.upload(m->host, true, false);
+    m->on_device = true;
+  }
+  return m->dev;
+}
+
+void permuted_values(const cfdla_matrix_s *m, const double *values, std::vector<double> &out) {
+  const size_t nnz = m->host.v.size();
+  out.resize(nnz);
+  if (m->perm.empty()) std::copy(values, values + nnz, out.begin());
+  else
+    for (size_t k = 0; k < nnz; ++k) out[k] = values[m->perm[k]];
+}
+
+cfdla_report to_report(const SolveOut &o) {
+  cfdla_report r;
+  std::memset(&r, 0, sizeof(r));
+  r.iterations = o.iters;
+  r.outer_iterations = o.outer;
+  r.initial_residual_norm = o.rn0;
+  r.final_residual_norm = o.rn;
+  r.threshold = o.thr;
+  r.termination = o.term;
+  return r;
+}
+
+bool is_symmetric(cfdla_matrix_s *m) {
+  if (m->symmetric >= 0) return m->symmetric == 1;
+  const HostCsr &a = m->host;
+  bool sym = a.n == a.m;
+  for (int64_t i = 0; i < a.n && sym; ++i)
+    for (int64_t k = a.rp[i]; k < a.rp[i + 1] && sym; ++k) {
+      int32_t j = a.ci[k];
+      auto b = a.ci.begin() + a.rp[j], e = a.ci.begin() + a.rp[j + 1];
+      auto it = std::lower_bound(b, e, (int32_t)i);
+      sym = it != e && *it == i && a.v[(size_t)(it - a.ci.begin())] == a.v[k];
+    }
+  m->symmetric = sym ? 1 : 0;
+  return sym;
+}
+
+uint64_t splitmix64(uint64_t &state) {
+  // SplitMix64 (Steele, Lea & Flood 2014)
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cfdla_last_error(void) { return g_error.c_str(); }
+
+const char *cfdla_version(void) { return "cfdla 0.1.0 (gfx950)"; }
+
+cfdla_status cfdla_init(int32_t device) {
+  return guarded([&] { init_device(device); });
+}
+
+cfdla_status cfdla_set_stream(void *hip_stream) {
+  return guarded([&] { set_stream(reinterpret_cast<hipStream_t>(hip_stream)); });
+}
+
+cfdla_status cfdla_synchronize(void) {
+  return guarded([&] { sync_stream(); });
+}
+
+cfdla_status cfdla_matrix_create(int64_t nrows, int64_t ncols, const int64_t *row_ptr, const int64_t *col_indices,
+                                 const double *values, cfdla_matrix *out) {
+  return guarded([&] {
+    require_ptr(out, "out");
+    auto m = std::make_unique<cfdla_matrix_s>();
+    ingest_csr(nrows, ncols, row_ptr, col_indices, values, m->host, m->perm);
+    // device copy is made on first use (devmat)
+    *out = m.release();
+  });
+}
+
+cfdla_status cfdla_matrix_create_device(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t *row_ptr_dev,
+                                        const int32_t *col_dev, const double *values_dev, cfdla_matrix *out) {
+  return guarded([&] {
+    require_ptr(out, "out");
+    require_ptr(row_ptr_dev, "row_ptr_dev");
+    CFDLA_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, CFDLA_ERR_INVALID_ARG, "sizes must be non-negative");
+    std::vector<int32_t> rp((size_t)nrows + 1), ci((size_t)nnz);
+    std::vector<double> v((size_t)nnz);
+    CFDLA_HIP(hipMemcpyAsync(rp.data(), row_ptr_dev, rp.size() * 4, hipMemcpyDeviceToHost, stream()));
+    if (nnz) {
+      require_ptr(col_dev, "col_dev");
+      require_ptr(values_dev, "values_dev");
+      CFDLA_HIP(hipMemcpyAsync(ci.data(), col_dev, ci.size() * 4, hipMemcpyDeviceToHost, stream()));
+      CFDLA_HIP(hipMemcpyAsync(v.data(), values_dev, v.size() * 8, hipMemcpyDeviceToHost, stream()));
+    }
+    sync_stream();
+    CFDLA_REQUIRE(rp[nrows] == nnz, CFDLA_ERR_DIMENSION, "row_ptr[nrows] != nnz");
+    std::vector<int64_t> rp64(rp.begin(), rp.end()), ci64(ci.begin(), ci.end());
+    auto m = std::make_unique<cfdla_matrix_s>();
+    ingest_csr(nrows, ncols, rp64.data(), ci64.data(), v.data(), m->host, m->perm);
+    // device copy is made on first use (devmat)
+    *out = m.release();
+  });
+}
+
+cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(values, "values");
+    std::vector<double> v;
+    permuted_values(m, values, v);
+    m->host.v = v;
+    m->symmetric = -1;
+    m->transpose.reset();
+    m->transpose_prec.reset();
+    if (m->on_device) m->dev.upload(m->host, true, false);
+  });
+}
+
+cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, int64_t *nnz) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    if (nrows) *nrows = m->host.n;
+    if (ncols) *ncols = m->host.m;
+    if (nnz) *nnz = m->host.nnz();
+  });
+}
+
+cfdla_status cfdla_matrix_destroy(cfdla_matrix m) {
+  return guarded([&] { delete m; });
+}
+
+cfdla_status cfdla_spmv(cfdla_matrix m, const double *x, double *y) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(x, "x");
+    require_ptr(y, "y");
+    Context &c = ctx();
+    double *dx = c.dv_at(9, std::max<int64_t>(1, m->host.m)), *dy = c.dv_at(10, std::max<int64_t>(1, m->host.n));
+    CFDLA_HIP(hipMemcpyAsync(dx, x, m->host.m * 8, hipMemcpyHostToDevice, stream()));
+    csr_apply<double>(devmat(m), EPI_Y, dx, nullptr, nullptr, 0.0, dy, c.s(3), false, OP_NONE, nullptr, c.ws);
+    CFDLA_HIP(hipMemcpyAsync(y, dy, m->host.n * 8, hipMemcpyDeviceToHost, stream()));
+    sync_stream();
+  });
+}
+
+cfdla_status cfdla_spmv_device(cfdla_matrix m, int32_t dtype, const void *x, void *y) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    Context &c = ctx();
+    if (dtype == CFDLA_F64)
+      csr_apply<double>(devmat(m), EPI_Y, static_cast<const double *>(x), nullptr, nullptr, 0.0,
+                        static_cast<double *>(y), c.s(3), false, OP_NONE, nullptr, c.ws);
+    else if (dtype == CFDLA_F32)
+      csr_apply<float>(devmat(m), EPI_Y, static_cast<const float *>(x), nullptr, nullptr, 0.0, static_cast<float *>(y),
+                       c.s(3), false, OP_NONE, nullptr, c.ws);
+    else
+      throw Error(CFDLA_ERR_INVALID_ARG, "dtype must be CFDLA_F64 or CFDLA_F32");
+  });
+}
+
+cfdla_status cfdla_residual_device(cfdla_matrix m, const double *b, const double *x, double *r, double *rnorm) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    Context &c = ctx();
+    Scal *s = c.s(3);
+    scal_reset(s, 0.0);
+    csr_apply<double>(devmat(m), EPI_RESID, x, b, nullptr, 0.0, r, s, false, OP_INIT_RR, r, c.ws);
+    Scal h = c.read(s);
+    if (rnorm) *rnorm = std::sqrt(h.rr);
+  });
+}
+
+cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(out, "out");
+    CFDLA_REQUIRE(m->host.n == m->host.m, CFDLA_ERR_DIMENSION, "Jacobi requires a square matrix");
+    auto p = std::make_unique<cfdla_precond_s>();
+    p->p = std::make_unique<JacobiPrecond>(m->host);
+    p->m = m;
+    *out = p.release();
+  });
+}
+
+cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *config, cfdla_precond *out) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(config, "config");
+    require_ptr(out, "out");
+    auto p = std::make_unique<cfdla_precond_s>();
+    p->p = std::make_unique<AmgPrecond>(m->host, *config);
+    p->m = m;
+    *out = p.release();
+  });
+}
+
+cfdla_status cfdla_precond_apply(cfdla_precond p, const double *r, double *z) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    require_ptr(r, "r");
+    require_ptr(z, "z");
+    Context &c = ctx();
+    const int64_t n = p->m->host.n;
+    double *dr = c.dv_at(9, std::max<int64_t>(1, n)), *dz = c.dv_at(10, std::max<int64_t>(1, n));
+    CFDLA_HIP(hipMemcpyAsync(dr, r, n * 8, hipMemcpyHostToDevice, stream()));
+    scal_reset(c.s(3), 0.0);
+    p->p->apply(dr, dz, c.s(3), false);
+    CFDLA_HIP(hipMemcpyAsync(z, dz, n * 8, hipMemcpyDeviceToHost, stream()));
+    sync_stream();
+  });
+}
+
+cfdla_status cfdla_precond_apply_device(cfdla_precond p, int32_t dtype, const void *r, void *z) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    Context &c = ctx();
+    scal_reset(c.s(3), 0.0);
+    if (dtype == CFDLA_F64) p->p->apply(static_cast<const double *>(r), static_cast<double *>(z), c.s(3), false);
+    else if (dtype == CFDLA_F32) p->p->apply(static_cast<const float *>(r), static_cast<float *>(z), c.s(3), false);
+    else throw Error(CFDLA_ERR_INVALID_ARG, "dtype must be CFDLA_F64 or CFDLA_F32");
+  });
+}
+
+cfdla_status cfdla_amg_update_values(cfdla_precond p, const double *values) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    require_ptr(values, "values");
+    auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
+    CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
+    std::vector<double> v;
+    permuted_values(p->m, values, v);
+    amg->update_values(v.data());
+  });
+}
+
+cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    require_ptr(out, "out");
+    auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
+    CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
+    std::memset(out, 0, sizeof(*out));
+    const auto &lv = amg->host.levels;
+    out->num_levels = (int32_t)lv.size();
+    auto count = [](const HostCsr &a) {
+      int64_t c = 0;
+      for (double x : a.v) c += std::fabs(x) > 1e-15;
+      return c;
+    };
+    int64_t fine = count(lv[0].A), tot = 0, vars = 0;
+    for (size_t l = 0; l < lv.size(); ++l) {
+      out->level_rows[l] = lv[l].A.n;
+      out->level_nnz[l] = lv[l].A.nnz();
+      tot += count(lv[l].A);
+      vars += lv[l].A.n;
+    }
+    out->operator_complexity = fine > 0 ? (double)tot / (double)fine : 1.0;
+    out->grid_complexity = lv[0].A.n > 0 ? (double)vars / (double)lv[0].A.n : 1.0;
+    out->setup_time = amg->setup_time;
+  });
+}
+
+cfdla_status cfdla_amg_export_level(cfdla_precond p, int32_t level, int32_t which, int64_t *nrows, int64_t *ncols,
+                                    int64_t *nnz, int64_t *row_ptr, int64_t *col_indices, double *values) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
+    CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
+    CFDLA_REQUIRE(level >= 0 && level < (int32_t)amg->host.levels.size(), CFDLA_ERR_INVALID_ARG,
+                  "level out of range");
+    const HostLevel &L = amg->host.levels[(size_t)level];
+    CFDLA_REQUIRE(which >= 0 && which <= 2, CFDLA_ERR_INVALID_ARG, "which must be 0 (A), 1 (R) or 2 (P)");
+    const HostCsr &a = which == 0 ? L.A : which == 1 ? L.R : L.P;
+    if (nrows) *nrows = a.n;
+    if (ncols) *ncols = a.m;
+    if (nnz) *nnz = a.nnz();
+    if (row_ptr) std::copy(a.rp.begin(), a.rp.end(), row_ptr);
+    if (col_indices) std::copy(a.ci.begin(), a.ci.end(), col_indices);
+    if (values) std::copy(a.v.begin(), a.v.end(), values);
+  });
+}
+
+cfdla_status cfdla_amg_export_map(cfdla_precond p, int32_t level, int64_t *fine_to_coarse) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    require_ptr(fine_to_coarse, "fine_to_coarse");
+    auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
+    CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
+    CFDLA_REQUIRE(level >= 0 && level < (int32_t)amg->host.levels.size(), CFDLA_ERR_INVALID_ARG,
+                  "level out of range");
+    const auto &m = amg->host.levels[(size_t)level].fine_to_coarse;
+    for (size_t i = 0; i < m.size(); ++i) fine_to_coarse[i] = m[i];
+  });
+}
+
+cfdla_status cfdla_precond_destroy(cfdla_precond p) {
+  return guarded([&] { delete p; });
+}
+
+cfdla_status cfdla_solve(cfdla_matrix m, cfdla_precond p, const double *b, double *x_inout,
+                         const cfdla_solver_config *config, cfdla_report *report) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(b, "b");
+    require_ptr(x_inout, "x");
+    require_ptr(config, "config");
+    CFDLA_REQUIRE(!p || p->m->host.n == m->host.n, CFDLA_ERR_DIMENSION, "preconditioner size differs from matrix");
+    Context &c = ctx();
+    const int64_t n = m->host.n;
+    double *db = c.dv_at(9, std::max<int64_t>(1, n)), *dx = c.dv_at(10, std::max<int64_t>(1, n));
+    CFDLA_HIP(hipMemcpyAsync(db, b, n * 8, hipMemcpyHostToDevice, stream()));
+    CFDLA_HIP(hipMemcpyAsync(dx, x_inout, n * 8, hipMemcpyHostToDevice, stream()));
+    SolveOut o = solve_device(devmat(m), p ? p->p.get() : nullptr, db, dx, *config);
+    CFDLA_HIP(hipMemcpyAsync(x_inout, dx, n * 8, hipMemcpyDeviceToHost, stream()));
+    sync_stream();
+    if (report) *report = to_report(o);
+  });
+}
+
+cfdla_status cfdla_solve_device(cfdla_matrix m, cfdla_precond p, const double *b_dev, double *x_dev,
+                                const cfdla_solver_config *config, cfdla_report *report) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(config, "config");
+    CFDLA_REQUIRE(!p || p->m->host.n == m->host.n, CFDLA_ERR_DIMENSION, "preconditioner size differs from matrix");
+    SolveOut o = solve_device(devmat(m), p ? p->p.get() : nullptr, b_dev, x_dev, *config);
+    if (report) *report = to_report(o);
+  });
+}
+
+cfdla_status cfdla_solve_transpose_device(cfdla_matrix m, cfdla_precond p, const double *g_dev, double *lambda_dev,
+                                          const cfdla_solver_config *config, cfdla_report *report) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(config, "config");
+    SolveOut o;
+    if (is_symmetric(m)) {
+      o = solve_device(devmat(m), p ? p->p.get() : nullptr, g_dev, lambda_dev, *config);
+    } else {
+      if (!m->transpose) {
+        auto t = std::make_unique<cfdla_matrix_s>();
+        HostCsr tr;
+        tr.n = m->host.m;
+        tr.m = m->host.n;
+        tr.rp.assign((size_t)tr.n + 1, 0);
+        for (int32_t c : m->host.ci) tr.rp[(size_t)c + 1]++;
+        for (int64_t j = 0; j < tr.n; ++j) tr.rp[j + 1] += tr.rp[j];
+        tr.ci.resize(m->host.ci.size());
+        tr.v.resize(m->host.v.size());
+        std::vector<int64_t> next(tr.rp.begin(), tr.rp.end() - 1);
+        for (int64_t i = 0; i < m->host.n; ++i)
+          for (int64_t k = m->host.rp[i]; k < m->host.rp[i + 1]; ++k) {
+            int64_t q = next[m->host.ci[k]]++;
+            tr.ci[q] = (int32_t)i;
+            tr.v[q] = m->host.v[k];
+          }
+        t->host = std::move(tr);
+        t->dev.upload(t->host, true, false);
+        t->on_device = true;
+        m->transpose_prec = std::make_unique<JacobiPrecond>(t->host);
+        m->transpose = std::move(t);
+      }
+      o = solve_device(m->transpose->dev, m->transpose_prec.get(), g_dev, lambda_dev, *config);
+    }
+    if (report) *report = to_report(o);
+  });
+}
+
+cfdla_status cfdla_coef_grad_poisson2d_device(int64_t nx, int64_t ny, double inv_h2, int32_t dir_mask,
+                                              const double *dir_val, const double *k, const double *p,
+                                              const double *lambda, const double *dJdk, double *dJdg) {
+  return guarded([&] {
+    CFDLA_REQUIRE(nx > 0 && ny > 0 && nx * ny < INT32_MAX, CFDLA_ERR_INVALID_ARG, "grid size out of range");
+    require_ptr(dir_val, "dir_val");
+    require_ptr(k, "k");
+    require_ptr(p, "p");
+    require_ptr(lambda, "lambda");
+    require_ptr(dJdg, "dJdg");
+    coef_grad_poisson2d(nx, ny, inv_h2, dir_mask, dir_val, k, p, lambda, dJdk, dJdg);
+    sync_stream();
+  });
+}
+
+cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n, double *out) {
+  return guarded([&] {
+    CFDLA_REQUIRE(n >= 0, CFDLA_ERR_INVALID_ARG, "n must be non-negative");
+    if (n) require_ptr(out, "out");
+    uint64_t st = seed;
+    const double scale = 1.0 / 9007199254740992.0;  // 2^-53
+    for (int64_t i = 0; i < n; ++i) {
+      double u = (double)(splitmix64(st) >> 11) * scale;
+      out[i] = lo + (hi - lo) * u;
+    }
+  });
+}
+
+cfdla_status cfdla_problem_pressure_correction(int64_t nxc, int64_t nyc, double dx, double dy,
+                                               const int32_t *dirichlet, int64_t *nnz, int64_t *row_ptr,
+                                               int64_t *col_indices, double *values) {
+  // Cell (i, j), i = x index (slow), j = y index (fast): row i*nyc + j.
+  // Neighbour terms and diagonal accumulate in the reference order
+  // west, east, south, north (correction.rs:291-334); columns sorted.
+  return guarded([&] {
+    CFDLA_REQUIRE(nxc >= 1 && nyc >= 1, CFDLA_ERR_INVALID_ARG, "grid must have at least one cell");
+    require_ptr(dirichlet, "dirichlet");
+    require_ptr(nnz, "nnz");
+    const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
+    int64_t count = 0;
+    for (int64_t i = 0; i < nxc; ++i)
+      for (int64_t j = 0; j < nyc; ++j) count += 1 + (i > 0) + (i + 1 < nxc) + (j > 0) + (j + 1 < nyc);
+    *nnz = count;
+    if (!row_ptr) return;
+    require_ptr(col_indices, "col_indices");
+    require_ptr(values, "values");
+    int64_t o = 0;
+    row_ptr[0] = 0;
+    for (int64_t i = 0; i < nxc; ++i)
+      for (int64_t j = 0; j < nyc; ++j) {
+        const int64_t row = i * nyc + j;
+        double ap = 0.0;
+        if (i > 0) ap += ix2; else if (dirichlet[0]) ap += ix2;
+        if (i + 1 < nxc) ap += ix2; else if (dirichlet[1]) ap += ix2;
+        if (j > 0) ap += iy2; else if (dirichlet[2]) ap += iy2;
+        if (j + 1 < nyc) ap += iy2; else if (dirichlet[3]) ap += iy2;
+        if (i > 0) { col_indices[o] = row - nyc; values[o++] = -ix2; }
+        if (j > 0) { col_indices[o] = row - 1; values[o++] = -iy2; }
+        col_indices[o] = row; values[o++] = ap;
+        if (j + 1 < nyc) { col_indices[o] = row + 1; values[o++] = -iy2; }
+        if (i + 1 < nxc) { col_indices[o] = row + nyc; values[o++] = -ix2; }
+        row_ptr[row + 1] = o;
+      }
+  });
+}
+
+cfdla_status cfdla_problem_varcoef(int64_t nx, int64_t ny, int64_t nz, double inv_h2, const double *k,
+                                   int32_t dir_mask, const double *dir_val, int64_t *nnz, int64_t *row_ptr,
+                                   int64_t *col_indices, double *values, double *rhs_bc) {
+  // Cell (x, y, z) -> row (z*ny + y)*nx + x.  Face coefficient
+  // 2 kP kN / (kP + kN) * inv_h2; diagonal accumulates the faces in the order
+  // -z, -y, -x, +x, +y, +z, then the Dirichlet faces in the same order.
+  return guarded([&] {
+    CFDLA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, CFDLA_ERR_INVALID_ARG, "grid must have at least one cell");
+    require_ptr(k, "k");
+    require_ptr(nnz, "nnz");
+    const bool three_d = nz > 1;
+    int64_t count = 0;
+    for (int64_t z = 0; z < nz; ++z)
+      for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x)
+          count += 1 + (x > 0) + (x + 1 < nx) + (y > 0) + (y + 1 < ny) + (three_d && z > 0) + (three_d && z + 1 < nz);
+    *nnz = count;
+    if (!row_ptr) return;
+    require_ptr(col_indices, "col_indices");
+    require_ptr(values, "values");
+    if (dir_mask) require_ptr(dir_val, "dir_val");
+    const int64_t sy = nx, sz = nx * ny;
+    int64_t o = 0;
+    row_ptr[0] = 0;
+    for (int64_t z = 0; z < nz; ++z)
+      for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x) {
+          const int64_t p = (z * ny + y) * nx + x;
+          const double kp = k[p];
+          auto coef = [&](int64_t q) { return (2.0 * kp * k[q] / (kp + k[q])) * inv_h2; };
+          const bool has[6] = {three_d && z > 0, y > 0, x > 0, x + 1 < nx, y + 1 < ny, three_d && z + 1 < nz};
+          const int64_t nb[6] = {p - sz, p - sy, p - 1, p + 1, p + sy, p + sz};
+          double c[6] = {0, 0, 0, 0, 0, 0};
+          double diag = 0.0;
+          for (int f = 0; f < 6; ++f)
+            if (has[f]) { c[f] = coef(nb[f]); diag += c[f]; }
+          const bool on[6] = {three_d && z == 0, y == 0, x == 0, x == nx - 1, y == ny - 1, three_d && z == nz - 1};
+          const int bit[6] = {4, 2, 0, 1, 3, 5};
+          double bc = 0.0;
+          for (int f = 0; f < 6; ++f)
+            if (on[f] && ((dir_mask >> bit[f]) & 1)) {
+              const double cd = kp * inv_h2;
+              diag += cd;
+              bc += cd * dir_val[bit[f]];
+            }
+          if (rhs_bc) rhs_bc[p] = bc;
+          for (int f = 0; f < 3; ++f)
+            if (has[f]) { col_indices[o] = nb[f]; values[o++] = -c[f]; }
+          col_indices[o] = p;
+          values[o++] = diag;
+          for (int f = 3; f < 6; ++f)
+            if (has[f]) { col_indices[o] = nb[f]; values[o++] = -c[f]; }
+          row_ptr[p + 1] = o;
+        }
+  });
+}
+
+}  // extern "C"

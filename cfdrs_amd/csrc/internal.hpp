@@ -1,0 +1,219 @@
+// Shared host-side definitions of libcfdla: error handling, device buffers,
+// device CSR storage, reduction scalars and kernel launcher declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/cfdla.h"
+
+namespace cfdla {
+
+struct Error : std::runtime_error {
+  cfdla_status code;
+  Error(cfdla_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define CFDLA_HIP(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::cfdla::Error(e_ == hipErrorOutOfMemory ? CFDLA_ERR_OOM : CFDLA_ERR_DEVICE, \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define CFDLA_REQUIRE(cond, code, msg)                 \
+  do {                                                 \
+    if (!(cond)) throw ::cfdla::Error((code), (msg));  \
+  } while (0)
+
+// Library device/stream state.
+void init_device(int dev);
+void ensure_init();
+hipStream_t stream();
+void set_stream(hipStream_t s);
+void sync_stream();
+int num_cus();
+
+// Owning device allocation (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    CFDLA_HIP(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) { if (n < count) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count) {
+    ensure(count);
+    if (count) CFDLA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+  }
+  void download(T *h, size_t count) const {
+    if (count) CFDLA_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+  }
+};
+
+// Host CSR with int32 column indices and int64 row pointers (setup side).
+struct HostCsr {
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> ci;
+  std::vector<double> v;
+  int64_t nnz() const { return (int64_t)ci.size(); }
+};
+
+// Device CSR: int32 row pointers and columns, FP64 values with an optional
+// FP32 copy, and the smoother's inverse diagonal (0 where |a_ii| <= 1e-15,
+// the skip rule of the reference Jacobi smoother).
+struct DevCsr {
+  int n = 0, m = 0;
+  int64_t nnz = 0;
+  int group = 8;  // lanes per row of the vector-CSR kernel (power of two, 1..64)
+  DevBuf<int32_t> rp, ci;
+  DevBuf<double> v64, sdinv64;
+  DevBuf<float> v32, sdinv32;
+  void upload(const HostCsr &h, bool want_fp32, bool want_dinv);
+  void update_values(const double *host_values, bool want_fp32, bool want_dinv);
+  void ensure_fp32();
+  template <typename T> const T *vals() const;
+  template <typename T> const T *sdinv() const;
+};
+template <> inline const double *DevCsr::vals<double>() const { return v64.p; }
+template <> inline const float *DevCsr::vals<float>() const { return v32.p; }
+template <> inline const double *DevCsr::sdinv<double>() const { return sdinv64.p; }
+template <> inline const float *DevCsr::sdinv<float>() const { return sdinv32.p; }
+
+int pick_group(int64_t n, int64_t nnz);
+
+// Device-resident scalars of one solve.  Reductions write red[]; the last
+// block of a reducing kernel then runs a finalize op (see kernels.hip) on
+// thread 0, so the Krylov coefficients never travel to the host.
+struct Scal {
+  double red[8];
+  double rz, alpha, beta, thr2, rr, pq;
+  double rho, omega, rn0;
+  double hn;      // GMRES: norm of the newest Arnoldi vector
+  double pad[2];
+  int32_t done;   // 0 running, 1 residual estimate below threshold, 2 stopped (term), 3 see BiCGSTAB
+  int32_t iters;  // total iterations (Arnoldi steps for GMRES)
+  int32_t term;
+  int32_t kstep;  // GMRES: Arnoldi steps in the current cycle
+};
+
+// Per-stream reduction workspace: block partials and an arrival counter that
+// the last block resets, so the workspace needs no clearing between kernels.
+struct RedWs {
+  DevBuf<double> partials;
+  DevBuf<unsigned> counter;
+  void init();
+};
+
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 2048;
+constexpr int kMaxRed = 8;
+
+enum FinalizeOp : int {
+  OP_NONE = 0,
+  OP_STORE = 1,        // red[] only
+  OP_INIT_RR = 2,      // rr = red0
+  OP_SET_RZ = 3,       // rz = red0
+  OP_CG_ALPHA = 4,     // pq = red0; alpha = rz / pq (NONPOS if pq <= 0)
+  OP_CG_RR = 5,        // rr = red0; iters++; done = 1 if rr <= thr2
+  OP_CG_BETA = 6,      // rz_new = red0; beta = rz_new / rz
+  OP_CG_RR_RZ = 7,     // rr = red0, rz_new = red1 (fused Jacobi update)
+  OP_BI_RHO = 8,       // BiCGSTAB: rho_new = red0 -> beta
+  OP_BI_ALPHA = 9,     // BiCGSTAB: alpha = rho / (rhat . v)
+  OP_BI_SNORM = 10,    // BiCGSTAB: ||s||^2 = red0
+  OP_BI_OMEGA = 11,    // BiCGSTAB: omega = (t.s)/(t.t), red0 = t.s, red1 = t.t
+  OP_BI_RR = 12,       // BiCGSTAB: rr = red0; iters++
+};
+
+enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3 };
+
+// ---- kernel launchers (kernels.hip) -------------------------------------
+// Vector-CSR product with epilogue:
+//   EPI_Y      y = A x
+//   EPI_RESID  y = b - A x
+//   EPI_JACOBI y = x + omega * dinv .* (b - A x)
+//   EPI_ADD    y = y + A x
+// When dot_op != OP_NONE, also reduces sum_i dotw_i * y_i (dotw defaults to
+// x when null) and runs dot_op.  Skipped entirely when *s->done != 0 and
+// check_done is set.
+template <typename T>
+void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, double omega, T *y,
+               Scal *s, bool check_done, int dot_op, const T *dotw, RedWs &ws);
+
+// Reductions: up to 2 dot products of (a0,b0), (a1,b1) then op.
+template <typename T>
+void dot2(int n, const T *a0, const T *b0, const T *a1, const T *b1, Scal *s, bool check_done, int op,
+          RedWs &ws);
+
+// x = omega * dinv .* b (first Jacobi sweep from a zero guess).
+template <typename T>
+void jacobi_first(int n, const T *dinv, double omega, const T *b, T *x, Scal *s, bool check_done);
+
+// CG update: x += alpha p; r -= alpha q; then rr (and with dinv: z = dinv r,
+// rz) reduced with op OP_CG_RR or OP_CG_RR_RZ.
+template <typename T>
+void cg_update(int n, const T *p, const T *q, T *x, T *r, const T *jdinv, T *z, Scal *s, RedWs &ws);
+
+// p = z + beta p
+template <typename T>
+void cg_direction(int n, const T *z, T *p, Scal *s);
+
+template <typename T>
+void copy_vec(int n, const T *src, T *dst, Scal *s, bool check_done);
+template <typename T>
+void fill_vec(int n, T value, T *dst);
+void f64_to_f32_scaled(int n, const double *src, double scale, float *dst);
+void axpy_f32_to_f64(int n, double a, const float *x, double *y);
+
+// x = Ainv * b (dense row-major FP64 inverse), b/x of type T.
+template <typename T>
+void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done);
+
+// BiCGSTAB vector kernels
+template <typename T>
+void bicg_p_update(int n, const T *r, const T *v, T *p, Scal *s);  // p = r + beta (p - omega v)
+template <typename T>
+void bicg_s_update(int n, const T *r, const T *v, T *sv, Scal *s, RedWs &ws);  // s = r - alpha v; ||s||^2
+template <typename T>
+void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s,
+                     RedWs &ws);  // x += alpha ph + omega sh; r = s - omega t; ||r||^2
+template <typename T>
+void axpy_scal(int n, const T *x, T *y, Scal *s, int which);  // y += s->{alpha} x
+
+// Scalars helpers
+void scal_reset(Scal *s, double thr2);
+void scal_set_done(Scal *s, int value);
+void scal_bicg_restart(Scal *s);  // rho = alpha = omega = 1, done = 0
+
+// Coefficient gradient of the 2D variable-coefficient Poisson operator.
+void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, const double *dir_val,
+                         const double *k, const double *p, const double *lam, const double *dJdk,
+                         double *dJdg);
+
+}  // namespace cfdla

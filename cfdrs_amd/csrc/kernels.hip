@@ -1,0 +1,585 @@
+// gfx950 kernels of libcfdla: vector-CSR SpMV with fused epilogues,
+// deterministic grid reductions whose last block advances the Krylov scalars
+// on the device, Krylov vector updates, Jacobi sweeps and the FP64 dense
+// coarse solve.
+//
+// Wave64 throughout.  Reductions: wave shuffle tree, then LDS across the 4
+// waves of a 256-thread block, block partials to global memory, and the last
+// block to arrive (atomic ticket) sums the partials in block order — the
+// result does not depend on block scheduling, so runs are bitwise
+// reproducible.
+#include <cmath>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace cfdla {
+namespace {
+
+__device__ void finalize(Scal *s, int op) {
+  switch (op) {
+    case OP_INIT_RR:
+      s->rr = s->red[0];
+      break;
+    case OP_SET_RZ:
+      s->rz = s->red[0];
+      break;
+    case OP_CG_ALPHA: {
+      double pq = s->red[0];
+      s->pq = pq;
+      if (!(pq > 0.0) || !isfinite(pq)) {
+        s->term = CFDLA_TERM_NONPOSITIVE_CURVATURE;
+        s->done = 2;
+      } else {
+        s->alpha = s->rz / pq;
+      }
+    } break;
+    case OP_CG_RR: {
+      double rr = s->red[0];
+      s->rr = rr;
+      s->iters += 1;
+      if (!isfinite(rr)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else if (rr <= s->thr2) {
+        s->done = 1;
+      }
+    } break;
+    case OP_CG_BETA: {
+      double rzn = s->red[0];
+      if (rzn == 0.0 || !isfinite(rzn)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else {
+        s->beta = rzn / s->rz;
+        s->rz = rzn;
+      }
+    } break;
+    case OP_CG_RR_RZ: {
+      double rr = s->red[0], rzn = s->red[1];
+      s->rr = rr;
+      s->iters += 1;
+      if (!isfinite(rr)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else if (rr <= s->thr2) {
+        s->done = 1;
+      } else if (rzn == 0.0 || !isfinite(rzn)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else {
+        s->beta = rzn / s->rz;
+        s->rz = rzn;
+      }
+    } break;
+    case OP_BI_RHO: {
+      double rho_new = s->red[0];
+      if (rho_new == 0.0 || !isfinite(rho_new)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else {
+        s->beta = (rho_new / s->rho) * (s->alpha / s->omega);
+        s->rho = rho_new;
+      }
+    } break;
+    case OP_BI_ALPHA: {
+      double rv = s->red[0];
+      if (rv == 0.0 || !isfinite(rv)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else {
+        s->alpha = s->rho / rv;
+      }
+    } break;
+    case OP_BI_SNORM: {
+      double ss = s->red[0];
+      s->iters += 1;
+      if (!isfinite(ss)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else if (ss <= s->thr2) {
+        s->done = 3;  // x += alpha * phat, then confirm on the host
+      }
+    } break;
+    case OP_BI_OMEGA: {
+      double ts = s->red[0], tt = s->red[1];
+      s->omega = tt > 0.0 ? ts / tt : 0.0;
+    } break;
+    case OP_BI_RR: {
+      double rr = s->red[0];
+      s->rr = rr;
+      if (!isfinite(rr)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else if (rr <= s->thr2) {
+        s->done = 1;
+      } else if (s->omega == 0.0 || !isfinite(s->omega)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      }
+    } break;
+    default:
+      break;
+  }
+}
+
+// Reducing kernels only write their block partials; the finalize op runs in
+// the single-block k_finalize launched right after them on the same stream.
+template <int NV>
+__device__ __forceinline__ void reduce_and_finalize(double (&v)[NV], Scal *, int, double *partials, unsigned *) {
+  block_partials<NV>(v, partials);
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_finalize(Scal *s, int op, int nblocks, const double *partials,
+                                                     int check_done) {
+  if (check_done && s->done) return;
+  double tot[NV];
+  sum_partials<NV>(nblocks, partials, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s->red[i] = tot[i];
+    finalize(s, op);
+  }
+}
+
+void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done) {
+  if (nv == 1)
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, stream(), s, op, nblocks, ws.partials.p,
+                       check_done ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kBlock), 0, stream(), s, op, nblocks, ws.partials.p,
+                       check_done ? 1 : 0);
+}
+
+// ---- vector-CSR product ------------------------------------------------------
+// y and dotw carry no __restrict__: the self dot products (r . r with
+// y = dotw = r) alias them, and the store of y[row] must precede the load of
+// dotw[row].
+template <int G, typename T, int EPI, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const T *__restrict__ av, const T *__restrict__ x,
+                                                const T *__restrict__ b, const T *__restrict__ dinv, T omega,
+                                                T *y, const T *dotw, Scal *s, int check_done, int op,
+                                                double *partials, unsigned *counter) {
+  if (check_done && s->done) return;
+  constexpr int GPB = kBlock / G;
+  const int lane = threadIdx.x & (G - 1);
+  int row = blockIdx.x * GPB + threadIdx.x / G;
+  const int stride = gridDim.x * GPB;
+  double dacc = 0.0;
+  for (; row < n; row += stride) {
+    const int rs = rp[row], re = rp[row + 1];
+    T sum = T(0);
+    for (int k = rs + lane; k < re; k += G) sum += av[k] * x[ci[k]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (lane == 0) {
+      T out;
+      if constexpr (EPI == EPI_Y) out = sum;
+      else if constexpr (EPI == EPI_RESID) out = b[row] - sum;
+      else if constexpr (EPI == EPI_JACOBI) out = x[row] + omega * dinv[row] * (b[row] - sum);
+      else out = y[row] + sum;
+      if constexpr (DOT) dacc += (double)(dotw == y ? out : dotw[row]) * (double)out;
+      y[row] = out;
+    }
+  }
+  if constexpr (DOT) {
+    double v[1] = {dacc};
+    reduce_and_finalize<1>(v, s, op, partials, counter);
+  }
+}
+
+template <int G, typename T, int EPI, bool DOT>
+void launch_csr_g(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
+                  Scal *s, bool check_done, int op, RedWs &ws) {
+  constexpr int GPB = kBlock / G;
+  int blocks = blocks_for(A.n, GPB);
+  hipLaunchKernelGGL((k_csr<G, T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.rp.p, A.ci.p,
+                     A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, op, ws.partials.p,
+                     ws.counter.p);
+  if (DOT) finish(1, s, op, blocks, ws, check_done);
+}
+
+template <typename T, int EPI, bool DOT>
+void launch_csr_e(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
+                  Scal *s, bool check_done, int op, RedWs &ws) {
+  switch (A.group) {
+    case 1: launch_csr_g<1, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    case 2: launch_csr_g<2, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    case 4: launch_csr_g<4, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    case 8: launch_csr_g<8, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    case 16: launch_csr_g<16, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    case 32: launch_csr_g<32, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+    default: launch_csr_g<64, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
+  }
+}
+
+// ---- reductions and vector updates ---------------------------------------------
+template <typename T, int NV>
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const T *__restrict__ a0, const T *__restrict__ b0,
+                                                const T *__restrict__ a1, const T *__restrict__ b1, Scal *s,
+                                                int check_done, int op, double *partials, unsigned *counter) {
+  if (check_done && s->done) return;
+  double v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    v[0] += (double)a0[i] * (double)b0[i];
+    if constexpr (NV > 1) v[1] += (double)a1[i] * (double)b1[i];
+  }
+  reduce_and_finalize<NV>(v, s, op, partials, counter);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_first(int n, const T *__restrict__ dinv, T omega,
+                                                         const T *__restrict__ b, T *__restrict__ x, Scal *s,
+                                                         int check_done) {
+  if (check_done && s->done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = omega * dinv[i] * b[i];
+}
+
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int n, const T *__restrict__ p, const T *__restrict__ q,
+                                                      T *__restrict__ x, T *__restrict__ r,
+                                                      const T *__restrict__ jdinv, T *__restrict__ z, Scal *s,
+                                                      double *partials, unsigned *counter) {
+  if (s->done) return;
+  const T alpha = (T)s->alpha;
+  double v[JAC ? 2 : 1];
+  v[0] = 0.0;
+  if constexpr (JAC) v[1] = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T ri = r[i] - alpha * q[i];
+    x[i] += alpha * p[i];
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+    if constexpr (JAC) {
+      T zi = jdinv[i] * ri;
+      z[i] = zi;
+      v[1] += (double)ri * (double)zi;
+    }
+  }
+  reduce_and_finalize<JAC ? 2 : 1>(v, s, JAC ? OP_CG_RR_RZ : OP_CG_RR, partials, counter);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction(int n, const T *__restrict__ z, T *__restrict__ p,
+                                                         Scal *s) {
+  if (s->done) return;
+  const T beta = (T)s->beta;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i] + beta * p[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy(int n, const T *__restrict__ src, T *__restrict__ dst, Scal *s,
+                                                 int check_done) {
+  if (check_done && s->done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = src[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(int n, T value, T *__restrict__ dst) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = value;
+}
+
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32_scaled(int n, const double *__restrict__ src, double scale,
+                                                              float *__restrict__ dst) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = (float)(src[i] * scale);
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy_f32_to_f64(int n, double a, const float *__restrict__ x,
+                                                            double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += a * (double)x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double *__restrict__ a,
+                                                       const T *__restrict__ b, T *__restrict__ x, Scal *s,
+                                                       int check_done) {
+  if (check_done && s->done) return;
+  const int lane = threadIdx.x & 63;
+  const int row = (int)((blockIdx.x * (unsigned)kBlock + threadIdx.x) >> 6);
+  if (row >= n) return;  // whole wave leaves together
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) acc += a[(size_t)row * n + j] * (double)b[j];
+  acc = wave_sum(acc);
+  if (lane == 0) x[row] = (T)acc;
+}
+
+// BiCGSTAB: p = r + beta (p - omega v)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int n, const T *__restrict__ r, const T *__restrict__ v,
+                                                   T *__restrict__ p, Scal *s) {
+  if (s->done) return;
+  const T beta = (T)s->beta, omega = (T)s->omega;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// BiCGSTAB: s = r - alpha v, ||s||^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int n, const T *__restrict__ r, const T *__restrict__ v,
+                                                   T *__restrict__ sv, Scal *s, double *partials,
+                                                   unsigned *counter) {
+  if (s->done) return;
+  const T alpha = (T)s->alpha;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    T si = r[i] - alpha * v[i];
+    sv[i] = si;
+    acc[0] += (double)si * (double)si;
+  }
+  reduce_and_finalize<1>(acc, s, OP_BI_SNORM, partials, counter);
+}
+
+// BiCGSTAB: x += alpha ph + omega sh; r = s - omega t; ||r||^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(int n, const T *__restrict__ ph, const T *__restrict__ sh,
+                                                    const T *__restrict__ sv, const T *__restrict__ t,
+                                                    T *__restrict__ x, T *__restrict__ r, Scal *s,
+                                                    double *partials, unsigned *counter) {
+  if (s->done) return;
+  const T alpha = (T)s->alpha, omega = (T)s->omega;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] += alpha * ph[i] + omega * sh[i];
+    T ri = sv[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+  }
+  reduce_and_finalize<1>(acc, s, OP_BI_RR, partials, counter);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy_alpha(int n, const T *__restrict__ x, T *__restrict__ y,
+                                                       Scal *s) {
+  const T a = (T)s->alpha;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += a * x[i];
+}
+
+__global__ void k_scal_reset(Scal *s, double thr2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int i = 0; i < 8; ++i) s->red[i] = 0.0;
+    s->rz = s->beta = s->rr = s->pq = s->hn = 0.0;
+    s->rho = s->omega = s->alpha = 1.0;
+    s->thr2 = thr2;
+    s->done = 0;
+    s->iters = 0;
+    s->kstep = 0;
+    s->term = CFDLA_TERM_MAX_ITERATIONS;
+  }
+}
+
+__global__ void k_scal_set_done(Scal *s, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->done = value;
+}
+
+__global__ void k_scal_bicg_restart(Scal *s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    s->rho = s->alpha = s->omega = 1.0;
+    s->done = 0;
+  }
+}
+
+// dJ/dg for the 2D variable-coefficient operator (one thread per cell).
+__global__ __launch_bounds__(kBlock) void k_coef_grad_2d(int nx, int ny, double ih2, int dir_mask,
+                                                         double dv0, double dv1, double dv2, double dv3,
+                                                         const double *__restrict__ k,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ lam,
+                                                         const double *__restrict__ dJdk,
+                                                         double *__restrict__ dJdg) {
+  const int n = nx * ny;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) {
+    const int x = c % nx, y = c / nx;
+    const double kp = k[c], pp = p[c], lp = lam[c];
+    double g = dJdk ? dJdk[c] : 0.0;
+    auto face = [&](int q) {
+      const double kq = k[q], s = kp + kq;
+      const double dkf = 2.0 * kq * kq / (s * s);
+      g -= ih2 * dkf * (lp - lam[q]) * (pp - p[q]);
+    };
+    if (x > 0) face(c - 1);
+    if (x + 1 < nx) face(c + 1);
+    if (y > 0) face(c - nx);
+    if (y + 1 < ny) face(c + nx);
+    if (x == 0 && (dir_mask & 1)) g -= ih2 * lp * (pp - dv0);
+    if (x == nx - 1 && (dir_mask & 2)) g -= ih2 * lp * (pp - dv1);
+    if (y == 0 && (dir_mask & 4)) g -= ih2 * lp * (pp - dv2);
+    if (y == ny - 1 && (dir_mask & 8)) g -= ih2 * lp * (pp - dv3);
+    dJdg[c] = kp * g;
+  }
+}
+
+}  // namespace
+
+// ---- launchers -----------------------------------------------------------------------
+template <typename T>
+void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, double omega, T *y, Scal *s,
+               bool check_done, int dot_op, const T *dotw, RedWs &ws) {
+  const bool dot = dot_op != OP_NONE;
+  if (!dotw) dotw = x;
+#define CFDLA_EPI_CASE(E)                                                                         \
+  case E:                                                                                         \
+    if (dot) launch_csr_e<T, E, true>(A, x, b, dinv, omega, y, dotw, s, check_done, dot_op, ws);   \
+    else launch_csr_e<T, E, false>(A, x, b, dinv, omega, y, dotw, s, check_done, dot_op, ws);     \
+    break;
+  switch (epi) {
+    CFDLA_EPI_CASE(EPI_Y)
+    CFDLA_EPI_CASE(EPI_RESID)
+    CFDLA_EPI_CASE(EPI_JACOBI)
+    CFDLA_EPI_CASE(EPI_ADD)
+    default:
+      throw Error(CFDLA_ERR_INVALID_ARG, "unknown CSR epilogue");
+  }
+#undef CFDLA_EPI_CASE
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void dot2(int n, const T *a0, const T *b0, const T *a1, const T *b1, Scal *s, bool check_done, int op,
+          RedWs &ws) {
+  int blocks = blocks_for(n, kBlock * 4);
+  if (a1)
+    hipLaunchKernelGGL((k_dot<T, 2>), dim3(blocks), dim3(kBlock), 0, stream(), n, a0, b0, a1, b1, s,
+                       check_done ? 1 : 0, op, ws.partials.p, ws.counter.p);
+  else
+    hipLaunchKernelGGL((k_dot<T, 1>), dim3(blocks), dim3(kBlock), 0, stream(), n, a0, b0, a1, b1, s,
+                       check_done ? 1 : 0, op, ws.partials.p, ws.counter.p);
+  finish(a1 ? 2 : 1, s, op, blocks, ws, check_done);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void jacobi_first(int n, const T *dinv, double omega, const T *b, T *x, Scal *s, bool check_done) {
+  hipLaunchKernelGGL((k_jacobi_first<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, dinv,
+                     (T)omega, b, x, s, check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cg_update(int n, const T *p, const T *q, T *x, T *r, const T *jdinv, T *z, Scal *s, RedWs &ws) {
+  int blocks = blocks_for(n, kBlock * 4);
+  if (jdinv)
+    hipLaunchKernelGGL((k_cg_update<T, true>), dim3(blocks), dim3(kBlock), 0, stream(), n, p, q, x, r, jdinv, z,
+                       s, ws.partials.p, ws.counter.p);
+  else
+    hipLaunchKernelGGL((k_cg_update<T, false>), dim3(blocks), dim3(kBlock), 0, stream(), n, p, q, x, r, jdinv,
+                       z, s, ws.partials.p, ws.counter.p);
+  finish(jdinv ? 2 : 1, s, jdinv ? OP_CG_RR_RZ : OP_CG_RR, blocks, ws, true);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cg_direction(int n, const T *z, T *p, Scal *s) {
+  hipLaunchKernelGGL((k_cg_direction<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, z, p, s);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void copy_vec(int n, const T *src, T *dst, Scal *s, bool check_done) {
+  hipLaunchKernelGGL((k_copy<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, src, dst, s,
+                     check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fill_vec(int n, T value, T *dst) {
+  hipLaunchKernelGGL((k_fill<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, value, dst);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void f64_to_f32_scaled(int n, const double *src, double scale, float *dst) {
+  hipLaunchKernelGGL(k_f64_to_f32_scaled, dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, src,
+                     scale, dst);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void axpy_f32_to_f64(int n, double a, const float *x, double *y) {
+  hipLaunchKernelGGL(k_axpy_f32_to_f64, dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, a, x, y);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done) {
+  int blocks = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((k_dense_gemv<T>), dim3(blocks), dim3(kBlock), 0, stream(), n, ainv, b, x, s,
+                     check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bicg_p_update(int n, const T *r, const T *v, T *p, Scal *s) {
+  hipLaunchKernelGGL((k_bicg_p<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, r, v, p, s);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bicg_s_update(int n, const T *r, const T *v, T *sv, Scal *s, RedWs &ws) {
+  const int blocks = blocks_for(n, kBlock * 4);
+  hipLaunchKernelGGL((k_bicg_s<T>), dim3(blocks), dim3(kBlock), 0, stream(), n, r, v, sv, s, ws.partials.p,
+                     ws.counter.p);
+  finish(1, s, OP_BI_SNORM, blocks, ws, true);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s, RedWs &ws) {
+  const int blocks = blocks_for(n, kBlock * 4);
+  hipLaunchKernelGGL((k_bicg_xr<T>), dim3(blocks), dim3(kBlock), 0, stream(), n, ph, sh, sv, t, x, r, s,
+                     ws.partials.p, ws.counter.p);
+  finish(1, s, OP_BI_RR, blocks, ws, true);
+  CFDLA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void axpy_scal(int n, const T *x, T *y, Scal *s, int /*which*/) {
+  hipLaunchKernelGGL((k_axpy_alpha<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, x, y, s);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void scal_reset(Scal *s, double thr2) {
+  hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, stream(), s, thr2);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void scal_set_done(Scal *s, int value) {
+  hipLaunchKernelGGL(k_scal_set_done, dim3(1), dim3(64), 0, stream(), s, value);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void scal_bicg_restart(Scal *s) {
+  hipLaunchKernelGGL(k_scal_bicg_restart, dim3(1), dim3(64), 0, stream(), s);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, const double *dir_val,
+                         const double *k, const double *p, const double *lam, const double *dJdk,
+                         double *dJdg) {
+  hipLaunchKernelGGL(k_coef_grad_2d, dim3(blocks_for(nx * ny, kBlock * 4)), dim3(kBlock), 0, stream(), (int)nx,
+                     (int)ny, inv_h2, dir_mask, dir_val[0], dir_val[1], dir_val[2], dir_val[3], k, p, lam, dJdk,
+                     dJdg);
+  CFDLA_HIP(hipGetLastError());
+}
+
+#define CFDLA_INSTANTIATE(T)                                                                                   \
+  template void csr_apply<T>(const DevCsr &, int, const T *, const T *, const T *, double, T *, Scal *, bool, \
+                             int, const T *, RedWs &);                                                        \
+  template void dot2<T>(int, const T *, const T *, const T *, const T *, Scal *, bool, int, RedWs &);          \
+  template void jacobi_first<T>(int, const T *, double, const T *, T *, Scal *, bool);                        \
+  template void cg_update<T>(int, const T *, const T *, T *, T *, const T *, T *, Scal *, RedWs &);            \
+  template void cg_direction<T>(int, const T *, T *, Scal *);                                                  \
+  template void copy_vec<T>(int, const T *, T *, Scal *, bool);                                                \
+  template void fill_vec<T>(int, T, T *);                                                                      \
+  template void dense_gemv<T>(int, const double *, const T *, T *, Scal *, bool);                             \
+  template void bicg_p_update<T>(int, const T *, const T *, T *, Scal *);                                      \
+  template void bicg_s_update<T>(int, const T *, const T *, T *, Scal *, RedWs &);                             \
+  template void bicg_x_r_update<T>(int, const T *, const T *, const T *, const T *, T *, T *, Scal *, RedWs &); \
+  template void axpy_scal<T>(int, const T *, T *, Scal *, int);
+
+CFDLA_INSTANTIATE(double)
+CFDLA_INSTANTIATE(float)
+
+}  // namespace cfdla

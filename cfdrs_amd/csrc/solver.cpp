@@ -1,0 +1,593 @@
+// Device runtime, preconditioners and Krylov drivers of libcfdla.
+//
+// The Krylov recurrences are driven from the host but keep every scalar on
+// the device: reductions finish in the last block of the reducing kernel,
+// which updates alpha/beta/rho/omega and the convergence flag in a Scal
+// record.  The host only enqueues kernels and polls the flag one step
+// behind (a pinned copy of Scal plus an event per step), so the GPU never
+// waits for the host, and kernels of steps enqueued after convergence exit
+// immediately.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+namespace cfdla {
+
+// ---------------------------------------------------------------------------
+// Device state
+// ---------------------------------------------------------------------------
+namespace {
+struct DeviceState {
+  bool ready = false;
+  int device = 0;
+  int cus = 0;
+  hipStream_t own = nullptr;
+  hipStream_t current = nullptr;
+  std::unique_ptr<Context> context;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  Scal *poll = nullptr;  // pinned, 2 slots
+};
+DeviceState &dstate() {
+  static DeviceState s;
+  return s;
+}
+}  // namespace
+
+void init_device(int dev) {
+  DeviceState &d = dstate();
+  if (d.ready) {
+    CFDLA_REQUIRE(dev == d.device, CFDLA_ERR_INVALID_ARG, "library already initialised on another device");
+    return;
+  }
+  int count = 0;
+  CFDLA_HIP(hipGetDeviceCount(&count));
+  CFDLA_REQUIRE(count > 0, CFDLA_ERR_DEVICE, "no HIP device visible");
+  CFDLA_REQUIRE(dev >= 0 && dev < count, CFDLA_ERR_INVALID_ARG, "device index out of range");
+  CFDLA_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  CFDLA_HIP(hipGetDeviceProperties(&prop, dev));
+  d.cus = prop.multiProcessorCount;
+  CFDLA_HIP(hipStreamCreateWithFlags(&d.own, hipStreamNonBlocking));
+  d.current = d.own;
+  d.device = dev;
+  CFDLA_HIP(hipEventCreateWithFlags(&d.ev[0], hipEventDisableTiming));
+  CFDLA_HIP(hipEventCreateWithFlags(&d.ev[1], hipEventDisableTiming));
+  CFDLA_HIP(hipHostMalloc(reinterpret_cast<void **>(&d.poll), 2 * sizeof(Scal), hipHostMallocDefault));
+  d.ready = true;
+  d.context = std::make_unique<Context>();
+  d.context->ws.init();
+  d.context->scal.alloc(4);
+  CFDLA_HIP(hipHostMalloc(reinterpret_cast<void **>(&d.context->host), sizeof(Scal), hipHostMallocDefault));
+}
+
+void ensure_init() {
+  if (!dstate().ready) init_device(0);
+}
+
+hipStream_t stream() {
+  ensure_init();
+  return dstate().current;
+}
+
+void set_stream(hipStream_t s) {
+  ensure_init();
+  dstate().current = (s == reinterpret_cast<hipStream_t>(CFDLA_LIBRARY_STREAM)) ? dstate().own : s;
+}
+
+int num_cus() {
+  ensure_init();
+  return dstate().cus;
+}
+
+void sync_stream() { CFDLA_HIP(hipStreamSynchronize(stream())); }
+
+Context &ctx() {
+  ensure_init();
+  return *dstate().context;
+}
+
+Scal Context::read(Scal *dev) {
+  CFDLA_HIP(hipMemcpyAsync(host, dev, sizeof(Scal), hipMemcpyDeviceToHost, stream()));
+  CFDLA_HIP(hipStreamSynchronize(stream()));
+  return *host;
+}
+
+void RedWs::init() {
+  partials.alloc((size_t)kMaxRed * kMaxBlocks);
+  counter.alloc(1);
+  CFDLA_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned), stream()));
+  CFDLA_HIP(hipStreamSynchronize(stream()));
+}
+
+// Lagged poll: after each enqueued step, copy Scal to a pinned slot and
+// record an event; check the previous step's slot.
+struct Poller {
+  int k = 0;
+  Scal *dev;
+  explicit Poller(Scal *s) : dev(s) {}
+  void post() {
+    DeviceState &d = dstate();
+    CFDLA_HIP(hipMemcpyAsync(d.poll + (k & 1), dev, sizeof(Scal), hipMemcpyDeviceToHost, stream()));
+    CFDLA_HIP(hipEventRecord(d.ev[k & 1], stream()));
+    ++k;
+  }
+  // State as of the step before the last one posted (nullptr if none).
+  const Scal *lagged() {
+    if (k < 2) return nullptr;
+    DeviceState &d = dstate();
+    CFDLA_HIP(hipEventSynchronize(d.ev[(k - 2) & 1]));
+    return d.poll + ((k - 2) & 1);
+  }
+  const Scal *latest() {
+    DeviceState &d = dstate();
+    CFDLA_HIP(hipEventSynchronize(d.ev[(k - 1) & 1]));
+    return d.poll + ((k - 1) & 1);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Device CSR
+// ---------------------------------------------------------------------------
+int pick_group(int64_t n, int64_t nnz) {
+  double avg = n > 0 ? (double)nnz / (double)n : 1.0;
+  int g = 1;
+  while (g < avg && g < 64) g *= 2;
+  return g;
+}
+
+namespace {
+void smoother_dinv(const HostCsr &h, std::vector<double> &d) {
+  d.assign((size_t)h.n, 0.0);
+  for (int64_t i = 0; i < h.n; ++i)
+    for (int64_t k = h.rp[i]; k < h.rp[i + 1]; ++k)
+      if (h.ci[k] == i) {
+        double a = h.v[k];
+        d[i] = std::fabs(a) > 1e-15 ? 1.0 / a : 0.0;
+      }
+}
+}  // namespace
+
+void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv) {
+  CFDLA_REQUIRE(h.n < INT32_MAX && h.m < INT32_MAX && h.nnz() < INT32_MAX, CFDLA_ERR_INVALID_ARG,
+                "matrix too large for 32-bit device indices");
+  n = (int)h.n;
+  m = (int)h.m;
+  nnz = h.nnz();
+  group = pick_group(h.n, nnz);
+  std::vector<int32_t> rp32((size_t)h.n + 1);
+  for (int64_t i = 0; i <= h.n; ++i) rp32[i] = (int32_t)h.rp[i];
+  rp.upload(rp32.data(), rp32.size());
+  ci.upload(h.ci.data(), h.ci.size());
+  v64.upload(h.v.data(), h.v.size());
+  std::vector<float> f;
+  if (want_fp32) {
+    f.resize(h.v.size());
+    for (size_t k = 0; k < f.size(); ++k) f[k] = (float)h.v[k];
+    v32.upload(f.data(), f.size());
+  }
+  std::vector<double> d;
+  std::vector<float> df;
+  if (want_dinv) {
+    smoother_dinv(h, d);
+    sdinv64.upload(d.data(), d.size());
+    df.resize(d.size());
+    for (size_t k = 0; k < d.size(); ++k) df[k] = (float)d[k];
+    sdinv32.upload(df.data(), df.size());
+  }
+  sync_stream();
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioners
+// ---------------------------------------------------------------------------
+JacobiPrecond::JacobiPrecond(const HostCsr &a) {
+  n = (int)a.n;
+  std::vector<double> inv((size_t)a.n, 1.0);
+  for (int64_t i = 0; i < a.n; ++i) {
+    double d = 0.0;
+    for (int64_t k = a.rp[i]; k < a.rp[i + 1]; ++k)
+      if (a.ci[k] == i) d += a.v[k];
+    CFDLA_REQUIRE(std::isfinite(d), CFDLA_ERR_NUMERICAL, "Jacobi preconditioner: non-finite diagonal");
+    inv[i] = std::fabs(d) <= 2.220446049250313e-16 ? 1.0 : 1.0 / d;
+  }
+  std::vector<float> f(inv.size());
+  for (size_t i = 0; i < inv.size(); ++i) f[i] = (float)inv[i];
+  d64.upload(inv.data(), inv.size());
+  d32.upload(f.data(), f.size());
+  sync_stream();
+}
+
+// z = dinv .* r via the fused CG kernel is the hot path; the standalone apply
+// uses a 1x1 "Jacobi sweep" from zero: z = 1 * dinv .* r.
+void JacobiPrecond::apply(const double *r, double *z, Scal *s, bool chk) { jacobi_first<double>(n, d64.p, 1.0, r, z, s, chk); }
+void JacobiPrecond::apply(const float *r, float *z, Scal *s, bool chk) { jacobi_first<float>(n, d32.p, 1.0, r, z, s, chk); }
+
+struct IdentityPrecond : Precond {
+  int n;
+  explicit IdentityPrecond(int n_) : n(n_) {}
+  int kind() const override { return 0; }
+  void apply(const double *r, double *z, Scal *s, bool chk) override { copy_vec<double>(n, r, z, s, chk); }
+  void apply(const float *r, float *z, Scal *s, bool chk) override { copy_vec<float>(n, r, z, s, chk); }
+};
+
+template <> double *DevLevel::b<double>() { return b64.p; }
+template <> float *DevLevel::b<float>() { return b32.p; }
+template <> double *DevLevel::x<double>() { return x64.p; }
+template <> float *DevLevel::x<float>() { return x32.p; }
+template <> double *DevLevel::t<double>() { return t64.p; }
+template <> float *DevLevel::t<float>() { return t32.p; }
+template <> double *DevLevel::r<double>() { return r64.p; }
+template <> float *DevLevel::r<float>() { return r32.p; }
+
+AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
+  CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "AMG requires a square matrix");
+  CFDLA_REQUIRE(c.max_levels >= 1, CFDLA_ERR_CONFIG, "AMG max_levels must be >= 1");
+  CFDLA_REQUIRE(c.pre_smooth_iterations >= 0 && c.post_smooth_iterations >= 0, CFDLA_ERR_CONFIG,
+                "AMG smoothing counts must be >= 0");
+  CFDLA_REQUIRE(c.coarsening >= 0 && c.coarsening <= 3, CFDLA_ERR_CONFIG, "AMG coarsening must be 0, 1, 2 or 3");
+  CFDLA_REQUIRE(c.interpolation >= 0 && c.interpolation <= 2, CFDLA_ERR_CONFIG,
+                "AMG interpolation must be 0, 1 or 2");
+  CFDLA_REQUIRE(c.smoother == 2, CFDLA_ERR_CONFIG,
+                "the device V-cycle smooths with Jacobi only (smoother = 2); Gauss-Seidel sweeps are sequential");
+  CFDLA_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, CFDLA_ERR_CONFIG, "AMG coarse_solver must be 0 or 1");
+  auto t0 = std::chrono::steady_clock::now();
+  build_hierarchy(a, cfg, host);
+  setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void AmgPrecond::upload() {
+  const size_t L = host.levels.size();
+  lv.clear();
+  lv.resize(L);
+  for (size_t l = 0; l < L; ++l) {
+    HostLevel &h = host.levels[l];
+    DevLevel &d = lv[l];
+    d.A.upload(h.A, true, true);
+    if (l + 1 < L) {
+      d.P.upload(h.P, true, false);
+      d.R.upload(h.R, true, false);
+    }
+    size_t n = (size_t)h.A.n;
+    if (l > 0) {
+      d.b64.alloc(n); d.x64.alloc(n); d.b32.alloc(n); d.x32.alloc(n);
+    }
+    d.t64.alloc(n); d.r64.alloc(n); d.t32.alloc(n); d.r32.alloc(n);
+  }
+  coarse_n = (int)host.levels.back().A.n;
+  if (cfg.coarse_solver == 1) {
+    CFDLA_REQUIRE(coarse_n <= 8192, CFDLA_ERR_CONFIG,
+                  "AMG coarsest level too large for the dense direct solve; raise max_levels");
+    std::vector<double> inv = dense_inverse(host.levels.back().A);
+    coarse_inv.upload(inv.data(), inv.size());
+  }
+  sync_stream();
+  on_device = true;
+}
+
+void AmgPrecond::update_values(const double *values) {
+  recompute_hierarchy(host, values);
+  if (on_device) upload();
+}
+
+template <typename T>
+void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk) {
+  DevLevel &L = lv[(size_t)l];
+  RedWs &ws = ctx().ws;
+  const int n = L.A.n;
+  const bool coarsest = (size_t)l + 1 == lv.size();
+  const double w = cfg.relaxation_factor;
+  const T *dinv = L.A.sdinv<T>();
+  if (coarsest && cfg.coarse_solver == 1) {
+    dense_gemv<T>(n, coarse_inv.p, b, x, s, chk);
+    return;
+  }
+  // amg.rs:423-503 with Jacobi: pre-smoothing from the zero guess, then on
+  // the coarsest level 10 more sweeps (amg.rs:501).
+  const int down = coarsest ? cfg.pre_smooth_iterations + 10 : cfg.pre_smooth_iterations;
+  const int up = coarsest ? 0 : cfg.post_smooth_iterations;
+  T *buf[2] = {x, L.t<T>()};
+  // number of buffer flips after the first sweep; the iterate must end in x
+  const int flips = (down > 0 ? down - 1 : 0) + up;
+  int cur = (flips % 2 == 0) ? 0 : 1;
+  if (down > 0) jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
+  else fill_vec<T>(n, T(0), buf[cur]);
+  for (int k = 1; k < down; ++k) {
+    csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
+    cur ^= 1;
+  }
+  if (!coarsest) {
+    DevLevel &C = lv[(size_t)l + 1];
+    csr_apply<T>(L.A, EPI_RESID, buf[cur], b, nullptr, 0.0, L.r<T>(), s, chk, OP_NONE, nullptr, ws);
+    csr_apply<T>(L.R, EPI_Y, L.r<T>(), nullptr, nullptr, 0.0, C.b<T>(), s, chk, OP_NONE, nullptr, ws);
+    vcycle<T>(l + 1, C.b<T>(), C.x<T>(), s, chk);
+    csr_apply<T>(L.P, EPI_ADD, C.x<T>(), nullptr, nullptr, 0.0, buf[cur], s, chk, OP_NONE, nullptr, ws);
+    for (int k = 0; k < up; ++k) {
+      csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
+      cur ^= 1;
+    }
+  }
+  if (cur != 0) copy_vec<T>(n, buf[cur], x, s, chk);
+}
+
+template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool);
+template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool);
+
+// ---------------------------------------------------------------------------
+// Krylov drivers
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T> T *vec(int slot, size_t n);
+template <> double *vec<double>(int slot, size_t n) { return ctx().dv_at(slot, n); }
+template <> float *vec<float>(int slot, size_t n) { return ctx().fv_at(slot, n); }
+
+template <typename T> const T *jdinv(const Precond *M);
+template <> const double *jdinv<double>(const Precond *M) { return M->jacobi_dinv64(); }
+template <> const float *jdinv<float>(const Precond *M) { return M->jacobi_dinv32(); }
+
+int auto_chunk(const Precond *M, int n) {
+  if (M->kind() == 2) return 1;
+  return n > 1000000 ? 2 : 8;
+}
+
+// r = b - A x, returns ||r||_2 (synchronous).
+template <typename T>
+double true_residual(const DevCsr &A, const T *b, const T *x, T *r, Scal *s) {
+  csr_apply<T>(A, EPI_RESID, x, b, nullptr, 0.0, r, s, false, OP_INIT_RR, r, ctx().ws);
+  Scal h = ctx().read(s);
+  return std::sqrt(h.rr);
+}
+
+// Preconditioned CG (Saad 2003, Alg. 9.1).  confirm: check the explicit
+// residual before reporting CONVERGED and restart from it otherwise.
+template <typename T>
+SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t max_it, int chunk, bool confirm) {
+  Context &c = ctx();
+  RedWs &ws = c.ws;
+  const int n = A.n;
+  T *r = vec<T>(0, n), *z = vec<T>(1, n), *p = vec<T>(2, n), *q = vec<T>(3, n);
+  Scal *s = c.s(0);
+  SolveOut o;
+  o.thr = thr;
+  scal_reset(s, thr * thr);
+  o.rn0 = o.rn = true_residual<T>(A, b, x, r, s);
+  if (o.rn0 <= thr) {
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
+  const T *jd = jdinv<T>(M);
+  auto start_direction = [&]() {
+    M->apply(r, z, s, false);
+    dot2<T>(n, r, z, nullptr, nullptr, s, false, OP_SET_RZ, ws);
+    copy_vec<T>(n, z, p, s, false);
+  };
+  start_direction();
+  if (chunk <= 0) chunk = auto_chunk(M, n);
+  int64_t enq = 0;
+  Poller poll(s);
+  bool finished = false;
+  while (!finished) {
+    const Scal *h = nullptr;
+    if (enq < max_it) {
+      int64_t steps = std::min<int64_t>(chunk, max_it - enq);
+      for (int64_t k = 0; k < steps; ++k) {
+        csr_apply<T>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        if (jd) {
+          cg_update<T>(n, p, q, x, r, jd, z, s, ws);
+        } else {
+          cg_update<T>(n, p, q, x, r, nullptr, nullptr, s, ws);
+          M->apply(r, z, s, true);
+          dot2<T>(n, r, z, nullptr, nullptr, s, true, OP_CG_BETA, ws);
+        }
+        cg_direction<T>(n, z, p, s);
+      }
+      enq += steps;
+      poll.post();
+      h = poll.lagged();
+    } else {
+      h = poll.latest();
+    }
+    if (!h) continue;
+    if (h->done == 0 && enq < max_it) continue;
+    sync_stream();
+    Scal now = c.read(s);
+    o.iters = now.iters;
+    if (now.done == 2) {
+      o.term = now.term;
+      finished = true;
+    } else if (now.done == 1) {
+      if (!confirm) {
+        o.term = CFDLA_TERM_CONVERGED;
+        o.rn = std::sqrt(now.rr);
+        return o;
+      }
+      double tn = true_residual<T>(A, b, x, r, s);
+      if (tn <= thr) {
+        o.term = CFDLA_TERM_CONVERGED;
+        o.rn = tn;
+        return o;
+      }
+      scal_set_done(s, 0);  // residual replacement: restart from the explicit residual
+      start_direction();
+      poll = Poller(s);
+    } else {
+      finished = true;  // budget exhausted
+    }
+  }
+  o.rn = true_residual<T>(A, b, x, r, s);
+  return o;
+}
+
+// Right-preconditioned BiCGSTAB (Saad 2003, Alg. 7.7).
+template <typename T>
+SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t max_it, int chunk,
+                  bool confirm) {
+  Context &c = ctx();
+  RedWs &ws = c.ws;
+  const int n = A.n;
+  T *r = vec<T>(0, n), *rhat = vec<T>(1, n), *p = vec<T>(2, n), *v = vec<T>(3, n);
+  T *ph = vec<T>(4, n), *sv = vec<T>(5, n), *sh = vec<T>(6, n), *t = vec<T>(7, n);
+  Scal *s = c.s(0);
+  SolveOut o;
+  o.thr = thr;
+  scal_reset(s, thr * thr);
+  o.rn0 = o.rn = true_residual<T>(A, b, x, r, s);
+  if (o.rn0 <= thr) {
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
+  auto restart = [&]() {
+    copy_vec<T>(n, r, rhat, s, false);
+    fill_vec<T>(n, T(0), p);
+    fill_vec<T>(n, T(0), v);
+    scal_bicg_restart(s);
+  };
+  restart();
+  if (chunk <= 0) chunk = auto_chunk(M, n);
+  int64_t enq = 0;
+  Poller poll(s);
+  bool finished = false;
+  while (!finished) {
+    const Scal *h = nullptr;
+    if (enq < max_it) {
+      int64_t steps = std::min<int64_t>(chunk, max_it - enq);
+      for (int64_t k = 0; k < steps; ++k) {
+        dot2<T>(n, rhat, r, nullptr, nullptr, s, true, OP_BI_RHO, ws);
+        bicg_p_update<T>(n, r, v, p, s);
+        M->apply(p, ph, s, true);
+        csr_apply<T>(A, EPI_Y, ph, nullptr, nullptr, 0.0, v, s, true, OP_BI_ALPHA, rhat, ws);
+        bicg_s_update<T>(n, r, v, sv, s, ws);
+        M->apply(sv, sh, s, true);
+        csr_apply<T>(A, EPI_Y, sh, nullptr, nullptr, 0.0, t, s, true, OP_NONE, nullptr, ws);
+        dot2<T>(n, t, sv, t, t, s, true, OP_BI_OMEGA, ws);
+        bicg_x_r_update<T>(n, ph, sh, sv, t, x, r, s, ws);
+      }
+      enq += steps;
+      poll.post();
+      h = poll.lagged();
+    } else {
+      h = poll.latest();
+    }
+    if (!h) continue;
+    if (h->done == 0 && enq < max_it) continue;
+    sync_stream();
+    Scal now = c.read(s);
+    o.iters = now.iters;
+    if (now.done == 2) {
+      o.term = now.term;
+      finished = true;
+    } else if (now.done == 1 || now.done == 3) {
+      if (now.done == 3) axpy_scal<T>(n, ph, x, s, 0);  // x += alpha * phat
+      double tn = true_residual<T>(A, b, x, r, s);
+      if (!confirm || tn <= thr) {
+        o.term = CFDLA_TERM_CONVERGED;
+        o.rn = tn;
+        return o;
+      }
+      scal_set_done(s, 0);
+      restart();
+      poll = Poller(s);
+    } else {
+      finished = true;
+    }
+  }
+  o.rn = true_residual<T>(A, b, x, r, s);
+  return o;
+}
+
+SolveOut run_fp64(const DevCsr &A, Precond *M, const double *b, double *x, double thr, const cfdla_solver_config &cfg) {
+  switch (cfg.kind) {
+    case CFDLA_CG: return pcg<double>(A, M, b, x, thr, cfg.max_iter, cfg.check_every, true);
+    case CFDLA_BICGSTAB: return bicgstab<double>(A, M, b, x, thr, cfg.max_iter, cfg.check_every, true);
+    default: return gmres_fp64(A, M, b, x, thr, cfg);
+  }
+}
+
+// FP32-inner / FP64-refined defect correction.
+SolveOut run_refined(const DevCsr &A, Precond *M, const double *b, double *x, double thr,
+                     const cfdla_solver_config &cfg) {
+  CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
+  Context &c = ctx();
+  const int n = A.n;
+  double *r64 = c.dv_at(8, n);
+  float *r32 = c.fv_at(8, n), *d32 = c.fv_at(9, n);
+  Scal *so = c.s(1);
+  SolveOut o;
+  o.thr = thr;
+  scal_reset(so, thr * thr);
+  double rn = true_residual<double>(A, b, x, r64, so);
+  o.rn0 = o.rn = rn;
+  if (rn <= thr) {
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
+  const double inner_tol = cfg.inner_tol > 0.0 ? cfg.inner_tol : 1e-5;
+  const int max_outer = cfg.max_outer > 0 ? cfg.max_outer : 30;
+  int stalls = 0;
+  for (int outer = 1; outer <= max_outer; ++outer) {
+    int64_t budget = cfg.max_iter - o.iters;
+    if (budget <= 0) break;
+    f64_to_f32_scaled(n, r64, 1.0 / rn, r32);
+    fill_vec<float>(n, 0.0f, d32);
+    SolveOut in;
+    if (cfg.kind == CFDLA_BICGSTAB) in = bicgstab<float>(A, M, r32, d32, inner_tol, budget, cfg.check_every, false);
+    else if (cfg.kind == CFDLA_CG) in = pcg<float>(A, M, r32, d32, inner_tol, budget, cfg.check_every, false);
+    else in = gmres_fp32_inner(A, M, r32, d32, inner_tol, budget, cfg);
+    o.iters += in.iters;
+    axpy_f32_to_f64(n, rn, d32, x);
+    double rn_new = true_residual<double>(A, b, x, r64, so);
+    o.outer = outer;
+    o.rn = rn_new;
+    if (rn_new <= thr) {
+      o.term = CFDLA_TERM_CONVERGED;
+      return o;
+    }
+    if (!(rn_new < 0.9 * rn)) {
+      if (++stalls >= 2 || in.iters == 0) {
+        o.term = (in.term == CFDLA_TERM_BREAKDOWN || in.term == CFDLA_TERM_NONPOSITIVE_CURVATURE)
+                     ? in.term
+                     : CFDLA_TERM_MAX_ITERATIONS;
+        return o;
+      }
+    } else {
+      stalls = 0;
+    }
+    rn = rn_new;
+  }
+  o.term = CFDLA_TERM_MAX_ITERATIONS;
+  return o;
+}
+
+}  // namespace
+
+SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, const cfdla_solver_config &cfg) {
+  CFDLA_REQUIRE(A.n == A.m, CFDLA_ERR_DIMENSION, "solve requires a square matrix");
+  CFDLA_REQUIRE(cfg.kind >= 0 && cfg.kind <= 2, CFDLA_ERR_CONFIG, "unknown solver kind");
+  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
+                "unknown precision");
+  CFDLA_REQUIRE(cfg.max_iter >= 1, CFDLA_ERR_CONFIG, "max_iter must be >= 1");
+  CFDLA_REQUIRE(std::isfinite(cfg.tol_abs) && std::isfinite(cfg.tol_rel) && cfg.tol_abs >= 0.0 && cfg.tol_rel >= 0.0,
+                CFDLA_ERR_CONFIG, "tolerances must be finite and non-negative");
+  std::unique_ptr<IdentityPrecond> ident;
+  if (!M) {
+    ident = std::make_unique<IdentityPrecond>(A.n);
+    M = ident.get();
+  }
+  // ||b|| for the relative threshold
+  Context &c = ctx();
+  Scal *s = c.s(2);
+  scal_reset(s, 0.0);
+  dot2<double>(A.n, b, b, nullptr, nullptr, s, false, OP_STORE, c.ws);
+  double bn = std::sqrt(c.read(s).red[0]);
+  CFDLA_REQUIRE(std::isfinite(bn), CFDLA_ERR_NUMERICAL, "right-hand side is not finite");
+  double thr = std::max(cfg.tol_abs, cfg.tol_rel * bn);
+  SolveOut o = cfg.precision == CFDLA_FP32_REFINED ? run_refined(A, M, b, x, thr, cfg) : run_fp64(A, M, b, x, thr, cfg);
+  sync_stream();
+  return o;
+}
+
+}  // namespace cfdla

@@ -1,0 +1,102 @@
+// Preconditioners and Krylov drivers of libcfdla (see solver.cpp).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "amg_setup.hpp"
+#include "internal.hpp"
+
+namespace cfdla {
+
+// Library context: reduction workspace, device scalars, pinned host mirror,
+// and pooled solver vectors.
+struct Context {
+  RedWs ws;
+  DevBuf<Scal> scal;  // 4 slots
+  Scal *host = nullptr;
+  DevBuf<double> dv[12];
+  DevBuf<float> fv[12];
+  double *dv_at(int i, size_t n) { dv[i].ensure(n); return dv[i].p; }
+  float *fv_at(int i, size_t n) { fv[i].ensure(n); return fv[i].p; }
+  Scal *s(int i) { return scal.p + i; }
+  Scal read(Scal *dev);  // synchronous copy of one slot
+};
+Context &ctx();
+
+struct Precond {
+  virtual ~Precond() = default;
+  virtual int kind() const = 0;  // 0 identity, 1 Jacobi, 2 AMG
+  virtual void apply(const double *r, double *z, Scal *s, bool check_done) = 0;
+  virtual void apply(const float *r, float *z, Scal *s, bool check_done) = 0;
+  virtual const double *jacobi_dinv64() const { return nullptr; }
+  virtual const float *jacobi_dinv32() const { return nullptr; }
+  virtual void ensure_fp32() {}
+};
+
+struct JacobiPrecond : Precond {
+  int n = 0;
+  DevBuf<double> d64;
+  DevBuf<float> d32;
+  explicit JacobiPrecond(const HostCsr &a);
+  int kind() const override { return 1; }
+  void apply(const double *r, double *z, Scal *s, bool check_done) override;
+  void apply(const float *r, float *z, Scal *s, bool check_done) override;
+  const double *jacobi_dinv64() const override { return d64.p; }
+  const float *jacobi_dinv32() const override { return d32.p; }
+};
+
+struct DevLevel {
+  DevCsr A, P, R;
+  DevBuf<double> b64, x64, t64, r64;
+  DevBuf<float> b32, x32, t32, r32;
+  template <typename T> T *b();
+  template <typename T> T *x();
+  template <typename T> T *t();
+  template <typename T> T *r();
+};
+
+struct AmgPrecond : Precond {
+  cfdla_amg_config cfg{};
+  HostHierarchy host;
+  std::vector<DevLevel> lv;
+  DevBuf<double> coarse_inv;
+  int coarse_n = 0;
+  double setup_time = 0.0;
+  bool on_device = false;
+  // Builds the host hierarchy only; device copies are made on first use.
+  AmgPrecond(const HostCsr &a, const cfdla_amg_config &c);
+  void upload();
+  void ensure_device() {
+    if (!on_device) upload();
+  }
+  void update_values(const double *values);
+  int kind() const override { return 2; }
+  void apply(const double *r, double *z, Scal *s, bool check_done) override {
+    ensure_device();
+    vcycle<double>(0, r, z, s, check_done);
+  }
+  void apply(const float *r, float *z, Scal *s, bool check_done) override {
+    ensure_device();
+    vcycle<float>(0, r, z, s, check_done);
+  }
+  template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done);
+};
+
+struct SolveOut {
+  int64_t iters = 0, outer = 0;
+  double rn0 = 0, rn = 0, thr = 0;
+  int term = CFDLA_TERM_MAX_ITERATIONS;
+};
+
+// GMRES(m) drivers (gmres.hip).
+SolveOut gmres_fp64(const DevCsr &A, Precond *M, const double *b, double *x, double thr,
+                    const cfdla_solver_config &cfg);
+SolveOut gmres_fp32_inner(const DevCsr &A, Precond *M, const float *b, float *x, double thr, int64_t max_it,
+                          const cfdla_solver_config &cfg);
+int restart_width(const cfdla_solver_config &cfg);
+
+// Solve on device FP64 vectors (b, x), dispatching on config kind/precision.
+SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, const cfdla_solver_config &cfg);
+
+}  // namespace cfdla

@@ -1,0 +1,81 @@
+"""Deterministic problem generators for the five workloads.
+
+All random inputs come from SplitMix64 streams in libcfdla
+(``cfdla_synth_uniform``), so the host, the GPU box and the tests see the
+same matrices and right-hand sides.  Matrix assembly runs in the library:
+
+* ``pressure_correction_2d``: cell-centred 5-point pressure-correction
+  Laplacian, all cells fluid, east wall Dirichlet
+  (crates/cfd-2d/src/pressure_velocity/correction.rs:273-343).
+* ``varcoef``: cell-centred variable-coefficient Poisson with harmonic-mean
+  face coefficients (2D or 3D).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _native as N
+
+SEED_C1 = 1001
+SEED_C2_K = 1002
+SEED_C2_B = 10002
+SEED_C5 = 1005
+
+
+def uniform(seed: int, n: int, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    out = np.empty(n)
+    N.call("cfdla_synth_uniform", C.c_uint64(seed), lo, hi, n, N.ptr(out))
+    return out
+
+
+def pressure_correction_2d(nxc: int, nyc: int, dx: float, dy: float, dirichlet=(0, 1, 0, 0)):
+    """(row_ptr, col, values) of the pressure-correction matrix."""
+    d = np.asarray(dirichlet, np.int32)
+    nnz = C.c_int64()
+    N.call("cfdla_problem_pressure_correction", nxc, nyc, dx, dy, N.ptr(d), C.byref(nnz), None, None, None)
+    rp = np.empty(nxc * nyc + 1, np.int64)
+    ci = np.empty(nnz.value, np.int64)
+    v = np.empty(nnz.value)
+    N.call("cfdla_problem_pressure_correction", nxc, nyc, dx, dy, N.ptr(d), C.byref(nnz), N.ptr(rp), N.ptr(ci),
+           N.ptr(v))
+    return rp, ci, v
+
+
+def varcoef(nx: int, ny: int, nz: int, inv_h2: float, k, dir_mask: int = 0, dir_val=None):
+    """(row_ptr, col, values, rhs_bc) of the variable-coefficient operator.
+    dir_mask bits: 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z."""
+    k = np.ascontiguousarray(k, np.float64)
+    dv = np.zeros(6) if dir_val is None else np.ascontiguousarray(dir_val, np.float64)
+    n = nx * ny * nz
+    nnz = C.c_int64()
+    N.call("cfdla_problem_varcoef", nx, ny, nz, inv_h2, N.ptr(k), dir_mask, N.ptr(dv), C.byref(nnz), None, None,
+           None, None)
+    rp = np.empty(n + 1, np.int64)
+    ci = np.empty(nnz.value, np.int64)
+    v = np.empty(nnz.value)
+    bc = np.empty(n)
+    N.call("cfdla_problem_varcoef", nx, ny, nz, inv_h2, N.ptr(k), dir_mask, N.ptr(dv), C.byref(nnz), N.ptr(rp),
+           N.ptr(ci), N.ptr(v), N.ptr(bc))
+    return rp, ci, v, bc
+
+
+def config1(n: int = 128):
+    """2D pressure-correction Poisson on n x n cells, h = 1/n, east Dirichlet; b ~ U[-1,1]."""
+    h = 1.0 / n
+    rp, ci, v = pressure_correction_2d(n, n, h, h, (0, 1, 0, 0))
+    return rp, ci, v, uniform(SEED_C1, n * n)
+
+
+def config2_k(n: int = 128):
+    """k = exp(g), g ~ U[-ln 10, ln 10] per cell."""
+    return np.exp(uniform(SEED_C2_K, n ** 3, -math.log(10.0), math.log(10.0)))
+
+
+def config2(n: int = 128):
+    """3D variable-coefficient Poisson on n^3 cells, Dirichlet on z = 0 (value 0); b ~ U[-1,1]."""
+    k = config2_k(n)
+    rp, ci, v, _ = varcoef(n, n, n, float(n * n), k, 1 << 4, np.zeros(6))
+    return rp, ci, v, uniform(SEED_C2_B, n ** 3)

@@ -1,0 +1,193 @@
+"""Device-resident CSR matrices.
+
+Mirrors the reference's ``SparseMatrix = leto_ops::CsrMatrix``
+(crates/cfd-math/src/sparse/mod.rs:53), built with
+``CsrMatrix::from_parts(values, col_indices, row_ptr, nrows, ncols)``, and
+``SparseMatrixBuilder`` (crates/cfd-math/src/sparse/builder.rs:70-386:
+duplicates summed in insertion order, rows sorted by column).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def _is_torch(a) -> bool:
+    return hasattr(a, "data_ptr") and hasattr(a, "is_cuda")
+
+
+def bind_torch_stream() -> None:
+    """Issue library work on torch's current HIP stream."""
+    import torch
+
+    N.call("cfdla_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+class CsrMatrix:
+    """CSR matrix copied to the GPU (int32 indices, FP64 values + FP32 copy).
+
+    The caller keeps its host arrays; ``row_ptr``/``col_indices``/``values``
+    return host copies in sorted-column order.
+    """
+
+    def __init__(self, handle, nrows: int, ncols: int, nnz: int, row_ptr, col_indices, values):
+        self._h = handle
+        self.nrows, self.ncols, self.nnz = nrows, ncols, nnz
+        self.row_ptr, self.col_indices, self.values = row_ptr, col_indices, values
+
+    @classmethod
+    def from_parts(cls, values, col_indices, row_ptr, nrows: int, ncols: int) -> "CsrMatrix":
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        ci = np.ascontiguousarray(col_indices, dtype=np.int64)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if rp.shape != (nrows + 1,):
+            raise N.CfdlaError(N.ERR_DIMENSION, f"row_ptr has {rp.size} entries, expected {nrows + 1}")
+        if ci.size != v.size or ci.size != int(rp[-1]):
+            raise N.CfdlaError(N.ERR_DIMENSION, "col_indices/values length differs from row_ptr[nrows]")
+        h = C.c_void_p()
+        N.call("cfdla_matrix_create", nrows, ncols, N.ptr(rp), N.ptr(ci), N.ptr(v), C.byref(h))
+        # keep host arrays in sorted order
+        order = np.concatenate([np.argsort(ci[rp[i]:rp[i + 1]], kind="stable") + rp[i] for i in range(nrows)]) \
+            if nrows and not _rows_sorted(rp, ci) else None
+        if order is not None:
+            ci, v = ci[order], v[order]
+        return cls(h, nrows, ncols, int(rp[-1]), rp, ci, v)
+
+    @classmethod
+    def from_scipy(cls, m) -> "CsrMatrix":
+        m = m.tocsr()
+        return cls.from_parts(m.data, m.indices, m.indptr, m.shape[0], m.shape[1])
+
+    @property
+    def shape(self):
+        return (self.nrows, self.ncols)
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("matrix has been closed")
+        return self._h
+
+    def update_values(self, values) -> None:
+        """Replace the values of the same pattern (caller's original order)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.nnz:
+            raise N.CfdlaError(N.ERR_DIMENSION, "values length differs from nnz")
+        N.call("cfdla_matrix_update_values", self.handle, N.ptr(v))
+
+    def spmv(self, x):
+        """y = A x (numpy host vector or torch device tensor, FP64 or FP32)."""
+        if _is_torch(x):
+            import torch
+
+            y = torch.empty(self.nrows, dtype=x.dtype, device=x.device)
+            self.spmv_into(x, y)
+            return y
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.ncols,):
+            raise N.CfdlaError(N.ERR_DIMENSION, f"x has shape {x.shape}, expected ({self.ncols},)")
+        y = np.empty(self.nrows)
+        N.call("cfdla_spmv", self.handle, N.ptr(x), N.ptr(y))
+        return y
+
+    def spmv_into(self, x, y) -> None:
+        if not _is_torch(x):
+            y[:] = self.spmv(x)
+            return
+        import torch
+
+        if x.numel() != self.ncols or y.numel() != self.nrows:
+            raise N.CfdlaError(N.ERR_DIMENSION, "vector length differs from matrix shape")
+        dt = {torch.float64: N.DT_F64, torch.float32: N.DT_F32}[x.dtype]
+        if y.dtype != x.dtype or not (x.is_contiguous() and y.is_contiguous()):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "x and y must be contiguous and of one dtype")
+        bind_torch_stream()
+        N.call("cfdla_spmv_device", self.handle, dt, x.data_ptr(), y.data_ptr())
+
+    def diagonal(self):
+        d = np.zeros(self.nrows)
+        for i in range(self.nrows):
+            s, e = self.row_ptr[i], self.row_ptr[i + 1]
+            hit = np.nonzero(self.col_indices[s:e] == i)[0]
+            if hit.size:
+                d[i] = self.values[s + hit[0]]
+        return d
+
+    def to_dense(self):
+        d = np.zeros((self.nrows, self.ncols))
+        rows = np.repeat(np.arange(self.nrows), np.diff(self.row_ptr))
+        np.add.at(d, (rows, self.col_indices), self.values)
+        return d
+
+    def close(self) -> None:
+        if self._h is not None and N._lib is not None:
+            N.lib().cfdla_matrix_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _rows_sorted(rp, ci) -> bool:
+    if ci.size < 2:
+        return True
+    d = np.diff(ci)
+    row_start = np.zeros(ci.size, dtype=bool)
+    row_start[rp[:-1][rp[:-1] < ci.size]] = True
+    return bool(np.all((d > 0) | row_start[1:]))
+
+
+class SparseMatrixBuilder:
+    """Triplet accumulator; ``build()`` sums duplicates in insertion order and
+    sorts each row (crates/cfd-math/src/sparse/builder.rs:303-324)."""
+
+    def __init__(self, rows: int, cols: int):
+        self.rows, self.cols = rows, cols
+        self._r: list[int] = []
+        self._c: list[int] = []
+        self._v: list[float] = []
+
+    def add_entry(self, row: int, col: int, value: float) -> None:
+        if not (0 <= row < self.rows and 0 <= col < self.cols):
+            raise N.CfdlaError(N.ERR_INVALID_ARG,
+                               f"Entry ({row}, {col}) out of bounds for {self.rows}x{self.cols} matrix")
+        self._r.append(row)
+        self._c.append(col)
+        self._v.append(float(value))
+
+    def add_triplets(self, triplets) -> None:
+        for r, c, v in triplets:
+            self.add_entry(r, c, v)
+
+    def build_arrays(self):
+        if not self._r:
+            return np.zeros(self.rows + 1, np.int64), np.zeros(0, np.int64), np.zeros(0)
+        r = np.asarray(self._r, np.int64)
+        c = np.asarray(self._c, np.int64)
+        v = np.asarray(self._v, np.float64)
+        order = np.lexsort((c, r))  # stable: insertion order kept among duplicates
+        r, c, v = r[order], c[order], v[order]
+        key_change = np.ones(r.size, dtype=bool)
+        key_change[1:] = (r[1:] != r[:-1]) | (c[1:] != c[:-1])
+        starts = np.nonzero(key_change)[0]
+        vals = np.empty(starts.size)
+        for k, s in enumerate(starts):  # sequential sum, first value as-is
+            e = starts[k + 1] if k + 1 < starts.size else r.size
+            acc = v[s]
+            for q in range(s + 1, e):
+                acc += v[q]
+            vals[k] = acc
+        rows, cols = r[starts], c[starts]
+        rp = np.zeros(self.rows + 1, np.int64)
+        np.add.at(rp, rows + 1, 1)
+        return np.cumsum(rp), cols, vals
+
+    def build(self) -> CsrMatrix:
+        rp, ci, v = self.build_arrays()
+        return CsrMatrix.from_parts(v, ci, rp, self.rows, self.cols)

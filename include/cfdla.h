@@ -1,0 +1,271 @@
+/*
+ * cfdla.h — C ABI of the MI355X-native sparse linear-solve library
+ * (libcfdla.so): CSR SpMV, PCG / BiCGSTAB / GMRES(m), Jacobi and
+ * aggregation-AMG preconditioners, FP32-inner / FP64-refined solves,
+ * adjoint solves with coefficient gradients, and batched network solves.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes.  No
+ * exceptions cross this boundary.  Every entry point returns a
+ * cfdla_status; on failure cfdla_last_error() returns a thread-local
+ * message.
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the CFDrs repository root).
+ *
+ * Threading: handles are not thread-safe.  Calls on one handle must be
+ * serialized by the caller (the reference serializes AMG application with a
+ * Mutex, crates/cfd-math/src/linear_solver/preconditioners/multigrid/amg.rs:519-552).
+ * All device work is issued on the library stream (cfdla_set_stream).
+ */
+#ifndef CFDLA_H
+#define CFDLA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Mirror cfd_core::error::Error variants
+ * (crates/cfd-core/src/error.rs:13-45): InvalidInput/InvalidConfiguration ->
+ * INVALID_ARG/CONFIG, DimensionMismatch -> DIMENSION, Numerical -> NUMERICAL,
+ * GpuCompute -> DEVICE.  A solve that stalls or breaks down still returns
+ * CFDLA_OK (value-semantic termination, krylov.rs:375-400). */
+typedef enum {
+  CFDLA_OK = 0,
+  CFDLA_ERR_INVALID_ARG = 1,
+  CFDLA_ERR_DIMENSION = 2,
+  CFDLA_ERR_CONFIG = 3,
+  CFDLA_ERR_NUMERICAL = 4,
+  CFDLA_ERR_DEVICE = 5,
+  CFDLA_ERR_OOM = 6
+} cfdla_status;
+
+/* Termination, mirrors athena Termination as classified by krylov::interpret
+ * (crates/cfd-math/src/linear_solver/krylov.rs:375-400):
+ * CONVERGED / INITIAL_RESIDUAL -> SolveOutcome::Converged,
+ * MAX_ITERATIONS -> Stalled, BREAKDOWN / NONPOSITIVE_CURVATURE -> BrokenDown. */
+typedef enum {
+  CFDLA_TERM_CONVERGED = 0,
+  CFDLA_TERM_INITIAL_RESIDUAL = 1,
+  CFDLA_TERM_MAX_ITERATIONS = 2,
+  CFDLA_TERM_BREAKDOWN = 3,
+  CFDLA_TERM_NONPOSITIVE_CURVATURE = 4
+} cfdla_termination;
+
+/* krylov::SolverKind (krylov.rs:534-560). */
+typedef enum { CFDLA_CG = 0, CFDLA_BICGSTAB = 1, CFDLA_GMRES = 2 } cfdla_solver_kind;
+
+/* Arithmetic of the solve.  FP32_REFINED: FP32 SpMV, smoother and inner
+ * Krylov iterations inside FP64 defect correction (outer residual, update
+ * and coarsest AMG solve in FP64). */
+typedef enum { CFDLA_FP64 = 0, CFDLA_FP32_REFINED = 1 } cfdla_precision;
+
+/* Value type of device vectors passed to *_device entry points. */
+typedef enum { CFDLA_F64 = 0, CFDLA_F32 = 1 } cfdla_dtype;
+
+typedef struct cfdla_matrix_s *cfdla_matrix;
+typedef struct cfdla_precond_s *cfdla_precond;
+typedef struct cfdla_batch_s *cfdla_batch;
+
+/* Solver configuration.  Replaces IterativeSolverConfig (ext leto-ops,
+ * used at krylov.rs:123-138) plus the restart argument of
+ * gmres_preconditioned (krylov.rs:150-160).  Stopping rule:
+ * ||b - A x||_2 <= max(tol_abs, tol_rel * ||b||_2), confirmed by an explicit
+ * FP64 true residual before CONVERGED is reported. */
+typedef struct {
+  int32_t kind;            /* cfdla_solver_kind */
+  int32_t restart;         /* GMRES restart width as requested */
+  int32_t restart_ladder;  /* 1: round restart up 8/16/32/64/128/256 as krylov.rs:84-114 */
+  int32_t precision;       /* cfdla_precision */
+  double tol_abs;
+  double tol_rel;
+  int64_t max_iter;        /* total (inner) iteration budget */
+  double inner_tol;        /* FP32_REFINED: relative tolerance of each inner solve */
+  int32_t max_outer;       /* FP32_REFINED: refinement step budget */
+  int32_t check_every;     /* host convergence poll interval in iterations (0 = auto) */
+} cfdla_solver_config;
+
+/* athena SolveReport {iterations, initial_residual_norm,
+ * final_residual_norm, threshold, termination} (krylov.rs:506-516). */
+typedef struct {
+  int64_t iterations;
+  int64_t outer_iterations;
+  double initial_residual_norm;
+  double final_residual_norm;
+  double threshold;
+  int32_t termination;  /* cfdla_termination */
+  int32_t reserved;
+} cfdla_report;
+
+/* AMGConfig (crates/cfd-math/src/linear_solver/preconditioners/multigrid/mod.rs:219-260).
+ * Enumerations: coarsening 0 = RugeStueben, 1 = Aggregation, 2 = Hybrid;
+ * interpolation 0 = Classical, 1 = Direct, 2 = SmoothedDirect (prolongator
+ * smoothing P = (I - smoothing_omega D^-1 A) P_direct, no reference
+ * counterpart); smoother 0 = GaussSeidel, 1 = SymmetricGaussSeidel,
+ * 2 = Jacobi (the device V-cycle runs Jacobi only);
+ * coarse_solver 0 = reference sweeps (pre + 10 smoother sweeps, amg.rs:501),
+ * 1 = FP64 dense direct solve. */
+typedef struct {
+  int32_t max_levels;
+  int32_t min_coarse_size;
+  int32_t coarsening;
+  int32_t interpolation;
+  int32_t smoother;
+  int32_t pre_smooth_iterations;
+  int32_t post_smooth_iterations;
+  int32_t coarse_solver;
+  double relaxation_factor;
+  double strength_threshold;
+  double smoothing_omega;
+  int32_t max_aggregate_size;
+  int32_t max_interpolation_points;
+} cfdla_amg_config;
+
+#define CFDLA_MAX_LEVELS 32
+
+/* AMGStatistics (multigrid/mod.rs:366-395). */
+typedef struct {
+  int32_t num_levels;
+  int32_t reserved;
+  int64_t level_rows[CFDLA_MAX_LEVELS];
+  int64_t level_nnz[CFDLA_MAX_LEVELS];
+  double operator_complexity;
+  double grid_complexity;
+  double setup_time;
+} cfdla_amg_statistics;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *cfdla_last_error(void);
+/* Library version string. */
+const char *cfdla_version(void);
+/* Select the HIP device and create the library stream.  Called implicitly
+ * by the first call that needs the device.  Replaces ComputeBackend::Gpu
+ * selection (crates/cfd-core/src/compute/backend.rs:81-92). */
+cfdla_status cfdla_init(int32_t device);
+/* Issue device work on an existing hipStream_t.  NULL selects the device's
+ * null stream (the default stream of PyTorch); CFDLA_LIBRARY_STREAM selects
+ * the library's own non-blocking stream (the initial choice). */
+#define CFDLA_LIBRARY_STREAM ((void *)(intptr_t)-1)
+cfdla_status cfdla_set_stream(void *hip_stream);
+/* Block until all work on the library stream has finished. */
+cfdla_status cfdla_synchronize(void);
+
+/* ---- matrices: SparseMatrix = leto_ops::CsrMatrix (crates/cfd-math/src/sparse/mod.rs:53),
+ *      CsrMatrix::from_parts(values, col_indices, row_ptr, nrows, ncols) --------- */
+
+/* Copies host CSR arrays (sorted or unsorted columns, int64 indices) to the
+ * device.  The caller keeps ownership of its arrays. */
+cfdla_status cfdla_matrix_create(int64_t nrows, int64_t ncols, const int64_t *row_ptr,
+                                 const int64_t *col_indices, const double *values,
+                                 cfdla_matrix *out);
+/* Same, from device int32 row_ptr / col_indices and f64 values (copied). */
+cfdla_status cfdla_matrix_create_device(int64_t nrows, int64_t ncols, int64_t nnz,
+                                        const int32_t *row_ptr_dev, const int32_t *col_dev,
+                                        const double *values_dev, cfdla_matrix *out);
+/* Replace the values of the same pattern (host array of nnz doubles). */
+cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values);
+cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
+
+/* y = A x on host vectors.  Replaces cfd_math::sparse::spmv / try_spmv
+ * (crates/cfd-math/src/sparse/operations.rs:26-80). */
+cfdla_status cfdla_spmv(cfdla_matrix m, const double *x, double *y);
+/* y = A x on device vectors of dtype (F64 or F32). */
+cfdla_status cfdla_spmv_device(cfdla_matrix m, int32_t dtype, const void *x, void *y);
+/* r = b - A x on device FP64 vectors; writes ||r||_2 to *rnorm if non-NULL. */
+cfdla_status cfdla_residual_device(cfdla_matrix m, const double *b, const double *x, double *r,
+                                   double *rnorm);
+
+/* ---- preconditioners ------------------------------------------------------ */
+
+/* Jacobi: z = D^-1 r with identity where |d_ii| <= f64::EPSILON
+ * (leto_ops::JacobiPreconditioner, crates/cfd-math/src/lib.rs:181-184;
+ * DiagJacobi, crates/cfd-1d/src/solver/core/linear_system.rs:270-299).
+ * Non-finite diagonal -> CFDLA_ERR_NUMERICAL. */
+cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out);
+/* AlgebraicMultigrid::new (amg.rs:133-152).  Host setup (coarsening,
+ * interpolation, Galerkin products); device V-cycle. */
+cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *config,
+                                      cfdla_precond *out);
+/* z = M^-1 r (z fully overwritten, amg.rs:553-554) on host vectors.
+ * Replaces Preconditioner::apply_to (amg.rs:506-558) and
+ * athena Preconditioner::apply (amg.rs:560-601). */
+cfdla_status cfdla_precond_apply(cfdla_precond p, const double *r, double *z);
+cfdla_status cfdla_precond_apply_device(cfdla_precond p, int32_t dtype, const void *r, void *z);
+/* AlgebraicMultigrid::recompute (amg.rs:179-214): new fine values, same
+ * pattern; coarse operators recomputed with the existing P and R. */
+cfdla_status cfdla_amg_update_values(cfdla_precond p, const double *values);
+cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out);
+/* Hierarchy export for structure checks.  which: 0 = A_l, 1 = R_l, 2 = P_l.
+ * First call with arrays NULL to get sizes. */
+cfdla_status cfdla_amg_export_level(cfdla_precond p, int32_t level, int32_t which, int64_t *nrows,
+                                    int64_t *ncols, int64_t *nnz, int64_t *row_ptr,
+                                    int64_t *col_indices, double *values);
+/* Aggregate map of level l (fine_to_coarse_map, -1 = None). */
+cfdla_status cfdla_amg_export_map(cfdla_precond p, int32_t level, int64_t *fine_to_coarse);
+cfdla_status cfdla_precond_destroy(cfdla_precond p);
+
+/* ---- solves ------------------------------------------------------------------ */
+
+/* Solve A x = b.  Replaces krylov::{cg,bicgstab,gmres}[_preconditioned]
+ * and SolverKind::{solve,solve_preconditioned} (krylov.rs:150-373,
+ * 562-612).  p NULL = identity.  x carries the initial guess in and the last
+ * iterate out, whatever the termination. */
+cfdla_status cfdla_solve(cfdla_matrix m, cfdla_precond p, const double *b, double *x_inout,
+                         const cfdla_solver_config *config, cfdla_report *report);
+/* Same on device FP64 vectors. */
+cfdla_status cfdla_solve_device(cfdla_matrix m, cfdla_precond p, const double *b_dev,
+                                double *x_dev, const cfdla_solver_config *config,
+                                cfdla_report *report);
+/* Adjoint solve A^T lambda = g (device FP64).  For a symmetric matrix the
+ * forward preconditioner is reused; otherwise the transpose is formed once
+ * and cached with an identity/Jacobi preconditioner.  No reference
+ * counterpart (the reference has no adjoint code). */
+cfdla_status cfdla_solve_transpose_device(cfdla_matrix m, cfdla_precond p, const double *g_dev,
+                                          double *lambda_dev, const cfdla_solver_config *config,
+                                          cfdla_report *report);
+
+/* ---- gradients (no reference counterpart) ----------------------------------- */
+
+/* Cell-centred variable-coefficient 5-point Poisson on nx*ny cells with
+ * face coefficient 2 kP kN/(kP+kN) * inv_h2 and Dirichlet faces
+ * (dir_mask bits 0..3 = -x,+x,-y,+y; values dir_val[4]) adding kP*inv_h2.
+ * Given log-coefficients g (k = exp g), the forward solution p, the adjoint
+ * lambda and the explicit objective derivative dJdk (per cell), writes
+ * dJ/dg = k * (dJdk - lambda^T dA/dk p + lambda^T db/dk) per cell.
+ * All device FP64 arrays of nx*ny. */
+cfdla_status cfdla_coef_grad_poisson2d_device(int64_t nx, int64_t ny, double inv_h2,
+                                              int32_t dir_mask, const double *dir_val,
+                                              const double *k, const double *p,
+                                              const double *lambda, const double *dJdk,
+                                              double *dJdg);
+
+/* ---- problem generators (host, deterministic) -------------------------------- */
+
+/* SplitMix64 uniform[lo, hi) stream: out[i] for i < n from seed. */
+cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n, double *out);
+/* Cell-centred pressure-correction Laplacian on nxc*nyc interior cells,
+ * all fluid (crates/cfd-2d/src/pressure_velocity/correction.rs:273-343);
+ * dirichlet[4] = {west, east, south, north}.  Two-phase: call with arrays NULL
+ * to get nnz. */
+cfdla_status cfdla_problem_pressure_correction(int64_t nxc, int64_t nyc, double dx, double dy,
+                                               const int32_t *dirichlet, int64_t *nnz,
+                                               int64_t *row_ptr, int64_t *col_indices,
+                                               double *values);
+/* Variable-coefficient cell-centred Poisson on nx*ny*nz cells (nz = 1 for
+ * 2D) with harmonic-mean faces; dir_mask bits 0..5 = -x,+x,-y,+y,-z,+z;
+ * rhs_bc receives the Dirichlet contributions.  Two-phase like above. */
+cfdla_status cfdla_problem_varcoef(int64_t nx, int64_t ny, int64_t nz, double inv_h2,
+                                   const double *k, int32_t dir_mask, const double *dir_val,
+                                   int64_t *nnz, int64_t *row_ptr, int64_t *col_indices,
+                                   double *values, double *rhs_bc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDLA_H */

@@ -1,0 +1,267 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Tolerances are stated per test:
+
+* integer-valued SpMV: bit-exact (every partial sum is an exact integer);
+* real-valued SpMV / preconditioner applications: relative 1e-13 / 1e-12
+  (the device sums row entries in a lane tree, the oracle sequentially);
+* solves: explicit FP64 relative residual ||b - A x|| / ||b|| <= 1e-10
+  (recomputed with the oracle's SpMV), and the solution error against an
+  oracle solve at 1e-13 is bounded as stated in each test.
+"""
+import numpy as np
+import pytest
+
+import cfdrs_amd as ca
+from cfdrs_amd import problems as P
+from conftest import grid_laplacian, dense_to_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_res(orc, x, b):
+    return np.linalg.norm(b - orc.spmv(x)) / np.linalg.norm(b)
+
+
+def random_csr(n, m, row_len_max, seed, integer=False, empty_every=0):
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        k = 0 if (empty_every and i % empty_every == 0) else int(rng.integers(0, row_len_max + 1))
+        cols = np.sort(rng.choice(m, size=min(k, m), replace=False))
+        vals = rng.integers(-9, 10, size=cols.size).astype(float) if integer else rng.normal(size=cols.size)
+        ci.extend(cols.tolist())
+        v.extend(vals.tolist())
+        rp.append(len(ci))
+    return np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
+
+
+@pytest.mark.parametrize("row_len", [1, 3, 7, 16, 40, 130])
+def test_spmv_integer_bit_exact(row_len):
+    n, m = 3001, 2777
+    rp, ci, v = random_csr(n, m, row_len, seed=row_len, integer=True, empty_every=97)
+    x = np.random.default_rng(1).integers(-50, 51, size=m).astype(float)
+    y = ca.CsrMatrix.from_parts(v, ci, rp, n, m).spmv(x)
+    assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
+
+
+def test_spmv_known_answer(known):
+    k = known["spmv_basic"]
+    a = ca.CsrMatrix.from_parts(k["values"], k["col_indices"], k["row_ptr"], 3, 3)
+    assert a.spmv(np.array(k["x"])).tolist() == k["y"]
+    with pytest.raises(ca.CfdlaError):
+        a.spmv(np.array([1.0, 2.0]))  # dimension mismatch (sparse/tests.rs:218-219)
+
+
+def test_spmv_config2_full_size_real():
+    rp, ci, v, b = P.config2(128)
+    n = 128 ** 3
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    y = a.spmv(b)
+    yo = O.Csr.from_arrays(rp, ci, v, n).spmv(b)
+    assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
+    # linearity and the Neumann/Dirichlet property at full size:
+    # A 1 vanishes except on the Dirichlet face z = 0, where it is k / h^2
+    ones = a.spmv(np.ones(n)).reshape(128, 128, 128)
+    k = P.config2_k(128).reshape(128, 128, 128)
+    assert np.max(np.abs(ones[1:])) <= 1e-9 * np.max(np.abs(v))
+    assert np.allclose(ones[0], k[0] * 128 * 128, rtol=1e-12)
+    y2 = a.spmv(2.0 * b + ones.ravel())
+    assert np.allclose(y2, 2.0 * y + a.spmv(ones.ravel()), rtol=1e-12, atol=1e-9 * np.max(np.abs(y)))
+
+
+def test_jacobi_apply_matches_oracle():
+    rp, ci, v, b = P.config1(64)
+    n = 64 * 64
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    z = np.empty(n)
+    ca.JacobiPreconditioner(a).apply_to(b, z)
+    assert np.array_equal(z, b * O.jacobi_inv_diag(O.Csr.from_arrays(rp, ci, v, n)))
+
+
+AMG_CASES = [
+    dict(coarsening="aggregation", interp="direct", coarse="direct", omega=0.8),
+    dict(coarsening="aggregation", interp="direct", coarse="sweeps", omega=1.0),
+    dict(coarsening="three_phase", interp="smoothed", coarse="direct", omega=0.8),
+    dict(coarsening="three_phase", interp="direct", coarse="sweeps", omega=0.7),
+]
+
+
+@pytest.mark.parametrize("case", AMG_CASES, ids=lambda c: f"{c['coarsening']}-{c['interp']}-{c['coarse']}")
+def test_amg_vcycle_matches_oracle(case):
+    nn = 20
+    n = nn ** 3
+    k = P.config2_k(nn)
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), k, 1 << 4, np.zeros(6))
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    smoothed = case["interp"] == "smoothed"
+    if case["coarsening"] == "aggregation":
+        cfg = ca.AMGConfig.gpu_aggregation(smoothed=smoothed, relaxation_factor=case["omega"])
+    else:
+        cfg = ca.AMGConfig.smoothed_aggregation(0.25, case["omega"], smoothed=smoothed)
+    cfg.coarse_solver = ca.CoarseSolver.Direct if case["coarse"] == "direct" else ca.CoarseSolver.ReferenceSweeps
+    amg = ca.AlgebraicMultigrid(a, cfg)
+    oamg = O.Amg(O.Csr.from_arrays(rp, ci, v, n), coarsening=case["coarsening"], interp=case["interp"],
+                 smoother="jacobi", omega=case["omega"], coarse_solve=case["coarse"], sa_omega=cfg.smoothing_omega,
+                 theta=cfg.strength_threshold)
+    r = P.uniform(77, n)
+    z = np.empty(n)
+    amg.apply_to(r, z)
+    zo = oamg.apply(r)
+    assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+def test_pcg_jacobi_config1():
+    rp, ci, v, b = P.config1(128)
+    n = 128 * 128
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    x = np.zeros(n)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+    rep = ca.cg_preconditioned(a, b, ca.JacobiPreconditioner(a), x, cfg)
+    assert rep.termination == ca.Termination.Converged
+    assert rel_res(orc, x, b) <= 1e-10
+    xo, orep = O.solve(orc, b, kind="cg", prec="jacobi", tol_rel=1e-13, max_it=50000)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    assert err <= 1e-5, err  # kappa(A) ~ 1e4 times the 1e-10 residual
+    # iteration counts agree with the oracle's textbook PCG to within a few steps
+    _, orep10 = O.solve(orc, b, kind="cg", prec="jacobi", tol_rel=1e-10, max_it=50000)
+    assert abs(rep.iterations - orep10.iterations) <= max(3, 0.02 * orep10.iterations)
+
+
+def _config2_small(nn=32):
+    n = nn ** 3
+    k = P.config2_k(nn)
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), k, 1 << 4, np.zeros(6))
+    b = P.uniform(P.SEED_C2_B, n)
+    return rp, ci, v, b, n
+
+
+@pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
+@pytest.mark.parametrize("recipe", ["greedy-direct", "three-phase-smoothed"])
+def test_amg_pcg_config2_small(precision, recipe):
+    rp, ci, v, b, n = _config2_small(32)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    cfg = ca.AMGConfig.gpu_aggregation(smoothed=False) if recipe == "greedy-direct" else \
+        ca.AMGConfig.smoothed_aggregation()
+    amg = ca.AlgebraicMultigrid(a, cfg)
+    x = np.zeros(n)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(2000).with_precision(precision)
+    rep = ca.cg_preconditioned(a, b, amg, x, cfg)
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    xo, _ = O.solve(orc, b, kind="cg", prec="jacobi", tol_rel=1e-13, max_it=100000)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def _momentum(nn=64, seed=3):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1, 1, nn * nn)
+    w = rng.uniform(-1, 1, nn * nn)
+    a = O.assemble_momentum_u(nn, nn, 1 / (nn - 1), 1 / (nn - 1), 1e-2, 1e-3, 1.0, u, w)
+    return a
+
+
+@pytest.mark.parametrize("kind", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
+def test_nonsymmetric_solvers(kind, precision):
+    orc = _momentum()
+    rp, ci, v = orc.arrays()
+    n = rp.size - 1
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    b = P.uniform(31, n)
+    x = np.zeros(n)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(5000).with_precision(precision)
+    jac = ca.JacobiPreconditioner(a)
+    if kind == "bicgstab":
+        rep = ca.bicgstab_preconditioned(a, b, jac, x, cfg)
+    else:
+        rep = ca.gmres_preconditioned(a, b, jac, x, cfg, 30)
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    xo = np.linalg.solve(orc.dense(), b)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
+
+
+def test_gmres_diagonal_known_answer(known):
+    k = known["gmres_diagonal"]
+    a = ca.CsrMatrix.from_parts(k["diag"], [0, 1], [0, 1, 2], 2, 2)
+    ws = ca.KrylovWorkspace(k["restart"], 2)
+    cfg = ca.IterativeSolverConfig.new(k["tolerance"]).with_max_iterations(k["max_iterations"])
+    for _ in range(2):
+        x = np.zeros(2)
+        rep = ca.gmres_preconditioned_with_workspace(a, np.array(k["b"]), None, x, cfg, ws)
+        assert rep.final_residual_norm <= rep.threshold
+        assert np.allclose(x, k["x"], rtol=0, atol=k["solution_tol"])
+
+
+@pytest.mark.parametrize("kind", ["cg", "bicgstab", "gmres"])
+def test_gmres_matches_oracle_iterations_exact_arithmetic_cases(kind):
+    # tiny SPD system: every method terminates within n steps (+ restarts)
+    rng = np.random.default_rng(12)
+    m = rng.normal(size=(10, 10))
+    d = m @ m.T + 10 * np.eye(10)
+    rp, ci, v = dense_to_csr(d)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, 10, 10)
+    b = rng.normal(size=10)
+    x = np.zeros(10)
+    cfg = ca.IterativeSolverConfig.new(1e-12).with_max_iterations(40)
+    rep = ca.SolverKind.Gmres(16).solve(a, b, x, cfg) if kind == "gmres" else (
+        ca.cg(a, b, x, cfg) if kind == "cg" else ca.bicgstab(a, b, x, cfg))
+    assert rep.termination == ca.Termination.Converged
+    assert np.allclose(x, np.linalg.solve(d, b), rtol=1e-9, atol=1e-11)
+
+
+def test_initial_residual_and_stall_and_curvature():
+    rp, ci, v, b = P.config1(32)
+    n = 1024
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    x = np.zeros(n)
+    rep = ca.cg(a, np.zeros(n), x, ca.IterativeSolverConfig.new(1e-10))
+    assert rep.termination == ca.Termination.InitialResidual and rep.iterations == 0
+    x = np.zeros(n)
+    rep = ca.cg(a, b, x, ca.IterativeSolverConfig.new(1e-12).with_max_iterations(3))
+    assert rep.termination == ca.Termination.MaxIterations and rep.iterations == 3
+    assert ca.interpret("p", rep).kind == ca.SolveOutcome.Stalled
+    assert np.any(x != 0)  # the last iterate is kept
+    d = ca.CsrMatrix.from_parts([1.0, -1.0, 2.0], [0, 1, 2], [0, 1, 2, 3], 3, 3)
+    x = np.zeros(3)
+    rep = ca.cg(d, np.ones(3), x, ca.IterativeSolverConfig.new(1e-12).with_max_iterations(10))
+    assert rep.termination in (ca.Termination.NonPositiveCurvature, ca.Termination.Breakdown)
+    assert ca.interpret("p", rep).kind == ca.SolveOutcome.BrokenDown
+
+
+def test_dimension_errors():
+    rp, ci, v, b = P.config1(8)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, 64, 64)
+    with pytest.raises(ca.SolveError):
+        ca.cg(a, np.ones(63), np.zeros(63), ca.IterativeSolverConfig.new(1e-8))
+    small = ca.CsrMatrix.from_parts([1.0], [0], [0, 1], 1, 1)
+    with pytest.raises(ca.SolveError):
+        ca.cg_preconditioned(a, b, ca.JacobiPreconditioner(small), np.zeros(64), ca.IterativeSolverConfig.new(1e-8))
+
+
+def test_solves_are_bitwise_reproducible():
+    rp, ci, v, b, n = _config2_small(24)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    amg = ca.AlgebraicMultigrid(a, ca.AMGConfig.smoothed_aggregation())
+    xs = []
+    for _ in range(2):
+        x = np.zeros(n)
+        ca.cg_preconditioned(a, b, amg, x, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500))
+        xs.append(x)
+    assert np.array_equal(xs[0], xs[1])
+
+
+def test_amg_recompute_then_solve():
+    rp, ci, v, b, n = _config2_small(16)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    amg = ca.AlgebraicMultigrid(a, ca.AMGConfig.smoothed_aggregation())
+    v2 = v * 3.0
+    a.update_values(v2)
+    amg.recompute(ca.CsrMatrix.from_parts(v2, ci, rp, n, n))
+    x = np.zeros(n)
+    rep = ca.cg_preconditioned(a, b, amg, x, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500))
+    assert rep.termination == ca.Termination.Converged
+    assert rel_res(O.Csr.from_arrays(rp, ci, v2, n), x, b) <= 1e-10
