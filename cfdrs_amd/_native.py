@@ -64,6 +64,10 @@ class AmgStatistics(C.Structure):
                 ("setup_time", C.c_double)]
 
 
+class BatchSchedule(C.Structure):
+    _fields_ = [("wave_max_rows", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+
 _VP = C.c_void_p
 _I64P = C.POINTER(C.c_int64)
 _F64P = C.POINTER(C.c_double)
@@ -99,10 +103,24 @@ SIGNATURES = {
     "cfdla_coef_grad_poisson2d_device": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_int32, _VP, _VP, _VP,
                                                    _VP, _VP, _VP]),
     "cfdla_synth_uniform": (C.c_int, [C.c_uint64, C.c_double, C.c_double, C.c_int64, _VP]),
+    "cfdla_synth_grf": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_uint64, _VP]),
     "cfdla_problem_pressure_correction": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _I64P,
                                                     _VP, _VP, _VP]),
     "cfdla_problem_varcoef": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_double, _VP, C.c_int32, _VP, _I64P,
                                         _VP, _VP, _VP, _VP]),
+    "cfdla_batch_create": (C.c_int, [C.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(_VP)]),
+    "cfdla_batch_update_conductance": (C.c_int, [_VP, _VP]),
+    "cfdla_batch_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
+    "cfdla_batch_export": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "cfdla_batch_set_schedule": (C.c_int, [_VP, C.POINTER(BatchSchedule)]),
+    "cfdla_batch_get_schedule": (C.c_int, [_VP, C.POINTER(BatchSchedule)]),
+    "cfdla_batch_solve": (C.c_int, [_VP, C.POINTER(SolverConfig), _VP, _VP]),
+    "cfdla_batch_solve_device": (C.c_int, [_VP, C.POINTER(SolverConfig), _VP, _VP, _VP]),
+    "cfdla_batch_autotune": (C.c_int, [_VP, C.POINTER(SolverConfig), C.c_int32, _VP, _VP, C.POINTER(BatchSchedule)]),
+    "cfdla_batch_edge_gradient_device": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "cfdla_batch_destroy": (C.c_int, [_VP]),
+    "cfdla_synth_networks": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "cfdla_rect_conductance": (C.c_int, [C.c_int64, _VP, _VP, _VP, C.c_double, _VP, _VP]),
 }
 
 _lib = None

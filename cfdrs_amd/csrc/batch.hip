@@ -1,0 +1,416 @@
+// Batched Jacobi-PCG for thousands of independent Kirchhoff networks in one
+// persistent launch.
+//
+// Reference per-network solve: crates/cfd-1d/src/solver/core/linear_system.rs:114-267
+// (Jacobi preconditioner :270-332, convergence measured on the
+// row-equilibrated residual :188-246).  The device solves the symmetric
+// Laplacian itself with Jacobi-PCG (the reference runs CG on the
+// row-scaled, nonsymmetric system) and stops on the same equilibrated
+// residual ||D (b - A x)|| <= max(tol_abs, tol_rel ||D b||), D = 1 / row max.
+//
+// Scheduling: the persistent grid pulls work items from a queue (one atomic
+// ticket per item, exits when the queue is drained).  A system with more
+// than wave_max_rows rows is one item for the whole 256-thread block; smaller
+// systems are grouped four to an item, one per wavefront, and reduce with
+// shuffles only.  Each team keeps the shared vectors (x, p) of its system in
+// LDS; every other vector is touched only by the lane that owns the row.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "network.hpp"
+#include "reduce.hpp"
+
+namespace cfdla {
+
+namespace {
+
+constexpr int kMaxBatchRows = 2048;  // rows of one block-team system (LDS: x and p, FP64)
+constexpr int kWaveSlice = kMaxBatchRows / (kBlock / 64);
+
+struct BArgs {
+  const int *sys_row, *rp, *ci;
+  const double *v64, *b, *jd64, *eq;
+  const float *v32, *jd32;
+  double *x, *r64, *z64, *q64;
+  float *r32, *z32, *q32, *d32;
+  BatchReport *rep;
+  const int *items;
+  unsigned *queue;
+  int n_block_items, n_wave_items, n_wave_sys;
+  double tol_abs, tol_rel, inner_tol;
+  int max_it, max_outer, refined;
+};
+
+struct WaveTeam {
+  static constexpr int size = 64;
+  __device__ static void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __device__ static double sum(double v, double *) { return wave_sum(v); }
+  __device__ static void sum2(double &a, double &b, double *) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+  }
+};
+
+struct BlockTeam {
+  static constexpr int size = kBlock;
+  __device__ static void sync() { __syncthreads(); }
+  __device__ static double sum(double v, double *sh) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+  }
+  __device__ static void sum2(double &a, double &b, double *sh) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      sh[threadIdx.x >> 6] = a;
+      sh[4 + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    a = sh[0] + sh[1] + sh[2] + sh[3];
+    b = sh[4] + sh[5] + sh[6] + sh[7];
+  }
+};
+
+// r = b - A x over the team's rows (x in LDS); returns sum (eq_i r_i)^2.
+template <class Team>
+__device__ double team_resid(const BArgs &A, int r0, int n, int tl, const double *xs, double *sh) {
+  double acc = 0.0;
+  for (int i = tl; i < n; i += Team::size) {
+    const int row = r0 + i;
+    double s = 0.0;
+    for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += A.v64[k] * xs[A.ci[k] - r0];
+    const double r = A.b[row] - s;
+    A.r64[row] = r;
+    const double e = A.eq[row] * r;
+    acc += e * e;
+  }
+  return Team::sum(acc, sh);
+}
+
+// Jacobi-PCG on A d = r (r given in rv), accumulating d into acc (indexed by
+// local row).  Returns the termination; iters counts across calls.
+template <class Team, typename T>
+__device__ int team_cg(const BArgs &A, int r0, int n, int tl, T *acc, T *ps, T *rv, T *zv, T *qv, const T *vals,
+                       const T *jd, double thr2, int &iters, double *sh) {
+  double rz = 0.0;
+  for (int i = tl; i < n; i += Team::size) {
+    const int row = r0 + i;
+    const T z = jd[row] * rv[row];
+    zv[row] = z;
+    ps[i] = z;
+    rz += (double)rv[row] * (double)z;
+  }
+  rz = Team::sum(rz, sh);
+  Team::sync();
+  while (iters < A.max_it) {
+    double pq = 0.0;
+    for (int i = tl; i < n; i += Team::size) {
+      const int row = r0 + i;
+      T s = T(0);
+      for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += vals[k] * ps[A.ci[k] - r0];
+      qv[row] = s;
+      pq += (double)ps[i] * (double)s;
+    }
+    pq = Team::sum(pq, sh);
+    if (!(pq > 0.0) || !isfinite(pq)) return CFDLA_TERM_NONPOSITIVE_CURVATURE;
+    const double alpha = rz / pq;
+    double rr = 0.0, rzn = 0.0;
+    for (int i = tl; i < n; i += Team::size) {
+      const int row = r0 + i;
+      acc[i] += (T)alpha * ps[i];
+      const T r = rv[row] - (T)alpha * qv[row];
+      rv[row] = r;
+      const T z = jd[row] * r;
+      zv[row] = z;
+      const double e = A.eq[row] * (double)r;
+      rr += e * e;
+      rzn += (double)r * (double)z;
+    }
+    Team::sum2(rr, rzn, sh);
+    ++iters;
+    if (rr <= thr2) return CFDLA_TERM_CONVERGED;
+    if (rzn == 0.0 || !isfinite(rzn)) return CFDLA_TERM_BREAKDOWN;
+    const double beta = rzn / rz;
+    rz = rzn;
+    for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i] + (T)beta * ps[i];
+    Team::sync();
+  }
+  return CFDLA_TERM_MAX_ITERATIONS;
+}
+
+template <class Team>
+__device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double *psd, double *sh) {
+  const int r0 = A.sys_row[sys], n = A.sys_row[sys + 1] - r0;
+  double bb = 0.0;
+  for (int i = tl; i < n; i += Team::size) {
+    const int row = r0 + i;
+    xs[i] = A.x[row];
+    const double e = A.eq[row] * A.b[row];
+    bb += e * e;
+  }
+  bb = Team::sum(bb, sh);
+  const double thr = fmax(A.tol_abs, A.tol_rel * sqrt(bb));
+  const double thr2 = thr * thr;
+  Team::sync();
+  double rr = team_resid<Team>(A, r0, n, tl, xs, sh);
+  const double rn0 = sqrt(rr);
+  int iters = 0, outer = 0, term;
+  if (rr <= thr2) {
+    term = CFDLA_TERM_INITIAL_RESIDUAL;
+  } else if (!A.refined) {
+    while (true) {
+      term = team_cg<Team, double>(A, r0, n, tl, xs, psd, A.r64, A.z64, A.q64, A.v64, A.jd64, thr2, iters, sh);
+      Team::sync();
+      rr = team_resid<Team>(A, r0, n, tl, xs, sh);  // explicit FP64 residual
+      if (rr <= thr2) {
+        term = CFDLA_TERM_CONVERGED;
+        break;
+      }
+      if (term != CFDLA_TERM_CONVERGED) break;
+      if (iters >= A.max_it) {
+        term = CFDLA_TERM_MAX_ITERATIONS;
+        break;
+      }
+      // the explicit residual now in r64 restarts the recurrence
+    }
+  } else {
+    float *ps = reinterpret_cast<float *>(psd);
+    float *d = A.d32 + r0;
+    double rn = rn0;
+    int stalls = 0;
+    term = CFDLA_TERM_MAX_ITERATIONS;
+    for (outer = 1; outer <= A.max_outer && iters < A.max_it; ++outer) {
+      const double sc = 1.0 / rn;
+      for (int i = tl; i < n; i += Team::size) {
+        A.r32[r0 + i] = (float)(A.r64[r0 + i] * sc);
+        d[i] = 0.0f;
+      }
+      const int t = team_cg<Team, float>(A, r0, n, tl, d, ps, A.r32, A.z32, A.q32, A.v32, A.jd32,
+                                          A.inner_tol * A.inner_tol, iters, sh);
+      for (int i = tl; i < n; i += Team::size) xs[i] += rn * (double)d[i];
+      Team::sync();
+      rr = team_resid<Team>(A, r0, n, tl, xs, sh);
+      const double rn2 = sqrt(rr);
+      if (rr <= thr2) {
+        term = CFDLA_TERM_CONVERGED;
+        break;
+      }
+      if (!(rn2 < 0.9 * rn)) {
+        if (++stalls >= 2) {
+          term = (t == CFDLA_TERM_BREAKDOWN || t == CFDLA_TERM_NONPOSITIVE_CURVATURE) ? t : CFDLA_TERM_MAX_ITERATIONS;
+          break;
+        }
+      } else {
+        stalls = 0;
+      }
+      rn = rn2;
+    }
+    if (outer > A.max_outer) outer = A.max_outer;
+  }
+  for (int i = tl; i < n; i += Team::size) A.x[r0 + i] = xs[i];
+  if (tl == 0) {
+    BatchReport r;
+    r.iterations = iters;
+    r.outer = outer;
+    r.rn0 = rn0;
+    r.rn = sqrt(rr);
+    r.thr = thr;
+    r.term = term;
+    r.pad = 0;
+    A.rep[sys] = r;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch(BArgs A) {
+  __shared__ double xs[kMaxBatchRows];
+  __shared__ double ps[kMaxBatchRows];
+  __shared__ double sh[8];
+  __shared__ int item_sh;
+  const int total = A.n_block_items + A.n_wave_items;
+  while (true) {
+    if (threadIdx.x == 0) item_sh = (int)atomicAdd(A.queue, 1u);
+    __syncthreads();
+    const int item = item_sh;
+    __syncthreads();
+    if (item >= total) break;  // queue drained: every block leaves
+    if (item < A.n_block_items) {
+      solve_system<BlockTeam>(A, A.items[item], threadIdx.x, xs, ps, sh);
+    } else {
+      const int w = threadIdx.x >> 6;
+      const int k = (item - A.n_block_items) * (kBlock / 64) + w;
+      if (k < A.n_wave_sys)
+        solve_system<WaveTeam>(A, A.items[A.n_block_items + k], threadIdx.x & 63, xs + w * kWaveSlice,
+                               ps + w * kWaveSlice, sh);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reset_queue(unsigned *q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0u;
+}
+
+}  // namespace
+
+void BatchSolver::build(const NetworkInput &in) {
+  nsys = in.nsys;
+  assemble_networks(in, host, rhs);
+  nrows = host.n;
+  sys_row.resize((size_t)nsys + 1);
+  for (int64_t s = 0; s <= nsys; ++s) sys_row[s] = (int32_t)in.node_off[s];
+  for (int64_t s = 0; s < nsys; ++s)
+    CFDLA_REQUIRE(in.node_off[s + 1] - in.node_off[s] <= kMaxBatchRows, CFDLA_ERR_CONFIG,
+                  "batched networks are limited to 2048 nodes each; solve larger ones with cfdla_solve");
+  on_device = false;
+}
+
+void BatchSolver::make_items() {
+  const int wm = std::min(std::max(sched.wave_max_rows, 0), kWaveSlice);
+  std::vector<int32_t> big, small;
+  for (int64_t s = 0; s < nsys; ++s) (sys_row[s + 1] - sys_row[s] > wm ? big : small).push_back((int32_t)s);
+  auto by_size = [&](int32_t a, int32_t b) {
+    const int na = sys_row[a + 1] - sys_row[a], nb = sys_row[b + 1] - sys_row[b];
+    return na != nb ? na > nb : a < b;
+  };
+  std::sort(big.begin(), big.end(), by_size);
+  std::sort(small.begin(), small.end(), by_size);
+  std::vector<int32_t> items(big);
+  items.insert(items.end(), small.begin(), small.end());
+  n_block_items = (int)big.size();
+  n_wave_sys = (int)small.size();
+  n_wave_items = (n_wave_sys + kBlock / 64 - 1) / (kBlock / 64);
+  if (items.empty()) items.push_back(0);
+  d_items.upload(items.data(), items.size());
+  sync_stream();
+}
+
+void BatchSolver::upload() {
+  std::vector<double> jd((size_t)nrows, 1.0), eq((size_t)nrows, 1.0);
+  std::vector<float> jd32((size_t)nrows), v32(host.v.size());
+  for (int64_t i = 0; i < nrows; ++i) {
+    double d = 0.0, mx = 0.0;
+    for (int64_t k = host.rp[i]; k < host.rp[i + 1]; ++k) {
+      if (host.ci[k] == i) d = host.v[k];
+      mx = std::max(mx, std::fabs(host.v[k]));
+    }
+    CFDLA_REQUIRE(std::isfinite(d), CFDLA_ERR_NUMERICAL, "network matrix has a non-finite diagonal");
+    jd[i] = std::fabs(d) <= 2.220446049250313e-16 ? 1.0 : 1.0 / d;  // linear_system.rs:286-295
+    eq[i] = (mx > 2.220446049250313e-16 && std::isfinite(mx)) ? 1.0 / mx : 1.0;  // linear_system.rs:161-169
+    jd32[i] = (float)jd[i];
+  }
+  for (size_t k = 0; k < v32.size(); ++k) v32[k] = (float)host.v[k];
+  std::vector<int32_t> rp32((size_t)nrows + 1);
+  for (int64_t i = 0; i <= nrows; ++i) rp32[i] = (int32_t)host.rp[i];
+  const size_t N = (size_t)std::max<int64_t>(1, nrows);
+  d_sys_row.upload(sys_row.data(), sys_row.size());
+  d_rp.upload(rp32.data(), rp32.size());
+  d_ci.upload(host.ci.data(), std::max<size_t>(1, host.ci.size()));
+  d_v64.upload(host.v.data(), std::max<size_t>(1, host.v.size()));
+  d_v32.upload(v32.data(), std::max<size_t>(1, v32.size()));
+  d_b.upload(rhs.data(), N);
+  d_jd64.upload(jd.data(), N);
+  d_eq.upload(eq.data(), N);
+  d_jd32.upload(jd32.data(), N);
+  d_r64.ensure(N); d_z64.ensure(N); d_q64.ensure(N);
+  d_r32.ensure(N); d_z32.ensure(N); d_q32.ensure(N); d_d32.ensure(N);
+  d_rep.ensure((size_t)std::max<int64_t>(1, nsys));
+  d_queue.ensure(1);
+  make_items();
+  sync_stream();
+  on_device = true;
+}
+
+void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *reports, const double *rhs_dev) {
+  CFDLA_REQUIRE(cfg.kind == CFDLA_CG, CFDLA_ERR_CONFIG, "batched networks are solved with CG (SPD Laplacians)");
+  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
+                "unknown precision");
+  CFDLA_REQUIRE(cfg.max_iter >= 1 && cfg.max_iter < INT32_MAX, CFDLA_ERR_CONFIG, "max_iter out of range");
+  if (!on_device) upload();
+  if (nsys == 0) return;
+  BArgs a;
+  a.sys_row = d_sys_row.p;
+  a.rp = d_rp.p;
+  a.ci = d_ci.p;
+  a.v64 = d_v64.p;
+  a.b = rhs_dev ? rhs_dev : d_b.p;
+  a.jd64 = d_jd64.p;
+  a.eq = d_eq.p;
+  a.v32 = d_v32.p;
+  a.jd32 = d_jd32.p;
+  a.x = x;
+  a.r64 = d_r64.p;
+  a.z64 = d_z64.p;
+  a.q64 = d_q64.p;
+  a.r32 = d_r32.p;
+  a.z32 = d_z32.p;
+  a.q32 = d_q32.p;
+  a.d32 = d_d32.p;
+  a.rep = d_rep.p;
+  a.items = d_items.p;
+  a.queue = d_queue.p;
+  a.n_block_items = n_block_items;
+  a.n_wave_items = n_wave_items;
+  a.n_wave_sys = n_wave_sys;
+  a.tol_abs = cfg.tol_abs;
+  a.tol_rel = cfg.tol_rel;
+  a.inner_tol = cfg.inner_tol > 0.0 ? cfg.inner_tol : 1e-5;
+  a.max_it = (int)cfg.max_iter;
+  a.max_outer = cfg.max_outer > 0 ? cfg.max_outer : 30;
+  a.refined = cfg.precision == CFDLA_FP32_REFINED ? 1 : 0;
+  const int grid = std::max(1, sched.blocks_per_cu) * num_cus();
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream(), d_queue.p);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBlock), 0, stream(), a);
+  CFDLA_HIP(hipGetLastError());
+  if (reports) {
+    CFDLA_HIP(hipMemcpyAsync(reports, d_rep.p, (size_t)nsys * sizeof(BatchReport), hipMemcpyDeviceToHost, stream()));
+  }
+  sync_stream();
+}
+
+void BatchSolver::set_values(const double *g, const NetworkInput &in) {
+  NetworkInput copy = in;
+  copy.g = g;
+  assemble_networks(copy, host, rhs);
+  if (on_device) upload();
+}
+
+namespace {
+// dJ/dparam_e = (dJ/dG_e|explicit - (l_i - l_j)(p_i - p_j)) * dG_e/dparam_e,
+// with l the adjoint restricted to free nodes (Dirichlet nodes carry 0).
+__global__ __launch_bounds__(kBlock) void k_network_grad(int64_t ne, const int *__restrict__ ei,
+                                                         const int *__restrict__ ej,
+                                                         const unsigned char *__restrict__ isdir,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ lam,
+                                                         const double *__restrict__ explicit_dg,
+                                                         const double *__restrict__ dg_dparam,
+                                                         double *__restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+    const int i = ei[e], j = ej[e];
+    const double li = isdir[i] ? 0.0 : lam[i], lj = isdir[j] ? 0.0 : lam[j];
+    double g = explicit_dg ? explicit_dg[e] : 0.0;
+    g -= (li - lj) * (p[i] - p[j]);
+    out[e] = dg_dparam ? g * dg_dparam[e] : g;
+  }
+}
+}  // namespace
+
+void network_gradient(int64_t ne, const int32_t *ei, const int32_t *ej, const uint8_t *isdir, const double *p,
+                      const double *lam, const double *explicit_dg, const double *dg_dparam, double *out) {
+  if (ne == 0) return;
+  hipLaunchKernelGGL(k_network_grad, dim3(blocks_for(ne, kBlock * 4)), dim3(kBlock), 0, stream(), ne, ei, ej, isdir,
+                     p, lam, explicit_dg, dg_dparam, out);
+  CFDLA_HIP(hipGetLastError());
+  sync_stream();
+}
+
+}  // namespace cfdla

@@ -7,6 +7,7 @@
 #include <memory>
 #include <numeric>
 
+#include "network.hpp"
 #include "solver.hpp"
 
 using namespace cfdla;
@@ -592,3 +593,317 @@ cfdla_status cfdla_problem_varcoef(int64_t nx, int64_t ny, int64_t nz, double in
 }
 
 }  // extern "C"
+
+// ---- batched networks ----------------------------------------------------------
+struct cfdla_batch_s {
+  BatchSolver solver;
+  std::vector<int64_t> node_off, edge_off, ei, ej;
+  std::vector<uint8_t> isdir;
+  std::vector<double> dirval, neumann, g;
+  DevBuf<int32_t> d_ei, d_ej;
+  DevBuf<uint8_t> d_isdir;
+  bool edges_on_device = false;
+  NetworkInput input() const {
+    NetworkInput in;
+    in.nsys = (int64_t)node_off.size() - 1;
+    in.node_off = node_off.data();
+    in.edge_off = edge_off.data();
+    in.ei = ei.data();
+    in.ej = ej.data();
+    in.g = g.data();
+    in.is_dirichlet = isdir.data();
+    in.dirichlet_value = dirval.data();
+    in.neumann = neumann.empty() ? nullptr : neumann.data();
+    return in;
+  }
+};
+
+static_assert(sizeof(BatchReport) == sizeof(cfdla_report), "report layouts differ");
+
+extern "C" {
+
+cfdla_status cfdla_batch_create(int64_t n_sys, const int64_t *node_offset, const int64_t *edge_offset,
+                                const int64_t *edge_i, const int64_t *edge_j, const double *conductance,
+                                const uint8_t *is_dirichlet, const double *dirichlet_value, const double *neumann,
+                                cfdla_batch *out) {
+  return guarded([&] {
+    require_ptr(out, "out");
+    CFDLA_REQUIRE(n_sys >= 0, CFDLA_ERR_INVALID_ARG, "n_sys must be non-negative");
+    require_ptr(node_offset, "node_offset");
+    require_ptr(edge_offset, "edge_offset");
+    CFDLA_REQUIRE(node_offset[0] == 0 && edge_offset[0] == 0, CFDLA_ERR_INVALID_ARG, "offsets must start at 0");
+    for (int64_t s = 0; s < n_sys; ++s)
+      CFDLA_REQUIRE(node_offset[s + 1] >= node_offset[s] && edge_offset[s + 1] >= edge_offset[s],
+                    CFDLA_ERR_INVALID_ARG, "offsets must be non-decreasing");
+    const int64_t N = node_offset[n_sys], E = edge_offset[n_sys];
+    if (E) {
+      require_ptr(edge_i, "edge_i");
+      require_ptr(edge_j, "edge_j");
+      require_ptr(conductance, "conductance");
+    }
+    if (N) {
+      require_ptr(is_dirichlet, "is_dirichlet");
+      require_ptr(dirichlet_value, "dirichlet_value");
+    }
+    auto b = std::make_unique<cfdla_batch_s>();
+    b->node_off.assign(node_offset, node_offset + n_sys + 1);
+    b->edge_off.assign(edge_offset, edge_offset + n_sys + 1);
+    b->ei.assign(edge_i, edge_i + E);
+    b->ej.assign(edge_j, edge_j + E);
+    b->g.assign(conductance, conductance + E);
+    b->isdir.assign(is_dirichlet, is_dirichlet + N);
+    b->dirval.assign(dirichlet_value, dirichlet_value + N);
+    if (neumann) b->neumann.assign(neumann, neumann + N);
+    b->solver.build(b->input());
+    *out = b.release();
+  });
+}
+
+cfdla_status cfdla_batch_update_conductance(cfdla_batch b, const double *conductance) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    if (!b->g.empty()) require_ptr(conductance, "conductance");
+    std::vector<double> g(conductance, conductance + b->g.size());
+    NetworkInput in = b->input();
+    b->solver.set_values(g.data(), in);  // validates before committing
+    b->g = std::move(g);
+  });
+}
+
+cfdla_status cfdla_batch_info(cfdla_batch b, int64_t *n_sys, int64_t *n_rows, int64_t *nnz) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    if (n_sys) *n_sys = b->solver.nsys;
+    if (n_rows) *n_rows = b->solver.nrows;
+    if (nnz) *nnz = b->solver.host.nnz();
+  });
+}
+
+cfdla_status cfdla_batch_export(cfdla_batch b, int64_t *row_ptr, int64_t *col_indices, double *values, double *rhs) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    const HostCsr &a = b->solver.host;
+    if (row_ptr) std::copy(a.rp.begin(), a.rp.end(), row_ptr);
+    if (col_indices) for (size_t k = 0; k < a.ci.size(); ++k) col_indices[k] = a.ci[k];
+    if (values) std::copy(a.v.begin(), a.v.end(), values);
+    if (rhs) std::copy(b->solver.rhs.begin(), b->solver.rhs.end(), rhs);
+  });
+}
+
+cfdla_status cfdla_batch_set_schedule(cfdla_batch b, const cfdla_batch_schedule *schedule) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(schedule, "schedule");
+    CFDLA_REQUIRE(schedule->wave_max_rows >= 0 && schedule->wave_max_rows <= 512, CFDLA_ERR_CONFIG,
+                  "wave_max_rows must be in [0, 512]");
+    CFDLA_REQUIRE(schedule->blocks_per_cu >= 1 && schedule->blocks_per_cu <= 16, CFDLA_ERR_CONFIG,
+                  "blocks_per_cu must be in [1, 16]");
+    b->solver.sched.wave_max_rows = schedule->wave_max_rows;
+    b->solver.sched.blocks_per_cu = schedule->blocks_per_cu;
+    if (b->solver.on_device) b->solver.make_items();
+  });
+}
+
+cfdla_status cfdla_batch_get_schedule(cfdla_batch b, cfdla_batch_schedule *schedule) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(schedule, "schedule");
+    schedule->wave_max_rows = b->solver.sched.wave_max_rows;
+    schedule->blocks_per_cu = b->solver.sched.blocks_per_cu;
+  });
+}
+
+cfdla_status cfdla_batch_solve(cfdla_batch b, const cfdla_solver_config *config, double *x_inout,
+                               cfdla_report *reports) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(config, "config");
+    const int64_t N = b->solver.nrows;
+    if (N) require_ptr(x_inout, "x");
+    Context &c = ctx();
+    double *dx = c.dv_at(10, std::max<int64_t>(1, N));
+    CFDLA_HIP(hipMemcpyAsync(dx, x_inout, N * 8, hipMemcpyHostToDevice, stream()));
+    b->solver.solve(dx, *config, reinterpret_cast<BatchReport *>(reports));
+    CFDLA_HIP(hipMemcpyAsync(x_inout, dx, N * 8, hipMemcpyDeviceToHost, stream()));
+    sync_stream();
+  });
+}
+
+cfdla_status cfdla_batch_solve_device(cfdla_batch b, const cfdla_solver_config *config, const double *rhs_dev,
+                                      double *x_dev, cfdla_report *reports) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(config, "config");
+    if (b->solver.nrows) require_ptr(x_dev, "x_dev");
+    b->solver.solve(x_dev, *config, reinterpret_cast<BatchReport *>(reports), rhs_dev);
+  });
+}
+
+cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *config, int32_t n_candidates,
+                                  const cfdla_batch_schedule *candidates, double *ms_out,
+                                  cfdla_batch_schedule *best) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(config, "config");
+    CFDLA_REQUIRE(n_candidates >= 1, CFDLA_ERR_INVALID_ARG, "no candidate schedules");
+    require_ptr(candidates, "candidates");
+    const int64_t N = std::max<int64_t>(1, b->solver.nrows);
+    DevBuf<double> x(N);
+    double best_ms = 1e300;
+    BatchSolver &s = b->solver;
+    BatchSchedule keep = s.sched, pick = s.sched;
+    hipEvent_t e0, e1;
+    CFDLA_HIP(hipEventCreate(&e0));
+    CFDLA_HIP(hipEventCreate(&e1));
+    for (int32_t k = 0; k < n_candidates; ++k) {
+      const cfdla_batch_schedule &cand = candidates[k];
+      CFDLA_REQUIRE(cand.wave_max_rows >= 0 && cand.wave_max_rows <= 512 && cand.blocks_per_cu >= 1 &&
+                        cand.blocks_per_cu <= 16,
+                    CFDLA_ERR_CONFIG, "candidate schedule out of range");
+      s.sched.wave_max_rows = cand.wave_max_rows;
+      s.sched.blocks_per_cu = cand.blocks_per_cu;
+      if (s.on_device) s.make_items();
+      fill_vec<double>((int)N, 0.0, x.p);
+      s.solve(x.p, *config, nullptr);  // warm (uploads on first use)
+      fill_vec<double>((int)N, 0.0, x.p);
+      CFDLA_HIP(hipEventRecord(e0, stream()));
+      s.solve(x.p, *config, nullptr);
+      CFDLA_HIP(hipEventRecord(e1, stream()));
+      CFDLA_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      CFDLA_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms_out) ms_out[k] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        pick = s.sched;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)keep;
+    s.sched = pick;
+    s.make_items();
+    if (best) {
+      best->wave_max_rows = pick.wave_max_rows;
+      best->blocks_per_cu = pick.blocks_per_cu;
+    }
+  });
+}
+
+cfdla_status cfdla_batch_edge_gradient_device(cfdla_batch b, const double *p, const double *lambda,
+                                              const double *explicit_dg, const double *dg_dparam, double *out) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    const int64_t E = (int64_t)b->ei.size();
+    if (E) {
+      require_ptr(p, "p");
+      require_ptr(lambda, "lambda");
+      require_ptr(out, "out");
+    }
+    if (!b->edges_on_device) {
+      std::vector<int32_t> gi((size_t)E), gj((size_t)E);
+      const int64_t ns = (int64_t)b->node_off.size() - 1;
+      for (int64_t s = 0; s < ns; ++s)
+        for (int64_t e = b->edge_off[s]; e < b->edge_off[s + 1]; ++e) {
+          gi[e] = (int32_t)(b->node_off[s] + b->ei[e]);
+          gj[e] = (int32_t)(b->node_off[s] + b->ej[e]);
+        }
+      b->d_ei.upload(gi.data(), std::max<size_t>(1, gi.size()));
+      b->d_ej.upload(gj.data(), std::max<size_t>(1, gj.size()));
+      b->d_isdir.upload(b->isdir.data(), std::max<size_t>(1, b->isdir.size()));
+      sync_stream();
+      b->edges_on_device = true;
+    }
+    network_gradient(E, b->d_ei.p, b->d_ej.p, b->d_isdir.p, p, lambda, explicit_dg, dg_dparam, out);
+  });
+}
+
+cfdla_status cfdla_batch_destroy(cfdla_batch b) {
+  return guarded([&] { delete b; });
+}
+
+cfdla_status cfdla_synth_networks(int64_t count, uint64_t seed_base, int64_t nmin, int64_t nmax, int64_t *node_offset,
+                                  int64_t *edge_offset, int64_t *edge_i, int64_t *edge_j, double *width,
+                                  double *length) {
+  return guarded([&] {
+    require_ptr(node_offset, "node_offset");
+    require_ptr(edge_offset, "edge_offset");
+    SynthNetworks s;
+    synth_networks(count, seed_base, nmin, nmax, s);
+    std::copy(s.node_off.begin(), s.node_off.end(), node_offset);
+    std::copy(s.edge_off.begin(), s.edge_off.end(), edge_offset);
+    if (!edge_i) return;
+    require_ptr(edge_j, "edge_j");
+    for (size_t e = 0; e < s.ei.size(); ++e) {
+      edge_i[e] = s.ei[e];
+      edge_j[e] = s.ej[e];
+    }
+    if (width) std::copy(s.width.begin(), s.width.end(), width);
+    if (length) std::copy(s.length.begin(), s.length.end(), length);
+  });
+}
+
+cfdla_status cfdla_rect_conductance(int64_t n, const double *width, const double *height, const double *length,
+                                    double viscosity, double *g, double *dg_dw) {
+  return guarded([&] {
+    CFDLA_REQUIRE(n >= 0, CFDLA_ERR_INVALID_ARG, "n must be non-negative");
+    if (n) {
+      require_ptr(width, "width");
+      require_ptr(height, "height");
+      require_ptr(length, "length");
+      require_ptr(g, "g");
+    }
+    for (int64_t e = 0; e < n; ++e) {
+      CFDLA_REQUIRE(width[e] > 0.0 && height[e] > 0.0 && length[e] >= 0.0, CFDLA_ERR_INVALID_ARG,
+                    "Invalid geometry: width and height must be > 0");
+      g[e] = 1.0 / rect_resistance(width[e], height[e], length[e], viscosity);
+      if (dg_dw) dg_dw[e] = rect_conductance_dw(width[e], height[e], length[e], viscosity);
+    }
+  });
+}
+
+}  // extern "C"
+
+extern "C" cfdla_status cfdla_synth_grf(int64_t nx, int64_t ny, double corr_len, uint64_t seed, double *out) {
+  // Gaussian random field on a periodic nx*ny grid with covariance
+  // exp(-r^2 / (2 corr_len^2)): white noise (Box-Muller on SplitMix64)
+  // convolved with a Gaussian of standard deviation corr_len / sqrt(2),
+  // then shifted and scaled to sample mean 0 and sample deviation 1.
+  return guarded([&] {
+    CFDLA_REQUIRE(nx >= 1 && ny >= 1 && corr_len > 0.0, CFDLA_ERR_INVALID_ARG, "invalid field size");
+    require_ptr(out, "out");
+    const int64_t n = nx * ny;
+    std::vector<double> w((size_t)n), t((size_t)n);
+    uint64_t st = seed;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < n; i += 2) {
+      double u1 = ((double)(splitmix64(st) >> 11) + 1.0) * (1.0 / 9007199254740992.0);  // (0, 1]
+      double u2 = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+      double rad = std::sqrt(-2.0 * std::log(u1));
+      w[i] = rad * std::cos(two_pi * u2);
+      if (i + 1 < n) w[i + 1] = rad * std::sin(two_pi * u2);
+    }
+    const double s = corr_len / std::sqrt(2.0);
+    const int64_t R = (int64_t)std::ceil(4.0 * s);
+    std::vector<double> ker((size_t)(2 * R + 1));
+    for (int64_t d = -R; d <= R; ++d) ker[d + R] = std::exp(-(double)(d * d) / (2.0 * s * s));
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        double acc = 0.0;
+        for (int64_t d = -R; d <= R; ++d) acc += ker[d + R] * w[y * nx + (((x + d) % nx) + nx) % nx];
+        t[y * nx + x] = acc;
+      }
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        double acc = 0.0;
+        for (int64_t d = -R; d <= R; ++d) acc += ker[d + R] * t[(((y + d) % ny) + ny) % ny * nx + x];
+        out[y * nx + x] = acc;
+      }
+    double mean = 0.0, var = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += out[i];
+    mean /= (double)n;
+    for (int64_t i = 0; i < n; ++i) var += (out[i] - mean) * (out[i] - mean);
+    const double sd = std::sqrt(var / (double)n);
+    for (int64_t i = 0; i < n; ++i) out[i] = sd > 0.0 ? (out[i] - mean) / sd : 0.0;
+  });
+}

@@ -88,6 +88,14 @@ struct HostCsr {
 // Device CSR: int32 row pointers and columns, FP64 values with an optional
 // FP32 copy, and the smoother's inverse diagonal (0 where |a_ii| <= 1e-15,
 // the skip rule of the reference Jacobi smoother).
+//
+// Square matrices with short rows whose column offsets |col - row| fit in
+// int16 also get a sliced-ELL copy: slices of 64 consecutive rows (one
+// wavefront, one lane per row), entries stored column-major within a slice
+// so every load of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32)
+// value segment plus a 128 B offset segment, padded to the longest row of the
+// slice with kSellPad offsets.  The vector-CSR kernel serves everything else.
+constexpr int16_t kSellPad = INT16_MIN;
 struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
@@ -95,16 +103,24 @@ struct DevCsr {
   DevBuf<int32_t> rp, ci;
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;
-  void upload(const HostCsr &h, bool want_fp32, bool want_dinv);
-  void update_values(const double *host_values, bool want_fp32, bool want_dinv);
-  void ensure_fp32();
+  bool sell = false;
+  int nslices = 0;
+  int64_t sell_entries = 0;
+  DevBuf<int32_t> sell_ptr;   // nslices + 1 entry offsets (multiples of 64)
+  DevBuf<int16_t> sell_delta; // col - row, kSellPad for padding
+  DevBuf<double> sell_v64;
+  DevBuf<float> sell_v32;
+  void upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell = true);
   template <typename T> const T *vals() const;
   template <typename T> const T *sdinv() const;
+  template <typename T> const T *sell_vals() const;
 };
 template <> inline const double *DevCsr::vals<double>() const { return v64.p; }
 template <> inline const float *DevCsr::vals<float>() const { return v32.p; }
 template <> inline const double *DevCsr::sdinv<double>() const { return sdinv64.p; }
 template <> inline const float *DevCsr::sdinv<float>() const { return sdinv32.p; }
+template <> inline const double *DevCsr::sell_vals<double>() const { return sell_v64.p; }
+template <> inline const float *DevCsr::sell_vals<float>() const { return sell_v32.p; }
 
 int pick_group(int64_t n, int64_t nnz);
 

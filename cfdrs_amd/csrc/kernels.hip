@@ -201,9 +201,72 @@ void launch_csr_g(const DevCsr &A, const T *x, const T *b, const T *dinv, double
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
+// ---- sliced-ELL product ------------------------------------------------------
+// One wavefront per 64-row slice, one lane per row, no cross-lane reduction.
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// so the slices are split into 8 contiguous ranges and XCD q walks range q:
+// the x gathers of neighbouring rows then hit the L2 of the XCD that owns
+// them.  The grid is a multiple of 8 blocks.
+template <typename T, int EPI, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *__restrict__ sp,
+                                                 const short *__restrict__ dl, const T *__restrict__ av,
+                                                 const T *__restrict__ x, const T *__restrict__ b,
+                                                 const T *__restrict__ dinv, T omega, T *y, const T *dotw, Scal *s,
+                                                 int check_done, double *partials) {
+  if (check_done && s->done) return;
+  constexpr int WPB = kBlock / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
+  const int chunk = (nslices + 7) >> 3;
+  const int s0 = xcd * chunk;
+  const int s1 = min(nslices, s0 + chunk);
+  double dacc = 0.0;
+  for (int sl = s0 + local * WPB + wave; sl < s1; sl += per * WPB) {
+    const int row = sl * 64 + lane;
+    if (row >= n) continue;
+    const int base = sp[sl];
+    const int w = (sp[sl + 1] - base) >> 6;
+    const short *dd = dl + base + lane;
+    const T *vv = av + base + lane;
+    T sum = T(0);
+#pragma unroll 4
+    for (int k = 0; k < w; ++k) {
+      const int d = dd[k * 64];
+      if (d != kSellPad) sum += vv[k * 64] * x[row + d];
+    }
+    T out;
+    if constexpr (EPI == EPI_Y) out = sum;
+    else if constexpr (EPI == EPI_RESID) out = b[row] - sum;
+    else if constexpr (EPI == EPI_JACOBI) out = x[row] + omega * dinv[row] * (b[row] - sum);
+    else out = y[row] + sum;
+    if constexpr (DOT) dacc += (double)(dotw == y ? out : dotw[row]) * (double)out;
+    y[row] = out;
+  }
+  if constexpr (DOT) {
+    double v[1] = {dacc};
+    block_partials<1>(v, partials);
+  }
+}
+
+template <typename T, int EPI, bool DOT>
+void launch_sell(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
+                 Scal *s, bool check_done, int op, RedWs &ws) {
+  int blocks = (A.nslices + kBlock / 64 - 1) / (kBlock / 64);
+  blocks = std::min(blocks, kMaxBlocks);
+  blocks = (blocks + 7) & ~7;
+  hipLaunchKernelGGL((k_sell<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices, A.sell_ptr.p,
+                     A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
+                     ws.partials.p);
+  if (DOT) finish(1, s, op, blocks, ws, check_done);
+}
+
 template <typename T, int EPI, bool DOT>
 void launch_csr_e(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                   Scal *s, bool check_done, int op, RedWs &ws) {
+  if (A.sell && A.sell_vals<T>() != nullptr) {
+    launch_sell<T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws);
+    return;
+  }
   switch (A.group) {
     case 1: launch_csr_g<1, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
     case 2: launch_csr_g<2, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;

@@ -151,7 +151,47 @@ void smoother_dinv(const HostCsr &h, std::vector<double> &d) {
 }
 }  // namespace
 
-void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv) {
+namespace {
+// Sliced-ELL layout when every offset fits in int16, rows are short and the
+// padding stays under 20 %.
+bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t> &delta, std::vector<double> &val) {
+  if (h.n != h.m || h.n == 0) return false;
+  if ((double)h.nnz() > 16.0 * (double)h.n) return false;
+  const int64_t nslices = (h.n + 63) / 64;
+  ptr.assign((size_t)nslices + 1, 0);
+  int64_t total = 0;
+  for (int64_t s = 0; s < nslices; ++s) {
+    int64_t w = 0;
+    for (int64_t r = s * 64; r < std::min<int64_t>(h.n, s * 64 + 64); ++r) {
+      w = std::max<int64_t>(w, h.rp[r + 1] - h.rp[r]);
+      for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) {
+        int64_t d = (int64_t)h.ci[k] - r;
+        if (d < -32767 || d > 32767) return false;
+      }
+    }
+    total += w * 64;
+    if (total >= INT32_MAX) return false;
+    ptr[s + 1] = (int32_t)total;
+  }
+  if ((double)total > 1.2 * (double)h.nnz() + 64.0 * 16.0) return false;
+  delta.assign((size_t)total, kSellPad);
+  val.assign((size_t)total, 0.0);
+  for (int64_t s = 0; s < nslices; ++s)
+    for (int64_t l = 0; l < 64; ++l) {
+      const int64_t r = s * 64 + l;
+      if (r >= h.n) break;
+      int64_t k0 = h.rp[r];
+      for (int64_t k = k0; k < h.rp[r + 1]; ++k) {
+        size_t pos = (size_t)(ptr[s] + (k - k0) * 64 + l);
+        delta[pos] = (int16_t)((int64_t)h.ci[k] - r);
+        val[pos] = h.v[k];
+      }
+    }
+  return true;
+}
+}  // namespace
+
+void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell) {
   CFDLA_REQUIRE(h.n < INT32_MAX && h.m < INT32_MAX && h.nnz() < INT32_MAX, CFDLA_ERR_INVALID_ARG,
                 "matrix too large for 32-bit device indices");
   n = (int)h.n;
@@ -177,6 +217,23 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv) {
     df.resize(d.size());
     for (size_t k = 0; k < d.size(); ++k) df[k] = (float)d[k];
     sdinv32.upload(df.data(), df.size());
+  }
+  std::vector<int32_t> sp;
+  std::vector<int16_t> sd;
+  std::vector<double> sv;
+  sell = allow_sell && build_sell(h, sp, sd, sv);
+  if (sell) {
+    nslices = (int)(sp.size() - 1);
+    sell_entries = sp.back();
+    sell_ptr.upload(sp.data(), sp.size());
+    sell_delta.upload(sd.data(), sd.size());
+    sell_v64.upload(sv.data(), sv.size());
+    if (want_fp32) {
+      std::vector<float> sf(sv.size());
+      for (size_t k = 0; k < sf.size(); ++k) sf[k] = (float)sv[k];
+      sell_v32.upload(sf.data(), sf.size());
+      sync_stream();
+    }
   }
   sync_stream();
 }

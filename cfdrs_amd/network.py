@@ -1,0 +1,239 @@
+"""Batched 1D Kirchhoff networks on the GPU.
+
+Each network is assembled as ``MatrixAssembler::assemble_into``
+(crates/cfd-1d/src/solver/core/matrix_assembly.rs:163-247) and solved with
+Jacobi-PCG to the row-equilibrated residual target of
+``LinearSystemSolver`` (crates/cfd-1d/src/solver/core/linear_system.rs:114-246).
+Thousands of networks go to one persistent launch; the row-bin schedule
+(which networks a single wavefront solves, how many resident blocks per
+compute unit) is chosen by ``NetworkBatch.autotune`` and cached as JSON.
+
+Rectangular-duct conductances follow
+crates/cfd-1d/src/physics/resistance/models/rectangular.rs:95-157,209-245.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .linear_solver import IterativeSolverConfig, SolveReport, Termination
+from .sparse import _is_torch, bind_torch_stream
+
+WATER_VISCOSITY = 1.002e-3  # Pa s
+CHANNEL_HEIGHT = 50e-6      # m
+INLET_PRESSURE = 1000.0     # Pa
+
+
+@dataclass
+class SynthNetworks:
+    node_offset: np.ndarray
+    edge_offset: np.ndarray
+    edge_i: np.ndarray
+    edge_j: np.ndarray
+    width: np.ndarray
+    length: np.ndarray
+
+    @property
+    def count(self) -> int:
+        return self.node_offset.size - 1
+
+    def sizes(self) -> np.ndarray:
+        return np.diff(self.node_offset)
+
+
+def synth_networks(count: int = 4096, seed_base: int = 4000, nmin: int = 50, nmax: int = 2000) -> SynthNetworks:
+    """Planar channel networks: random spanning tree of a grid plus extra grid
+    edges (degrees 2-4); graph g is seeded with seed_base + g."""
+    no = np.empty(count + 1, np.int64)
+    eo = np.empty(count + 1, np.int64)
+    N.call("cfdla_synth_networks", count, C.c_uint64(seed_base), nmin, nmax, N.ptr(no), N.ptr(eo), None, None, None,
+           None)
+    E = int(eo[-1])
+    ei, ej = np.empty(E, np.int64), np.empty(E, np.int64)
+    w, L = np.empty(E), np.empty(E)
+    N.call("cfdla_synth_networks", count, C.c_uint64(seed_base), nmin, nmax, N.ptr(no), N.ptr(eo), N.ptr(ei),
+           N.ptr(ej), N.ptr(w), N.ptr(L))
+    return SynthNetworks(no, eo, ei, ej, w, L)
+
+
+def rect_conductance(width, height, length, viscosity=WATER_VISCOSITY):
+    """(G, dG/dw) of rectangular ducts, G = 1 / R."""
+    w = np.ascontiguousarray(width, np.float64)
+    h = np.ascontiguousarray(np.broadcast_to(height, w.shape), np.float64)
+    L = np.ascontiguousarray(np.broadcast_to(length, w.shape), np.float64)
+    g, dg = np.empty_like(w), np.empty_like(w)
+    N.call("cfdla_rect_conductance", w.size, N.ptr(w), N.ptr(h), N.ptr(L), viscosity, N.ptr(g), N.ptr(dg))
+    return g, dg
+
+
+def _report(r: N.Report) -> SolveReport:
+    return SolveReport(r.iterations, r.initial_residual_norm, r.final_residual_norm, r.threshold,
+                       Termination(r.termination), r.outer_iterations)
+
+
+class NetworkBatch:
+    """Independent networks packed into one block-diagonal system."""
+
+    def __init__(self, node_offset, edge_offset, edge_i, edge_j, conductance, is_dirichlet, dirichlet_value,
+                 neumann=None):
+        self.node_offset = np.ascontiguousarray(node_offset, np.int64)
+        self.edge_offset = np.ascontiguousarray(edge_offset, np.int64)
+        self.edge_i = np.ascontiguousarray(edge_i, np.int64)
+        self.edge_j = np.ascontiguousarray(edge_j, np.int64)
+        self.conductance = np.ascontiguousarray(conductance, np.float64)
+        self.is_dirichlet = np.ascontiguousarray(is_dirichlet, np.uint8)
+        self.dirichlet_value = np.ascontiguousarray(dirichlet_value, np.float64)
+        self.neumann = None if neumann is None else np.ascontiguousarray(neumann, np.float64)
+        h = C.c_void_p()
+        N.call("cfdla_batch_create", self.node_offset.size - 1, N.ptr(self.node_offset), N.ptr(self.edge_offset),
+               N.ptr(self.edge_i), N.ptr(self.edge_j), N.ptr(self.conductance), N.ptr(self.is_dirichlet),
+               N.ptr(self.dirichlet_value), N.ptr(self.neumann), C.byref(h))
+        self._h = h
+        ns, nr, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        N.call("cfdla_batch_info", h, C.byref(ns), C.byref(nr), C.byref(nz))
+        self.n_sys, self.n_rows, self.nnz = ns.value, nr.value, nz.value
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("batch has been closed")
+        return self._h
+
+    def update_conductance(self, conductance) -> None:
+        g = np.ascontiguousarray(conductance, np.float64)
+        N.call("cfdla_batch_update_conductance", self.handle, N.ptr(g))
+        self.conductance = g
+
+    def export(self):
+        """(row_ptr, col_indices, values, rhs) of the assembled batch."""
+        rp = np.empty(self.n_rows + 1, np.int64)
+        ci = np.empty(self.nnz, np.int64)
+        v = np.empty(self.nnz)
+        b = np.empty(self.n_rows)
+        N.call("cfdla_batch_export", self.handle, N.ptr(rp), N.ptr(ci), N.ptr(v), N.ptr(b))
+        return rp, ci, v, b
+
+    @property
+    def schedule(self) -> dict:
+        s = N.BatchSchedule()
+        N.call("cfdla_batch_get_schedule", self.handle, C.byref(s))
+        return {"wave_max_rows": s.wave_max_rows, "blocks_per_cu": s.blocks_per_cu}
+
+    @schedule.setter
+    def schedule(self, d: dict) -> None:
+        s = N.BatchSchedule(int(d["wave_max_rows"]), int(d["blocks_per_cu"]))
+        N.call("cfdla_batch_set_schedule", self.handle, C.byref(s))
+
+    def solve(self, x=None, config: IterativeSolverConfig | None = None, rhs=None):
+        """Solve all networks.  x: numpy (host) or torch CUDA float64 packed
+        pressures (initial guess in, solution out).  Returns (x, reports)."""
+        config = config or IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+        cfg = config.native(N.KIND_CG)
+        reps = (N.Report * max(1, self.n_sys))()
+        if x is not None and _is_torch(x):
+            import torch
+
+            if x.dtype != torch.float64 or x.numel() != self.n_rows or not x.is_contiguous():
+                raise N.CfdlaError(N.ERR_DIMENSION, "x must be a contiguous float64 tensor of n_rows")
+            bind_torch_stream()
+            N.call("cfdla_batch_solve_device", self.handle, C.byref(cfg), None if rhs is None else rhs.data_ptr(),
+                   x.data_ptr(), reps)
+        else:
+            if rhs is not None:
+                raise N.CfdlaError(N.ERR_INVALID_ARG, "a right-hand side override takes device tensors")
+            x = np.zeros(self.n_rows) if x is None else x
+            if x.dtype != np.float64 or x.shape != (self.n_rows,) or not x.flags.c_contiguous:
+                raise N.CfdlaError(N.ERR_DIMENSION, "x must be a contiguous float64 array of n_rows")
+            N.call("cfdla_batch_solve", self.handle, C.byref(cfg), N.ptr(x), reps)
+        return x, [_report(reps[k]) for k in range(self.n_sys)]
+
+    def edge_gradient(self, p, lam, explicit_dg=None, dg_dparam=None):
+        """Device tensors: dJ/dparam per edge from pressures p and adjoint lam."""
+        import torch
+
+        out = torch.empty(self.edge_i.size, dtype=torch.float64, device=p.device)
+        bind_torch_stream()
+        N.call("cfdla_batch_edge_gradient_device", self.handle, p.data_ptr(), lam.data_ptr(),
+               None if explicit_dg is None else explicit_dg.data_ptr(),
+               None if dg_dparam is None else dg_dparam.data_ptr(), out.data_ptr())
+        return out
+
+    # ---- autotuning -------------------------------------------------------------------
+    DEFAULT_CANDIDATES = [(w, b) for w in (0, 64, 128, 256, 512) for b in (1, 2, 4)]
+
+    def signature(self, config: IterativeSolverConfig) -> str:
+        sizes = np.diff(self.node_offset)
+        hist = np.bincount(np.minimum(np.log2(np.maximum(sizes, 1)).astype(int), 12), minlength=13)
+        return f"{int(config.precision)}|{config.tolerance:g}|" + ",".join(str(int(h)) for h in hist)
+
+    def autotune(self, config: IterativeSolverConfig | None = None, candidates=None, cache_path: str | None = None):
+        """Time candidate (wave_max_rows, blocks_per_cu) schedules with a full
+        solve each, apply and return the fastest; results are cached in JSON
+        keyed by precision, tolerance and a histogram of network sizes."""
+        config = config or IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+        cache_path = cache_path or os.path.join(os.path.expanduser("~"), ".cache", "cfdrs_amd",
+                                                "batch_autotune.json")
+        key = self.signature(config)
+        cache = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        if key in cache:
+            self.schedule = cache[key]["best"]
+            return cache[key]
+        cands = candidates or self.DEFAULT_CANDIDATES
+        arr = (N.BatchSchedule * len(cands))(*[N.BatchSchedule(w, b) for w, b in cands])
+        ms = (C.c_double * len(cands))()
+        best = N.BatchSchedule()
+        cfg = config.native(N.KIND_CG)
+        N.call("cfdla_batch_autotune", self.handle, C.byref(cfg), len(cands), arr, ms, C.byref(best))
+        entry = {"best": {"wave_max_rows": best.wave_max_rows, "blocks_per_cu": best.blocks_per_cu},
+                 "timings_ms": {f"{w}/{b}": ms[k] for k, (w, b) in enumerate(cands)}}
+        cache[key] = entry
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        return entry
+
+    def close(self) -> None:
+        if self._h is not None and N._lib is not None:
+            N.lib().cfdla_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def channel_batch(nets: SynthNetworks, inlet_pressure: float = INLET_PRESSURE, height: float = CHANNEL_HEIGHT,
+                  viscosity: float = WATER_VISCOSITY) -> NetworkBatch:
+    """Batch with node 0 at the inlet pressure and the last node grounded."""
+    g, _ = rect_conductance(nets.width, height, nets.length, viscosity)
+    n_total = int(nets.node_offset[-1])
+    is_dir = np.zeros(n_total, np.uint8)
+    dval = np.zeros(n_total)
+    is_dir[nets.node_offset[:-1]] = 1
+    dval[nets.node_offset[:-1]] = inlet_pressure
+    is_dir[nets.node_offset[1:] - 1] = 1
+    return NetworkBatch(nets.node_offset, nets.edge_offset, nets.edge_i, nets.edge_j, g, is_dir, dval)
+
+
+def outlet_flux(nets: SynthNetworks, g, p):
+    """J_s = sum over edges touching the outlet (last node) of G_e (p_other - p_outlet), per network."""
+    sys_of_edge = np.repeat(np.arange(nets.count), np.diff(nets.edge_offset))
+    outlet = nets.node_offset[1:] - 1 - nets.node_offset[:-1]  # local id
+    o = outlet[sys_of_edge]
+    base = nets.node_offset[:-1][sys_of_edge]
+    at_i = nets.edge_i == o
+    at_j = nets.edge_j == o
+    other = np.where(at_i, nets.edge_j, nets.edge_i) + base
+    touch = at_i | at_j
+    contrib = np.where(touch, g * (p[other] - p[o + base]), 0.0)
+    return np.bincount(sys_of_edge, weights=contrib, minlength=nets.count)

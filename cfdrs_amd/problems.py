@@ -74,6 +74,13 @@ def config2_k(n: int = 128):
     return np.exp(uniform(SEED_C2_K, n ** 3, -math.log(10.0), math.log(10.0)))
 
 
+def grf(nx: int, ny: int, corr_len: float = 16.0, seed: int = SEED_C5) -> np.ndarray:
+    """Gaussian random field, covariance exp(-r^2 / (2 corr_len^2)), sample mean 0, deviation 1."""
+    out = np.empty(nx * ny)
+    N.call("cfdla_synth_grf", nx, ny, corr_len, C.c_uint64(seed), N.ptr(out))
+    return out
+
+
 def config2(n: int = 128):
     """3D variable-coefficient Poisson on n^3 cells, Dirichlet on z = 0 (value 0); b ~ U[-1,1]."""
     k = config2_k(n)

@@ -244,10 +244,79 @@ cfdla_status cfdla_coef_grad_poisson2d_device(int64_t nx, int64_t ny, double inv
                                               const double *lambda, const double *dJdk,
                                               double *dJdg);
 
+/* ---- batched Kirchhoff networks ------------------------------------------------ */
+
+/* Many independent networks solved in one persistent launch.  No reference
+ * counterpart for the batching itself; each network is assembled as
+ * MatrixAssembler::assemble_into (crates/cfd-1d/src/solver/core/matrix_assembly.rs:163-247)
+ * and solved like LinearSystemSolver::solve_with_initial_guess
+ * (crates/cfd-1d/src/solver/core/linear_system.rs:114-155): Jacobi-PCG, with
+ * convergence and the reported residual norms measured on the
+ * row-equilibrated residual (linear_system.rs:188-246).  Networks are
+ * limited to 2048 nodes each. */
+typedef struct {
+  int32_t wave_max_rows; /* networks with at most this many nodes are solved by one wavefront (<= 512) */
+  int32_t blocks_per_cu; /* persistent grid = blocks_per_cu * compute units */
+} cfdla_batch_schedule;
+
+/* node_offset / edge_offset: n_sys + 1 prefix offsets; edge_i / edge_j:
+ * local node ids per edge; is_dirichlet / dirichlet_value / neumann (nullable):
+ * per node.  Errors follow classify_boundary_conditions_into and assemble_into
+ * (matrix_assembly.rs:92-198): self loops, non-positive or non-finite
+ * conductance, a network without a Dirichlet node, isolated free nodes. */
+cfdla_status cfdla_batch_create(int64_t n_sys, const int64_t *node_offset, const int64_t *edge_offset,
+                                const int64_t *edge_i, const int64_t *edge_j, const double *conductance,
+                                const uint8_t *is_dirichlet, const double *dirichlet_value,
+                                const double *neumann, cfdla_batch *out);
+/* Re-assemble with new conductances (same graphs), e.g. per Picard step. */
+cfdla_status cfdla_batch_update_conductance(cfdla_batch b, const double *conductance);
+cfdla_status cfdla_batch_info(cfdla_batch b, int64_t *n_sys, int64_t *n_rows, int64_t *nnz);
+/* Assembled block-diagonal system (global column ids) and right-hand side. */
+cfdla_status cfdla_batch_export(cfdla_batch b, int64_t *row_ptr, int64_t *col_indices, double *values,
+                                double *rhs);
+cfdla_status cfdla_batch_set_schedule(cfdla_batch b, const cfdla_batch_schedule *schedule);
+cfdla_status cfdla_batch_get_schedule(cfdla_batch b, cfdla_batch_schedule *schedule);
+/* Solve every network (config.kind must be CFDLA_CG).  x: packed node
+ * pressures, initial guess in, solution out; reports: n_sys entries or NULL. */
+cfdla_status cfdla_batch_solve(cfdla_batch b, const cfdla_solver_config *config, double *x_inout,
+                               cfdla_report *reports);
+/* Same on device vectors; rhs_dev NULL uses the assembled right-hand side. */
+cfdla_status cfdla_batch_solve_device(cfdla_batch b, const cfdla_solver_config *config,
+                                      const double *rhs_dev, double *x_dev, cfdla_report *reports);
+/* Time each candidate schedule with one full solve; ms_out[k] gets its time
+ * and *best the fastest, which is also applied to the batch. */
+cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *config, int32_t n_candidates,
+                                  const cfdla_batch_schedule *candidates, double *ms_out,
+                                  cfdla_batch_schedule *best);
+/* Edge gradient dJ/dparam_e = (dJ/dG_e|explicit - (l_i - l_j)(p_i - p_j)) * dG_e/dparam_e
+ * from device pressures p and adjoint l (Dirichlet nodes' l taken as 0);
+ * explicit_dg and dg_dparam are device arrays per edge or NULL.  No
+ * reference counterpart. */
+cfdla_status cfdla_batch_edge_gradient_device(cfdla_batch b, const double *p, const double *lambda,
+                                              const double *explicit_dg, const double *dg_dparam,
+                                              double *out);
+cfdla_status cfdla_batch_destroy(cfdla_batch b);
+
 /* ---- problem generators (host, deterministic) -------------------------------- */
+
+/* count planar channel networks on a grid: random spanning tree plus extra
+ * grid edges (degrees 2-4), n ~ U{nmin..nmax}, widths U[50, 500] um,
+ * lengths U[1, 10] mm; graph g uses SplitMix64 state seed_base + g.
+ * Two-phase: with edge_i NULL only the offsets are written. */
+cfdla_status cfdla_synth_networks(int64_t count, uint64_t seed_base, int64_t nmin, int64_t nmax,
+                                  int64_t *node_offset, int64_t *edge_offset, int64_t *edge_i,
+                                  int64_t *edge_j, double *width, double *length);
+/* Rectangular-duct conductance G = 1/R (rectangular.rs:95-157,209-245) and,
+ * if dg_dw is non-NULL, dG/dw at fixed h, L, mu. */
+cfdla_status cfdla_rect_conductance(int64_t n, const double *width, const double *height,
+                                    const double *length, double viscosity, double *g, double *dg_dw);
 
 /* SplitMix64 uniform[lo, hi) stream: out[i] for i < n from seed. */
 cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n, double *out);
+/* Periodic Gaussian random field on nx*ny cells (x fastest) with covariance
+ * exp(-r^2 / (2 corr_len^2)) in cells, normalised to sample mean 0 and
+ * sample standard deviation 1. */
+cfdla_status cfdla_synth_grf(int64_t nx, int64_t ny, double corr_len, uint64_t seed, double *out);
 /* Cell-centred pressure-correction Laplacian on nxc*nyc interior cells,
  * all fluid (crates/cfd-2d/src/pressure_velocity/correction.rs:273-343);
  * dirichlet[4] = {west, east, south, north}.  Two-phase: call with arrays NULL

@@ -1066,6 +1066,45 @@ CascadeOut network_cascade(const Csr &a, const double *b, double *x, double tol,
   return {2, r.iterations};
 }
 
+// Adjoint gradient of a scalar J(p, k) for the 2D variable-coefficient
+// operator of assemble_varcoef (nz = 1), k = exp(g):
+//   dJ/dg = k * (dJ/dk|explicit + lambda^T (db/dk - dA/dk p)).
+// Written as a scatter over faces (each face updates both cells), the
+// reverse of the device kernel's per-cell gather.  dir_mask bits 0..3 =
+// -x, +x, -y, +y.  No counterpart in the reference.
+void coef_grad_2d(i64 nx, i64 ny, double inv_h2, int dir_mask, const double *dir_val, const double *k,
+                  const double *p, const double *lam, const double *dJdk, double *dJdg) {
+  const i64 n = nx * ny;
+  std::vector<double> dk((size_t)n, 0.0);
+  for (i64 c = 0; c < n; ++c) dk[c] = dJdk ? dJdk[c] : 0.0;
+  auto face = [&](i64 a, i64 b) {
+    // coefficient c = 2 ka kb / (ka + kb) * inv_h2 couples a and b
+    double s = k[a] + k[b];
+    double dc_da = 2.0 * k[b] * k[b] / (s * s) * inv_h2;
+    double dc_db = 2.0 * k[a] * k[a] / (s * s) * inv_h2;
+    double w = (lam[a] - lam[b]) * (p[a] - p[b]);  // lambda^T dA/dc p
+    dk[a] -= dc_da * w;
+    dk[b] -= dc_db * w;
+  };
+  for (i64 y = 0; y < ny; ++y)
+    for (i64 x = 0; x < nx; ++x) {
+      i64 c = y * nx + x;
+      if (x + 1 < nx) face(c, c + 1);
+      if (y + 1 < ny) face(c, c + nx);
+    }
+  for (i64 y = 0; y < ny; ++y)
+    for (i64 x = 0; x < nx; ++x) {
+      i64 c = y * nx + x;
+      const bool on[4] = {x == 0, x == nx - 1, y == 0, y == ny - 1};
+      for (int f = 0; f < 4; ++f)
+        if (on[f] && (dir_mask >> f & 1)) {
+          // diagonal += k inv_h2, rhs += k inv_h2 value
+          dk[c] += lam[c] * inv_h2 * dir_val[f] - lam[c] * inv_h2 * p[c];
+        }
+    }
+  for (i64 c = 0; c < n; ++c) dJdg[c] = k[c] * dk[c];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1225,3 +1264,9 @@ int orc_network_cascade(void *h, const double *b, double *x, double tol, int64_t
 }
 
 }  // extern "C"
+
+extern "C" void orc_coef_grad_2d(int64_t nx, int64_t ny, double inv_h2, int32_t dir_mask, const double *dir_val,
+                                 const double *k, const double *p, const double *lam, const double *dJdk,
+                                 double *dJdg) {
+  coef_grad_2d(nx, ny, inv_h2, dir_mask, dir_val, k, p, lam, dJdk, dJdg);
+}

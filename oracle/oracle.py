@@ -317,3 +317,17 @@ def network_cascade(a: Csr, b, tol=1e-6, max_it=1000, use_cg=True):
     it = C.c_int64()
     path = lib().orc_network_cascade(a.h, b, x, tol, max_it, int(use_cg), C.byref(it))
     return x, path, it.value
+
+
+def coef_grad_2d(nx, ny, inv_h2, dir_mask, dir_val, k, p, lam, dJdk=None):
+    """Adjoint dJ/dg (k = exp g) of the 2D variable-coefficient operator."""
+    L = lib()
+    fn = L.orc_coef_grad_2d
+    fn.restype = None
+    fn.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_int32, F64P, F64P, F64P, F64P, C.c_void_p, F64P]
+    out = np.empty(nx * ny)
+    dj = None if dJdk is None else np.ascontiguousarray(dJdk, np.float64)
+    fn(nx, ny, inv_h2, dir_mask, np.ascontiguousarray(dir_val, np.float64), np.ascontiguousarray(k, np.float64),
+       np.ascontiguousarray(p, np.float64), np.ascontiguousarray(lam, np.float64),
+       None if dj is None else dj.ctypes.data, out)
+    return out

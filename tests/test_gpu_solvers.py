@@ -45,6 +45,37 @@ def test_spmv_integer_bit_exact(row_len):
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
 
 
+@pytest.mark.parametrize("shape", [(37, 1, 1), (64, 64, 1), (33, 17, 9), (128, 128, 128)])
+def test_spmv_stencil_integer_bit_exact(shape):
+    # stencil matrices take the sliced-ELL kernel (int16 column offsets);
+    # integer values and x make every partial sum exact
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    k = np.round(P.uniform(3, n, 1.0, 9.0))
+    rp, ci, v, _ = P.varcoef(nx, ny, nz, 1.0, k, 0, np.zeros(6))
+    v = np.round(v * 4.0)  # integer-valued, same pattern
+    x = np.round(P.uniform(4, n, -20.0, 20.0))
+    y = ca.CsrMatrix.from_parts(v, ci, rp, n, n).spmv(x)
+    assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n).spmv(x))
+
+
+def test_spmv_fp32_device_tensor():
+    import torch
+
+    rp, ci, v, b = P.config2(48)
+    n = 48 ** 3
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    ref = O.Csr.from_arrays(rp, ci, v, n).spmv(b.astype(np.float32).astype(np.float64))
+    x32 = torch.from_numpy(b.astype(np.float32)).cuda()
+    y32 = a.spmv(x32).cpu().numpy().astype(np.float64)
+    # FP32 values and products: relative error per row bounded by ~nnz_row * 2^-23
+    scale = np.abs(v).max() * np.abs(b).max()
+    assert np.max(np.abs(y32 - ref)) <= 8 * 7 * 2.0 ** -23 * scale
+    x64 = torch.from_numpy(b).cuda()
+    assert np.max(np.abs(a.spmv(x64).cpu().numpy() - O.Csr.from_arrays(rp, ci, v, n).spmv(b))) <= \
+        1e-13 * scale
+
+
 def test_spmv_known_answer(known):
     k = known["spmv_basic"]
     a = ca.CsrMatrix.from_parts(k["values"], k["col_indices"], k["row_ptr"], 3, 3)
