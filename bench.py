@@ -46,6 +46,8 @@ def main() -> None:
     ap.add_argument("--theta", type=float, default=0.25)
     ap.add_argument("--omega", type=float, default=0.8)
     ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--nu", type=int, default=2, help="pre- and post-smoothing sweeps")
+    ap.add_argument("--sa-omega", type=float, default=2.0 / 3.0, help="prolongator smoothing weight")
     ap.add_argument("--unsmoothed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-launches", type=int, default=50)
@@ -70,6 +72,8 @@ def main() -> None:
     nnz = ci.size
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
     cfg = ca.AMGConfig.smoothed_aggregation(args.theta, args.omega, smoothed=not args.unsmoothed)
+    cfg.pre_smooth_iterations = cfg.post_smooth_iterations = args.nu
+    cfg.smoothing_omega = args.sa_omega
     log("AMG setup")
     amg = ca.AlgebraicMultigrid(a, cfg)
     st = amg.statistics

@@ -104,6 +104,10 @@ SIGNATURES = {
                                                    _VP, _VP, _VP]),
     "cfdla_synth_uniform": (C.c_int, [C.c_uint64, C.c_double, C.c_double, C.c_int64, _VP]),
     "cfdla_synth_grf": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_uint64, _VP]),
+    "cfdla_problem_momentum_u": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, _VP, _VP, _I64P, _VP, _VP, _VP]),
+    "cfdla_problem_pressure_rhs": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                             _VP, _VP, C.c_int32, _VP]),
     "cfdla_problem_pressure_correction": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _I64P,
                                                     _VP, _VP, _VP]),
     "cfdla_problem_varcoef": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_double, _VP, C.c_int32, _VP, _I64P,

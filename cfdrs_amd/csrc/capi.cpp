@@ -141,6 +141,12 @@ bool is_symmetric(cfdla_matrix_s *m) {
   return sym;
 }
 
+}  // namespace
+
+void cfdla::set_last_error(const std::string &msg) { g_error = msg; }
+
+namespace {
+
 uint64_t splitmix64(uint64_t &state) {
   // SplitMix64 (Steele, Lea & Flood 2014)
   uint64_t z = (state += 0x9E3779B97F4A7C15ull);
@@ -500,17 +506,31 @@ cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n,
 cfdla_status cfdla_problem_pressure_correction(int64_t nxc, int64_t nyc, double dx, double dy,
                                                const int32_t *dirichlet, int64_t *nnz, int64_t *row_ptr,
                                                int64_t *col_indices, double *values) {
-  // Cell (i, j), i = x index (slow), j = y index (fast): row i*nyc + j.
+  // Cell (i, j), i = x index (slow), j = y index (fast): index i*nyc + j.
   // Neighbour terms and diagonal accumulate in the reference order
-  // west, east, south, north (correction.rs:291-334); columns sorted.
+  // west, east, south, north (correction.rs:291-334); columns sorted.  With
+  // no Dirichlet side the first cell is the pressure reference: its row and
+  // column are dropped and later rows shift down by one (correction.rs:225-246),
+  // giving nxc*nyc - 1 rows.
   return guarded([&] {
     CFDLA_REQUIRE(nxc >= 1 && nyc >= 1, CFDLA_ERR_INVALID_ARG, "grid must have at least one cell");
     require_ptr(dirichlet, "dirichlet");
     require_ptr(nnz, "nnz");
+    const bool pin = !(dirichlet[0] || dirichlet[1] || dirichlet[2] || dirichlet[3]);
+    CFDLA_REQUIRE(!pin || nxc * nyc > 1, CFDLA_ERR_INVALID_ARG, "a pinned system needs at least two cells");
     const double ix2 = 1.0 / (dx * dx), iy2 = 1.0 / (dy * dy);
+    auto map = [&](int64_t idx) -> int64_t { return pin ? (idx == 0 ? -1 : idx - 1) : idx; };
     int64_t count = 0;
     for (int64_t i = 0; i < nxc; ++i)
-      for (int64_t j = 0; j < nyc; ++j) count += 1 + (i > 0) + (i + 1 < nxc) + (j > 0) + (j + 1 < nyc);
+      for (int64_t j = 0; j < nyc; ++j) {
+        const int64_t idx = i * nyc + j;
+        if (map(idx) < 0) continue;
+        count += 1;
+        if (i > 0 && map(idx - nyc) >= 0) ++count;
+        if (i + 1 < nxc) ++count;
+        if (j > 0 && map(idx - 1) >= 0) ++count;
+        if (j + 1 < nyc) ++count;
+      }
     *nnz = count;
     if (!row_ptr) return;
     require_ptr(col_indices, "col_indices");
@@ -519,17 +539,18 @@ cfdla_status cfdla_problem_pressure_correction(int64_t nxc, int64_t nyc, double 
     row_ptr[0] = 0;
     for (int64_t i = 0; i < nxc; ++i)
       for (int64_t j = 0; j < nyc; ++j) {
-        const int64_t row = i * nyc + j;
+        const int64_t idx = i * nyc + j, row = map(idx);
+        if (row < 0) continue;
         double ap = 0.0;
         if (i > 0) ap += ix2; else if (dirichlet[0]) ap += ix2;
         if (i + 1 < nxc) ap += ix2; else if (dirichlet[1]) ap += ix2;
         if (j > 0) ap += iy2; else if (dirichlet[2]) ap += iy2;
         if (j + 1 < nyc) ap += iy2; else if (dirichlet[3]) ap += iy2;
-        if (i > 0) { col_indices[o] = row - nyc; values[o++] = -ix2; }
-        if (j > 0) { col_indices[o] = row - 1; values[o++] = -iy2; }
+        if (i > 0 && map(idx - nyc) >= 0) { col_indices[o] = map(idx - nyc); values[o++] = -ix2; }
+        if (j > 0 && map(idx - 1) >= 0) { col_indices[o] = map(idx - 1); values[o++] = -iy2; }
         col_indices[o] = row; values[o++] = ap;
-        if (j + 1 < nyc) { col_indices[o] = row + 1; values[o++] = -iy2; }
-        if (i + 1 < nxc) { col_indices[o] = row + nyc; values[o++] = -ix2; }
+        if (j + 1 < nyc) { col_indices[o] = map(idx + 1); values[o++] = -iy2; }
+        if (i + 1 < nxc) { col_indices[o] = map(idx + nyc); values[o++] = -ix2; }
         row_ptr[row + 1] = o;
       }
   });

@@ -32,6 +32,9 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::cfdla::Error((code), (msg));  \
   } while (0)
 
+// Thread-local message returned by cfdla_last_error.
+void set_last_error(const std::string &msg);
+
 // Library device/stream state.
 void init_device(int dev);
 void ensure_init();

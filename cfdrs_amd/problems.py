@@ -32,11 +32,13 @@ def uniform(seed: int, n: int, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
 
 
 def pressure_correction_2d(nxc: int, nyc: int, dx: float, dy: float, dirichlet=(0, 1, 0, 0)):
-    """(row_ptr, col, values) of the pressure-correction matrix."""
+    """(row_ptr, col, values) of the pressure-correction matrix; without a
+    Dirichlet side the first cell is pinned (nxc*nyc - 1 rows)."""
     d = np.asarray(dirichlet, np.int32)
     nnz = C.c_int64()
     N.call("cfdla_problem_pressure_correction", nxc, nyc, dx, dy, N.ptr(d), C.byref(nnz), None, None, None)
-    rp = np.empty(nxc * nyc + 1, np.int64)
+    rows = nxc * nyc - (0 if np.any(d) else 1)
+    rp = np.empty(rows + 1, np.int64)
     ci = np.empty(nnz.value, np.int64)
     v = np.empty(nnz.value)
     N.call("cfdla_problem_pressure_correction", nxc, nyc, dx, dy, N.ptr(d), C.byref(nnz), N.ptr(rp), N.ptr(ci),
@@ -79,6 +81,68 @@ def grf(nx: int, ny: int, corr_len: float = 16.0, seed: int = SEED_C5) -> np.nda
     out = np.empty(nx * ny)
     N.call("cfdla_synth_grf", nx, ny, corr_len, C.c_uint64(seed), N.ptr(out))
     return out
+
+
+SEED_C3 = 1003
+
+
+def cavity_velocity(n: int = 512, lid: float = 1.0, noise: float = 0.05, seed: int = SEED_C3):
+    """Node velocities of the lid-driven cavity on n x n nodes (unit square):
+    u = lid (y)^4, v = 0, plus noise * lid * U[-1, 1] on interior nodes; the
+    walls carry no-slip values and the lid (top row) u = lid."""
+    y = np.linspace(0.0, 1.0, n)
+    u = np.tile((lid * y ** 4)[:, None], (1, n)).ravel()
+    v = np.zeros(n * n)
+    r = uniform(seed, 2 * n * n).reshape(2, n, n)
+    interior = np.zeros((n, n), bool)
+    interior[1:-1, 1:-1] = True
+    u = u.reshape(n, n)
+    v = v.reshape(n, n)
+    u[interior] += noise * lid * r[0][interior]
+    v[interior] += noise * lid * r[1][interior]
+    u[0, :] = u[:, 0] = u[:, -1] = 0.0
+    u[-1, :] = lid
+    v[0, :] = v[-1, :] = v[:, 0] = v[:, -1] = 0.0
+    return u.ravel(), v.ravel()
+
+
+def momentum_u(n, dx, dy, dt, mu, rho, u, v):
+    """(row_ptr, col, values) of the first-order-upwind x-momentum operator."""
+    u = np.ascontiguousarray(u, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    nnz = C.c_int64()
+    N.call("cfdla_problem_momentum_u", n, n, dx, dy, dt, mu, rho, N.ptr(u), N.ptr(v), C.byref(nnz), None, None, None)
+    rp = np.empty(n * n + 1, np.int64)
+    ci = np.empty(nnz.value, np.int64)
+    vals = np.empty(nnz.value)
+    N.call("cfdla_problem_momentum_u", n, n, dx, dy, dt, mu, rho, N.ptr(u), N.ptr(v), C.byref(nnz), N.ptr(rp),
+           N.ptr(ci), N.ptr(vals))
+    return rp, ci, vals
+
+
+def pressure_rhs(n, dx, dy, rho, dt, u, v, pin=True):
+    out = np.empty((n - 2) * (n - 2) - (1 if pin else 0))
+    N.call("cfdla_problem_pressure_rhs", n, n, dx, dy, rho, dt, N.ptr(np.ascontiguousarray(u, np.float64)),
+           N.ptr(np.ascontiguousarray(v, np.float64)), 1 if pin else 0, N.ptr(out))
+    return out
+
+
+def config3(n: int = 512, re: float = 1000.0, dt: float = 0.05):
+    """Lid-driven cavity at Re = 1000 on n x n nodes: the x-momentum system
+    (right-hand side rho V / dt u_old) and the pinned pressure-correction
+    system on the (n-2)^2 interior cells."""
+    h = 1.0 / (n - 1)
+    mu, rho = 1.0 / re, 1.0
+    u, v = cavity_velocity(n)
+    mrp, mci, mv = momentum_u(n, h, h, dt, mu, rho, u, v)
+    b_m = rho * h * h / dt * u
+    b_m[np.r_[0:n, (n - 1) * n:n * n]] = u[np.r_[0:n, (n - 1) * n:n * n]]
+    for j in range(n):
+        b_m[j * n] = u[j * n]
+        b_m[j * n + n - 1] = u[j * n + n - 1]
+    prp, pci, pv = pressure_correction_2d(n - 2, n - 2, h, h, (0, 0, 0, 0))
+    b_p = pressure_rhs(n, h, h, rho, dt, u, v, pin=True)
+    return (mrp, mci, mv, b_m), (prp, pci, pv, b_p)
 
 
 def config2(n: int = 128):

@@ -306,6 +306,19 @@ cfdla_status cfdla_batch_destroy(cfdla_batch b);
 cfdla_status cfdla_synth_networks(int64_t count, uint64_t seed_base, int64_t nmin, int64_t nmax,
                                   int64_t *node_offset, int64_t *edge_offset, int64_t *edge_i,
                                   int64_t *edge_j, double *width, double *length);
+/* First-order-upwind + diffusion momentum operator for the x velocity on an
+ * nx*ny node grid (crates/cfd-2d/src/physics/momentum/coefficients.rs:199-277,406-414;
+ * assembly crates/cfd-2d/src/physics/momentum/solve.rs:264-308); boundary
+ * nodes are identity rows; u, v node fields indexed j*nx + i.  Two-phase. */
+cfdla_status cfdla_problem_momentum_u(int64_t nx, int64_t ny, double dx, double dy, double dt, double mu,
+                                      double rho, const double *u, const double *v, int64_t *nnz,
+                                      int64_t *row_ptr, int64_t *col_indices, double *values);
+/* Pressure-correction right-hand side -rho/dt div u* at the interior cells
+ * (correction.rs:345-356), rows as cfdla_problem_pressure_correction with
+ * pin_reference = 1 when no Dirichlet side is present. */
+cfdla_status cfdla_problem_pressure_rhs(int64_t nx, int64_t ny, double dx, double dy, double rho, double dt,
+                                        const double *u, const double *v, int32_t pin_reference,
+                                        double *rhs);
 /* Rectangular-duct conductance G = 1/R (rectangular.rs:95-157,209-245) and,
  * if dg_dw is non-NULL, dG/dw at fixed h, L, mu. */
 cfdla_status cfdla_rect_conductance(int64_t n, const double *width, const double *height,
@@ -319,8 +332,9 @@ cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n,
 cfdla_status cfdla_synth_grf(int64_t nx, int64_t ny, double corr_len, uint64_t seed, double *out);
 /* Cell-centred pressure-correction Laplacian on nxc*nyc interior cells,
  * all fluid (crates/cfd-2d/src/pressure_velocity/correction.rs:273-343);
- * dirichlet[4] = {west, east, south, north}.  Two-phase: call with arrays NULL
- * to get nnz. */
+ * dirichlet[4] = {west, east, south, north}.  Without a Dirichlet side the
+ * first cell is pinned as in correction.rs:225-246 and the system has
+ * nxc*nyc - 1 rows.  Two-phase: call with arrays NULL to get nnz. */
 cfdla_status cfdla_problem_pressure_correction(int64_t nxc, int64_t nyc, double dx, double dy,
                                                const int32_t *dirichlet, int64_t *nnz,
                                                int64_t *row_ptr, int64_t *col_indices,

@@ -854,23 +854,35 @@ Report gmres(const Csr &a, const Prec &M, const double *b, double *x, double tol
 // crates/cfd-2d/src/pressure_velocity/correction.rs:273-343.  nxc, nyc are
 // interior cell counts (the reference's nx-2, ny-2); row index
 // (i-1)*(ny-2)+(j-1) with i the x index.  dirichlet = {west, east, south, north}.
+// Without any Dirichlet side the first cell is pinned as the pressure
+// reference: system size n - 1, map_index drops it (correction.rs:225-246).
 Csr assemble_pressure_correction(i64 nxc, i64 nyc, double dx, double dy, const int dirichlet[4]) {
   i64 nx = nxc + 2, ny = nyc + 2, n = nxc * nyc;
+  bool has_dirichlet = dirichlet[0] || dirichlet[1] || dirichlet[2] || dirichlet[3];
+  i64 ref = has_dirichlet ? -1 : 0;
+  i64 size = has_dirichlet ? n : n - 1;
+  auto map_index = [&](i64 idx) -> i64 {
+    if (ref < 0) return idx;
+    if (idx == ref) return -1;
+    return idx < ref ? idx : idx - 1;
+  };
   double dx2 = 1.0 / (dx * dx), dy2 = 1.0 / (dy * dy);
-  Builder bd(n, n);
+  Builder bd(size, size);
   for (i64 i = 1; i < nx - 1; ++i)
     for (i64 j = 1; j < ny - 1; ++j) {
       i64 idx = (i - 1) * (ny - 2) + (j - 1);
+      if (idx == ref) continue;
+      i64 row = map_index(idx);
       double ap = 0.0;
-      if (i > 1) { ap += dx2; bd.add(idx, idx - (ny - 2), -dx2); }
+      if (i > 1) { ap += dx2; i64 c = map_index(idx - (ny - 2)); if (c >= 0) bd.add(row, c, -dx2); }
       else if (i == 1 && dirichlet[0]) ap += dx2;
-      if (i < nx - 2) { ap += dx2; bd.add(idx, idx + (ny - 2), -dx2); }
+      if (i < nx - 2) { ap += dx2; i64 c = map_index(idx + (ny - 2)); if (c >= 0) bd.add(row, c, -dx2); }
       else if (i == nx - 2 && dirichlet[1]) ap += dx2;
-      if (j > 1) { ap += dy2; bd.add(idx, idx - 1, -dy2); }
+      if (j > 1) { ap += dy2; i64 c = map_index(idx - 1); if (c >= 0) bd.add(row, c, -dy2); }
       else if (j == 1 && dirichlet[2]) ap += dy2;
-      if (j < ny - 2) { ap += dy2; bd.add(idx, idx + 1, -dy2); }
+      if (j < ny - 2) { ap += dy2; i64 c = map_index(idx + 1); if (c >= 0) bd.add(row, c, -dy2); }
       else if (j == ny - 2 && dirichlet[3]) ap += dy2;
-      bd.add(idx, idx, ap);
+      bd.add(row, row, ap);
     }
   return bd.build();
 }

@@ -61,7 +61,28 @@ def test_synth_uniform_reproducible_and_in_range():
     assert not np.array_equal(a, P.uniform(43, 10000))
 
 
-@pytest.mark.parametrize("n,dirichlet", [(16, (0, 1, 0, 0)), (7, (1, 0, 1, 1)), (1, (0, 0, 0, 0))])
+def test_momentum_assembly_matches_oracle():
+    n = 40
+    u, v = P.cavity_velocity(n, seed=7)
+    h = 1.0 / (n - 1)
+    rp, ci, vals = P.momentum_u(n, h, h, 0.05, 1e-3, 1.0, u, v)
+    o = O.assemble_momentum_u(n, n, h, h, 0.05, 1e-3, 1.0, u, v).arrays()
+    assert np.array_equal(rp, o[0]) and np.array_equal(ci, o[1]) and np.array_equal(vals, o[2])
+
+
+def test_pressure_rhs_vanishes_for_divergence_free_field():
+    # u = d psi/dy, v = -d psi/dx with psi = x^2 y + 3 x y^2: central differences are exact
+    n = 30
+    h = 1.0 / (n - 1)
+    x, y = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))  # [j, i]
+    u = (x ** 2 + 6 * x * y).ravel()
+    v = -(2 * x * y + 3 * y ** 2).ravel()
+    assert np.max(np.abs(P.pressure_rhs(n, h, h, 1.0, 0.1, u, v))) < 1e-10
+    assert P.pressure_rhs(n, h, h, 1.0, 0.1, u, v, pin=False).size == (n - 2) ** 2
+
+
+@pytest.mark.parametrize("n,dirichlet", [(16, (0, 1, 0, 0)), (7, (1, 0, 1, 1)), (1, (0, 0, 0, 0)),
+                                         (9, (0, 0, 0, 0))])
 def test_pressure_correction_assembly_matches_oracle(n, dirichlet):
     h = 1.0 / n
     rp, ci, v = P.pressure_correction_2d(n, n + 3, h, h * 0.5, dirichlet)
