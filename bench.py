@@ -7,9 +7,11 @@ residual of 1e-10.
 A step is one solve from x = 0 with the matrix, hierarchy and right-hand side
 resident in HBM.  ``value`` is the FP32-inner / FP64-refined time-to-1e-10 in
 milliseconds; the plain FP64 time, their ratio and the SpMV bandwidth are
-reported beside it.  ``roofline`` is the FP64 CSR SpMV (the dominant kernel
-family), timed with HIP events on the library stream; ``cpu_baseline`` is
-the CPU FP64 oracle solving the same system on one host core.
+reported beside it.  ``roofline`` is the FP64 y = A x of the fine operator
+in its device format (64-row sliced ELL with int16 column offsets for this
+stencil), timed with HIP events on the stream it is launched on, against the
+bytes that format must move; ``cpu_baseline`` is the CPU FP64 oracle solving
+the same system on one host core.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -31,10 +33,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n: int, nnz: int, s: int) -> int:
-    # CSR with int32 indices: values + columns per nonzero, row pointer +
-    # x read + y written per row, one extra row pointer
-    return nnz * (s + 4) + n * (2 * s + 4) + 4
+def spmv_bytes(n: int, nnz: int, s: int, storage: dict) -> int:
+    """Bytes one y = A x must move in the matrix's device format, x read once
+    and y written once (s = bytes per value)."""
+    if storage["format"] == "sell64":
+        # padded slots: value + int16 column offset; one slice pointer per slice
+        return storage["stored_entries"] * (s + 2) + (storage["units"] + 1) * 4 + 2 * n * s
+    # CSR: value + int32 column per nonzero, row pointers, row-block bounds
+    return nnz * (s + 4) + (n + 1) * 4 + (storage["units"] + 1) * 4 + 2 * n * s
 
 
 def main() -> None:
@@ -48,6 +54,7 @@ def main() -> None:
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--nu", type=int, default=2, help="pre- and post-smoothing sweeps")
     ap.add_argument("--sa-omega", type=float, default=2.0 / 3.0, help="prolongator smoothing weight")
+    ap.add_argument("--min-coarse", type=int, default=None, help="stop coarsening at this many rows")
     ap.add_argument("--unsmoothed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-launches", type=int, default=50)
@@ -74,6 +81,8 @@ def main() -> None:
     cfg = ca.AMGConfig.smoothed_aggregation(args.theta, args.omega, smoothed=not args.unsmoothed)
     cfg.pre_smooth_iterations = cfg.post_smooth_iterations = args.nu
     cfg.smoothing_omega = args.sa_omega
+    if args.min_coarse is not None:
+        cfg.min_coarse_size = args.min_coarse
     log("AMG setup")
     amg = ca.AlgebraicMultigrid(a, cfg)
     st = amg.statistics
@@ -116,6 +125,9 @@ def main() -> None:
     # SpMV roofline: FP64 and FP32 CSR SpMV on the fine operator, HIP events
     # recorded on the stream the library launches on (torch's current stream).
     spmv = {}
+    storage = a.storage()
+    kernel = (f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)"
+              if storage["format"] == "sell64" else f"k_csr_rb ({storage['units']} row blocks)")
     for dt, s in ((torch.float64, 8), (torch.float32, 4)):
         xv = torch.from_numpy(b_host).to(dev).to(dt)
         yv = torch.empty_like(xv)
@@ -128,13 +140,13 @@ def main() -> None:
         e1.record()
         e1.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
-        nbytes = spmv_bytes(n, nnz, s)
+        nbytes = spmv_bytes(n, nnz, s, storage)
         spmv[s] = dict(us=us, bytes=nbytes, gbs=nbytes / (us * 1e-6) / 1e9)
         log(f"SpMV fp{8 * s}: {us:.2f} us, {nbytes / 1e6:.2f} MB algorithmic, {spmv[s]['gbs']:.1f} GB/s")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rp, ci, v, b_host, n, args)
+        cpu = cpu_baseline(rp, ci, v, b_host, n, args, cfg)
 
     t64, tref = results["FP64"]["ms"], results["FP32_REFINED"]["ms"]
     line = {
@@ -166,8 +178,10 @@ def main() -> None:
         "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_csr<8,double,EPI_Y> (FP64 CSR SpMV, 8 lanes per row)",
-                     "avg_us": round(spmv[8]["us"], 3), "algorithmic_bytes": spmv[8]["bytes"]},
+                     "kernel": "FP64 y = A x, " + kernel,
+                     "avg_us": round(spmv[8]["us"], 3), "algorithmic_bytes": spmv[8]["bytes"],
+                     "csr_equivalent_gbs": round(spmv_bytes(n, nnz, 8, {"format": "csr", "units": 0}) /
+                                                 (spmv[8]["us"] * 1e-6) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -183,7 +197,7 @@ def a_residual(a, b, x, r) -> float:
     return out.value
 
 
-def cpu_baseline(rp, ci, v, b_host, n, args):
+def cpu_baseline(rp, ci, v, b_host, n, args, cfg):
     """The CPU FP64 oracle solving the same system with the same AMG recipe
     on one host core: setup excluded, solve from x = 0 to 1e-10."""
     import numpy as np
@@ -193,9 +207,10 @@ def cpu_baseline(rp, ci, v, b_host, n, args):
     log("CPU oracle baseline (one core)")
     orc = O.Csr.from_arrays(rp, ci, v, n)
     t0 = time.perf_counter()
-    amg = O.Amg(orc, coarsening="three_phase", theta=args.theta,
-                interp="direct" if args.unsmoothed else "smoothed", smoother="jacobi", omega=args.omega,
-                coarse_solve="direct")
+    amg = O.Amg(orc, coarsening="three_phase", theta=cfg.strength_threshold,
+                interp="direct" if args.unsmoothed else "smoothed", smoother="jacobi",
+                omega=cfg.relaxation_factor, coarse_solve="direct", sa_omega=cfg.smoothing_omega,
+                min_coarse=cfg.min_coarse_size, pre=cfg.pre_smooth_iterations, post=cfg.post_smooth_iterations)
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     x, rep = O.solve(orc, b_host, kind="cg", prec="amg", amg=amg, tol_rel=1e-10, max_it=1000)

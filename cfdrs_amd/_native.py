@@ -84,6 +84,7 @@ SIGNATURES = {
     "cfdla_matrix_create_device": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _VP, _VP, _VP, C.POINTER(_VP)]),
     "cfdla_matrix_update_values": (C.c_int, [_VP, _VP]),
     "cfdla_matrix_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
+    "cfdla_matrix_storage": (C.c_int, [_VP, _I32P, _I64P, _I64P]),
     "cfdla_matrix_destroy": (C.c_int, [_VP]),
     "cfdla_spmv": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_spmv_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
@@ -137,6 +138,15 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise NativeLibraryMissing(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C cfdrs_amd/csrc")
+        # One HIP runtime per process: torch ships its own libamdhip64 /
+        # libhsa-runtime64.  Loaded first, they satisfy libcfdla's
+        # libamdhip64.so.7 dependency by soname, so both share one runtime
+        # (and torch's stream handles are valid here).  Loaded after
+        # libcfdla, torch would bring a second HSA runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -234,6 +234,16 @@ cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, i
   });
 }
 
+cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stored_entries, int64_t *units) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    const DevCsr &d = devmat(m);
+    if (format) *format = d.sell ? CFDLA_STORAGE_SELL64 : CFDLA_STORAGE_CSR_ROWBLOCK;
+    if (stored_entries) *stored_entries = d.sell ? d.sell_entries : d.nnz;
+    if (units) *units = d.sell ? d.nslices : d.nrb;
+  });
+}
+
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m) {
   return guarded([&] { delete m; });
 }

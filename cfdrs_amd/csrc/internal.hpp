@@ -97,13 +97,19 @@ struct HostCsr {
 // wavefront, one lane per row), entries stored column-major within a slice
 // so every load of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32)
 // value segment plus a 128 B offset segment, padded to the longest row of the
-// slice with kSellPad offsets.  The vector-CSR kernel serves everything else.
+// slice with kSellPad offsets.
+//
+// Every other matrix runs the row-block kernel: rows are packed in order into
+// blocks of at most kRbRows rows and kRbCap nonzeros (a longer row gets a
+// block of its own); rbp holds the nrb + 1 first rows of the blocks.
 constexpr int16_t kSellPad = INT16_MIN;
+constexpr int kRbCap = 2048;
+constexpr int kRbRows = 512;
 struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
-  int group = 8;  // lanes per row of the vector-CSR kernel (power of two, 1..64)
-  DevBuf<int32_t> rp, ci;
+  int nrb = 0;
+  DevBuf<int32_t> rp, ci, rbp;
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;
   bool sell = false;
@@ -125,11 +131,12 @@ template <> inline const float *DevCsr::sdinv<float>() const { return sdinv32.p;
 template <> inline const double *DevCsr::sell_vals<double>() const { return sell_v64.p; }
 template <> inline const float *DevCsr::sell_vals<float>() const { return sell_v32.p; }
 
-int pick_group(int64_t n, int64_t nnz);
+void build_row_blocks(const HostCsr &h, std::vector<int32_t> &rbp);
 
-// Device-resident scalars of one solve.  Reductions write red[]; the last
-// block of a reducing kernel then runs a finalize op (see kernels.hip) on
-// thread 0, so the Krylov coefficients never travel to the host.
+// Device-resident scalars of one solve.  A reducing kernel writes block
+// partials; the single-block finalize kernel launched after it sums them into
+// red[] and runs a finalize op on thread 0, so the Krylov coefficients never
+// travel to the host.
 struct Scal {
   double red[8];
   double rz, alpha, beta, thr2, rr, pq;

@@ -1,13 +1,12 @@
-// gfx950 kernels of libcfdla: vector-CSR SpMV with fused epilogues,
-// deterministic grid reductions whose last block advances the Krylov scalars
-// on the device, Krylov vector updates, Jacobi sweeps and the FP64 dense
-// coarse solve.
+// gfx950 kernels of libcfdla: CSR and sliced-ELL products with fused
+// epilogues, deterministic grid reductions whose finalize kernel advances the
+// Krylov scalars on the device, Krylov vector updates, Jacobi sweeps and the
+// FP64 dense coarse solve.
 //
 // Wave64 throughout.  Reductions: wave shuffle tree, then LDS across the 4
-// waves of a 256-thread block, block partials to global memory, and the last
-// block to arrive (atomic ticket) sums the partials in block order — the
-// result does not depend on block scheduling, so runs are bitwise
-// reproducible.
+// waves of a 256-thread block, one partial per block to global memory, and a
+// single-block kernel that sums the partials in block order — the result does
+// not depend on block scheduling, so runs are bitwise reproducible.
 #include <cmath>
 
 #include "internal.hpp"
@@ -152,52 +151,162 @@ void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done) {
                        check_done ? 1 : 0);
 }
 
-// ---- vector-CSR product ------------------------------------------------------
+// ---- row-block CSR product ----------------------------------------------------
+// The host packs rows in order into blocks of at most kRbRows rows and kRbCap
+// nonzeros.  A workgroup streams its block's nonzeros with coalesced,
+// independent loads (up to kRbCap / kBlock = 8 column/value pairs per thread,
+// all issued before the x gathers), parks the products in LDS, then sums each
+// row from LDS with 1..64 threads (the largest power of two that fits the
+// block's row count) and a shuffle tree.  Row lengths of 4..700 nonzeros (AMG
+// transfer operators and Galerkin products) all keep every lane busy with
+// 8 loads in flight.  A row longer than kRbCap is a block of its own and is
+// summed block-strided straight from memory.
+//
+// Blocks are dealt to the 8 XCDs round-robin; XCD q walks the q-th
+// contiguous range of row blocks, so neighbouring rows share its L2.
+//
 // y and dotw carry no __restrict__: the self dot products (r . r with
 // y = dotw = r) alias them, and the store of y[row] must precede the load of
 // dotw[row].
-template <int G, typename T, int EPI, bool DOT>
-__global__ __launch_bounds__(kBlock) void k_csr(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                                                const T *__restrict__ av, const T *__restrict__ x,
-                                                const T *__restrict__ b, const T *__restrict__ dinv, T omega,
-                                                T *y, const T *dotw, Scal *s, int check_done, int op,
-                                                double *partials, unsigned *counter) {
+// Epilogue operands are loaded before the row sum is known, so their loads
+// overlap the entry loads instead of following the reduction.
+template <typename T>
+struct EpiIn {
+  T b, d, x, y, w;
+};
+
+template <typename T, int EPI, bool DOT>
+__device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, const T *__restrict__ b,
+                                             const T *__restrict__ dinv, const T *y, const T *dotw) {
+  EpiIn<T> e{T(0), T(0), T(0), T(0), T(0)};
+  if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI) e.b = b[row];
+  if constexpr (EPI == EPI_JACOBI) {
+    e.d = dinv[row];
+    e.x = x[row];
+  }
+  if constexpr (EPI == EPI_ADD) e.y = y[row];
+  if constexpr (DOT) {
+    if (dotw != y) e.w = dotw[row];
+  }
+  return e;
+}
+
+template <typename T, int EPI, bool DOT>
+__device__ __forceinline__ void epi_store(const EpiIn<T> &e, int row, T sum, T omega, T *y, const T *dotw,
+                                          double &dacc) {
+  T out;
+  if constexpr (EPI == EPI_Y) out = sum;
+  else if constexpr (EPI == EPI_RESID) out = e.b - sum;
+  else if constexpr (EPI == EPI_JACOBI) out = e.x + omega * e.d * (e.b - sum);
+  else out = e.y + sum;
+  if constexpr (DOT) dacc += (double)(dotw == y ? out : e.w) * (double)out;
+  y[row] = out;
+}
+
+template <typename T, int EPI, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int *__restrict__ rbp, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, const T *__restrict__ av,
+                                                   const T *__restrict__ x, const T *__restrict__ b,
+                                                   const T *__restrict__ dinv, T omega, T *y, const T *dotw,
+                                                   Scal *s, int check_done, double *partials) {
   if (check_done && s->done) return;
-  constexpr int GPB = kBlock / G;
-  const int lane = threadIdx.x & (G - 1);
-  int row = blockIdx.x * GPB + threadIdx.x / G;
-  const int stride = gridDim.x * GPB;
+  constexpr int U = kRbCap / kBlock;
+  __shared__ T prod[kRbCap];
+  __shared__ T wsum[kBlock / 64];
+  const int tid = threadIdx.x;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
+  const int chunk = (nrb + 7) >> 3;
+  const int q0 = xcd * chunk, q1 = min(nrb, q0 + chunk);
   double dacc = 0.0;
-  for (; row < n; row += stride) {
-    const int rs = rp[row], re = rp[row + 1];
-    T sum = T(0);
-    for (int k = rs + lane; k < re; k += G) sum += av[k] * x[ci[k]];
+  for (int q = q0 + local; q < q1; q += per) {
+    const int r0 = rbp[q], r1 = rbp[q + 1];
+    const int e0 = rp[r0], ne = rp[r1] - e0;
+    const int nr = r1 - r0;
+    if (ne > kRbCap) {
+      // a single long row, summed block-strided from memory
+      const EpiIn<T> e = epi_load<T, EPI, DOT>(r0, x, b, dinv, y, dotw);
+      T acc = T(0);
+      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * x[ci[e0 + k]];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-    if (lane == 0) {
-      T out;
-      if constexpr (EPI == EPI_Y) out = sum;
-      else if constexpr (EPI == EPI_RESID) out = b[row] - sum;
-      else if constexpr (EPI == EPI_JACOBI) out = x[row] + omega * dinv[row] * (b[row] - sum);
-      else out = y[row] + sum;
-      if constexpr (DOT) dacc += (double)(dotw == y ? out : dotw[row]) * (double)out;
-      y[row] = out;
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if ((tid & 63) == 0) wsum[tid >> 6] = acc;
+      __syncthreads();
+      if (tid == 0) {
+        T sum = T(0);
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) sum += wsum[w];
+        epi_store<T, EPI, DOT>(e, r0, sum, omega, y, dotw, dacc);
+      }
+      __syncthreads();
+      continue;
     }
+    // reduction layout: tpr threads per row when the rows fit the block
+    const bool wide = nr <= kBlock;
+    int tpr = 1;
+    if (wide)
+      while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
+    const int lane = tid & (tpr - 1);
+    const bool own = wide && tid / tpr < nr;
+    const int rl = wide ? min(tid / tpr, nr - 1) : 0;
+    // Loads and gathers are branch-free (clamped indices; the surplus lanes
+    // re-read the last entry or row), so the 2U entry loads, the row bounds
+    // and epilogue operands, then the U gathers are all in flight together.
+    const int ks = rp[r0 + rl] - e0;
+    const int kr = rp[r0 + rl + 1] - e0;
+    const EpiIn<T> e = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, y, dotw);
+    int c[U];
+    T v[U], xv[U];
+    if (ne > 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = min(tid + u * kBlock, ne - 1);
+        c[u] = ci[e0 + k];
+        v[u] = av[e0 + k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = tid + u * kBlock;
+        if (k < ne) prod[k] = v[u] * xv[u];
+      }
+    }
+    __syncthreads();
+    if (wide) {
+      const int ke = own ? kr : ks;
+      T sum = T(0);
+      for (int k = ks + lane; k < ke; k += tpr) sum += prod[k];
+      for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (own && lane == 0) epi_store<T, EPI, DOT>(e, r0 + rl, sum, omega, y, dotw, dacc);
+    } else {
+      for (int r = tid; r < nr; r += kBlock) {
+        const EpiIn<T> er = epi_load<T, EPI, DOT>(r0 + r, x, b, dinv, y, dotw);
+        const int kb = rp[r0 + r] - e0, kq = rp[r0 + r + 1] - e0;
+        T sum = T(0);
+        for (int k = kb; k < kq; ++k) sum += prod[k];
+        epi_store<T, EPI, DOT>(er, r0 + r, sum, omega, y, dotw, dacc);
+      }
+    }
+    __syncthreads();
   }
   if constexpr (DOT) {
-    double v[1] = {dacc};
-    reduce_and_finalize<1>(v, s, op, partials, counter);
+    double dv[1] = {dacc};
+    block_partials<1>(dv, partials);
   }
 }
 
-template <int G, typename T, int EPI, bool DOT>
-void launch_csr_g(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
-                  Scal *s, bool check_done, int op, RedWs &ws) {
-  constexpr int GPB = kBlock / G;
-  int blocks = blocks_for(A.n, GPB);
-  hipLaunchKernelGGL((k_csr<G, T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.rp.p, A.ci.p,
-                     A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, op, ws.partials.p,
-                     ws.counter.p);
+inline int xcd_grid(int units, int units_per_block) {
+  int blocks = (units + units_per_block - 1) / units_per_block;
+  blocks = std::max(1, std::min(blocks, kMaxBlocks));
+  return (blocks + 7) & ~7;
+}
+
+template <typename T, int EPI, bool DOT>
+void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
+                   Scal *s, bool check_done, int op, RedWs &ws) {
+  const int blocks = xcd_grid(A.nrb, 1);
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb, A.rbp.p, A.rp.p,
+                     A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
@@ -228,19 +337,29 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *
     const int w = (sp[sl + 1] - base) >> 6;
     const short *dd = dl + base + lane;
     const T *vv = av + base + lane;
+    const EpiIn<T> e = epi_load<T, EPI, DOT>(row, x, b, dinv, y, dotw);
+    // Branch-free: indices clamped to the slice, padding gathers x[row] and
+    // is masked out of the sum, so all loads of a chunk are in flight at once.
     T sum = T(0);
-#pragma unroll 4
-    for (int k = 0; k < w; ++k) {
-      const int d = dd[k * 64];
-      if (d != kSellPad) sum += vv[k * 64] * x[row + d];
+    for (int k0 = 0; k0 < w; k0 += 8) {
+      int d[8];
+      bool ok[8];
+      T v[8], xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = min(k0 + u, w - 1);
+        d[u] = dd[k * 64];
+        v[u] = vv[k * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        ok[u] = k0 + u < w && d[u] != kSellPad;
+        xv[u] = x[row + (ok[u] ? d[u] : 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sum += ok[u] ? v[u] * xv[u] : T(0);
     }
-    T out;
-    if constexpr (EPI == EPI_Y) out = sum;
-    else if constexpr (EPI == EPI_RESID) out = b[row] - sum;
-    else if constexpr (EPI == EPI_JACOBI) out = x[row] + omega * dinv[row] * (b[row] - sum);
-    else out = y[row] + sum;
-    if constexpr (DOT) dacc += (double)(dotw == y ? out : dotw[row]) * (double)out;
-    y[row] = out;
+    epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc);
   }
   if constexpr (DOT) {
     double v[1] = {dacc};
@@ -251,9 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *
 template <typename T, int EPI, bool DOT>
 void launch_sell(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                  Scal *s, bool check_done, int op, RedWs &ws) {
-  int blocks = (A.nslices + kBlock / 64 - 1) / (kBlock / 64);
-  blocks = std::min(blocks, kMaxBlocks);
-  blocks = (blocks + 7) & ~7;
+  const int blocks = xcd_grid(A.nslices, kBlock / 64);
   hipLaunchKernelGGL((k_sell<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices, A.sell_ptr.p,
                      A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
                      ws.partials.p);
@@ -267,15 +384,7 @@ void launch_csr_e(const DevCsr &A, const T *x, const T *b, const T *dinv, double
     launch_sell<T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws);
     return;
   }
-  switch (A.group) {
-    case 1: launch_csr_g<1, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    case 2: launch_csr_g<2, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    case 4: launch_csr_g<4, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    case 8: launch_csr_g<8, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    case 16: launch_csr_g<16, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    case 32: launch_csr_g<32, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-    default: launch_csr_g<64, T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws); break;
-  }
+  launch_csr_rb<T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws);
 }
 
 // ---- reductions and vector updates ---------------------------------------------

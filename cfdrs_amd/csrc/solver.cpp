@@ -132,11 +132,19 @@ struct Poller {
 // ---------------------------------------------------------------------------
 // Device CSR
 // ---------------------------------------------------------------------------
-int pick_group(int64_t n, int64_t nnz) {
-  double avg = n > 0 ? (double)nnz / (double)n : 1.0;
-  int g = 1;
-  while (g < avg && g < 64) g *= 2;
-  return g;
+void build_row_blocks(const HostCsr &h, std::vector<int32_t> &rbp) {
+  rbp.assign(1, 0);
+  int64_t rows = 0, nz = 0;
+  for (int64_t r = 0; r < h.n; ++r) {
+    const int64_t len = h.rp[r + 1] - h.rp[r];
+    if (rows > 0 && (nz + len > kRbCap || rows == kRbRows)) {
+      rbp.push_back((int32_t)r);
+      rows = nz = 0;
+    }
+    rows += 1;
+    nz += len;
+  }
+  if (rows > 0) rbp.push_back((int32_t)h.n);
 }
 
 namespace {
@@ -197,10 +205,13 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   n = (int)h.n;
   m = (int)h.m;
   nnz = h.nnz();
-  group = pick_group(h.n, nnz);
   std::vector<int32_t> rp32((size_t)h.n + 1);
   for (int64_t i = 0; i <= h.n; ++i) rp32[i] = (int32_t)h.rp[i];
   rp.upload(rp32.data(), rp32.size());
+  std::vector<int32_t> blocks;
+  build_row_blocks(h, blocks);
+  nrb = (int)blocks.size() - 1;
+  rbp.upload(blocks.data(), blocks.size());
   ci.upload(h.ci.data(), h.ci.size());
   v64.upload(h.v.data(), h.v.size());
   std::vector<float> f;

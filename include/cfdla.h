@@ -169,6 +169,15 @@ cfdla_status cfdla_matrix_create_device(int64_t nrows, int64_t ncols, int64_t nn
 /* Replace the values of the same pattern (host array of nnz doubles). */
 cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values);
 cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+/* Device storage chosen for the matrix (uploads it if needed): *format is
+ * CFDLA_STORAGE_SELL64 (64-row slices, int16 column offsets; *stored_entries
+ * counts the padded slots) or CFDLA_STORAGE_CSR_ROWBLOCK (int32 columns;
+ * *stored_entries = nnz).  *units = slices or row blocks.  No counterpart in
+ * the reference, whose CSR lives in host memory; used to state the bytes a
+ * product moves. */
+#define CFDLA_STORAGE_CSR_ROWBLOCK 0
+#define CFDLA_STORAGE_SELL64 1
+cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stored_entries, int64_t *units);
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
 
 /* y = A x on host vectors.  Replaces cfd_math::sparse::spmv / try_spmv

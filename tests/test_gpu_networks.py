@@ -60,9 +60,11 @@ def test_batch_matches_dense(small, precision, wave_max):
         n0, n1 = nets.node_offset[s], nets.node_offset[s + 1]
         err = np.linalg.norm(x[n0:n1] - xs[n0:n1]) / np.linalg.norm(xs[n0:n1])
         assert err <= 1e-7, (s, err)
-    # reported norms agree with the host recomputation
+    # reported norms agree with the host recomputation; two evaluations of
+    # ||D(b - A x)|| / ||D b|| in different summation orders differ by about
+    # (row length) * eps * ||D(|b| + |A||x|)|| / ||D b||, i.e. ~1e-15 here
     rep_res = np.array([r.final_residual_norm / max(r.threshold / 1e-10, 1e-300) for r in reps])
-    assert np.allclose(rep_res, res, rtol=1e-6, atol=1e-18)
+    assert np.allclose(rep_res, res, rtol=1e-6, atol=1e-14)
 
 
 def test_batch_torch_device_and_warm_start(small):
@@ -104,11 +106,15 @@ def test_full_size_batch_properties():
     q = g * (x[i] - x[j])
     net = np.bincount(i, weights=-q, minlength=batch.n_rows) + np.bincount(j, weights=q, minlength=batch.n_rows)
     free = batch.is_dirichlet == 0
-    scale = np.bincount(i, weights=np.abs(q), minlength=batch.n_rows) + np.bincount(j, weights=np.abs(q),
-                                                                                     minlength=batch.n_rows)
+    # scale: the node's total conductance times the pressure range (the inlet
+    # pressure); a residual of 1e-10 in the row-equilibrated norm leaves
+    # |net| / scale near 1e-10, a dropped edge leaves G * dp / scale ~ 1e-3
+    scale = (np.bincount(i, weights=g, minlength=batch.n_rows) +
+             np.bincount(j, weights=g, minlength=batch.n_rows)) * NW.INLET_PRESSURE
     assert np.max(np.abs(net[free]) / np.maximum(scale[free], 1e-300)) <= 1e-8
-    # pressures bounded by the boundary values (discrete maximum principle)
-    assert x.min() >= -1e-9 and x.max() <= NW.INLET_PRESSURE * (1 + 1e-12)
+    # pressures bounded by the boundary values (discrete maximum principle),
+    # up to the 1e-7 relative solution error a 1e-10 residual allows
+    assert x.min() >= -1e-7 * NW.INLET_PRESSURE and x.max() <= NW.INLET_PRESSURE * (1 + 1e-7)
 
 
 def test_width_gradient_matches_finite_differences():

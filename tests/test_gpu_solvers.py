@@ -134,7 +134,7 @@ def test_amg_vcycle_matches_oracle(case):
     amg = ca.AlgebraicMultigrid(a, cfg)
     oamg = O.Amg(O.Csr.from_arrays(rp, ci, v, n), coarsening=case["coarsening"], interp=case["interp"],
                  smoother="jacobi", omega=case["omega"], coarse_solve=case["coarse"], sa_omega=cfg.smoothing_omega,
-                 theta=cfg.strength_threshold)
+                 theta=cfg.strength_threshold, min_coarse=cfg.min_coarse_size)
     r = P.uniform(77, n)
     z = np.empty(n)
     amg.apply_to(r, z)
