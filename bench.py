@@ -58,6 +58,10 @@ def main() -> None:
     ap.add_argument("--unsmoothed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-launches", type=int, default=50)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the secondary workloads (2D Poisson, cavity, networks, adjoint)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 device-to-device copies of 512 MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is described, not run",
@@ -122,12 +126,17 @@ def main() -> None:
     err = float(torch.linalg.norm(results["FP32_REFINED"]["x"] - results["FP64"]["x"]) /
                 torch.linalg.norm(results["FP64"]["x"]))
 
-    # SpMV roofline: FP64 and FP32 CSR SpMV on the fine operator, HIP events
+    # SpMV roofline: FP64 and FP32 y = A x on the fine operator, HIP events
     # recorded on the stream the library launches on (torch's current stream).
+    # "cold": each timed launch follows a 1 GiB scrub that evicts the matrix
+    # from L2 and the 256 MB Infinity Cache, as inside a solve, where the
+    # V-cycle streams other operators between two products; "warm": launches
+    # back to back, the matrix partly cache-resident.
     spmv = {}
     storage = a.storage()
     kernel = (f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)"
               if storage["format"] == "sell64" else f"k_csr_rb ({storage['units']} row blocks)")
+    scrub = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     for dt, s in ((torch.float64, 8), (torch.float32, 4)):
         xv = torch.from_numpy(b_host).to(dev).to(dt)
         yv = torch.empty_like(xv)
@@ -139,10 +148,33 @@ def main() -> None:
             a.spmv_into(xv, yv)
         e1.record()
         e1.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
+        warm = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
+        cold = 0.0
+        for _ in range(args.spmv_launches):
+            scrub.fill_(1.0)
+            e0.record()
+            a.spmv_into(xv, yv)
+            e1.record()
+            e1.synchronize()
+            cold += e0.elapsed_time(e1) * 1e3
+        cold /= args.spmv_launches
         nbytes = spmv_bytes(n, nnz, s, storage)
-        spmv[s] = dict(us=us, bytes=nbytes, gbs=nbytes / (us * 1e-6) / 1e9)
-        log(f"SpMV fp{8 * s}: {us:.2f} us, {nbytes / 1e6:.2f} MB algorithmic, {spmv[s]['gbs']:.1f} GB/s")
+        spmv[s] = dict(us=cold, bytes=nbytes, gbs=nbytes / (cold * 1e-6) / 1e9, warm_us=warm,
+                       warm_gbs=nbytes / (warm * 1e-6) / 1e9)
+        log(f"SpMV fp{8 * s}: cold {cold:.2f} us ({spmv[s]['gbs']:.1f} GB/s), warm {warm:.2f} us "
+            f"({spmv[s]['warm_gbs']:.1f} GB/s), {nbytes / 1e6:.2f} MB algorithmic")
+    del scrub
+
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 536870912 bytes read and written")
+        del src, dst
+
+    others = None if args.no_configs else secondary_configs(args)
 
     cpu = None
     if not args.no_cpu:
@@ -176,15 +208,125 @@ def main() -> None:
         "refined_vs_fp64_solution_rel_diff": err,
         "spmv_fp64_gbs": round(spmv[8]["gbs"], 1),
         "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
+        "spmv_fp64_warm_gbs": round(spmv[8]["warm_gbs"], 1),
+        "spmv_fp32_warm_gbs": round(spmv[4]["warm_gbs"], 1),
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "FP64 y = A x, " + kernel,
-                     "avg_us": round(spmv[8]["us"], 3), "algorithmic_bytes": spmv[8]["bytes"],
+                     "avg_us": round(spmv[8]["us"], 3), "timing": "cold caches (1 GiB scrub before each launch)",
+                     "algorithmic_bytes": spmv[8]["bytes"],
                      "csr_equivalent_gbs": round(spmv_bytes(n, nnz, 8, {"format": "csr", "units": 0}) /
                                                  (spmv[8]["us"] * 1e-6) / 1e9, 1)},
         "cpu_baseline": cpu,
+        "other_workloads": others,
     }
     print(json.dumps(line), flush=True)
+
+
+def _time_solves(fn, steps: int, warmup: int = 1):
+    """Mean wall ms of fn() over steps calls (device synchronized), and its last result."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
+def secondary_configs(args) -> dict:
+    """Time-to-1e-10 on the four other workloads (inputs resident on the
+    device, setup excluded): 2D pressure-correction Jacobi-PCG (128^2), the
+    lid-driven cavity momentum (BiCGSTAB, GMRES(30)) and pressure (AMG-PCG)
+    systems (512^2), 4096 batched Kirchhoff networks, and the adjoint
+    gradient of the outlet flux on a 256^2 Gaussian-random-field
+    permeability."""
+    import numpy as np
+    import torch
+
+    import cfdrs_amd as ca
+    from cfdrs_amd import adjoint as ADJ
+    from cfdrs_amd import network as NW
+    from cfdrs_amd import problems as P
+
+    dev = torch.device("cuda:0")
+    steps = 3
+    out = {}
+
+    def krylov(a, b_host, prec, kind, precision, restart=30):
+        b = torch.from_numpy(b_host).to(dev)
+        x = torch.zeros_like(b)
+        cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+
+        def run():
+            x.zero_()
+            if kind == "cg":
+                return ca.cg_preconditioned(a, b, prec, x, cfg)
+            if kind == "bicgstab":
+                return ca.bicgstab_preconditioned(a, b, prec, x, cfg)
+            return ca.gmres_preconditioned(a, b, prec, x, cfg, restart)
+
+        ms, rep = _time_solves(run, steps)
+        return {"ms": round(ms, 3), "iterations": rep.iterations, "outer": rep.outer_iterations,
+                "termination": rep.termination.name, "solves_per_s": round(1e3 / ms, 1)}
+
+    # 2D pressure correction, 128^2, Jacobi-PCG
+    rp, ci, v, b1 = P.config1(128)
+    a1 = ca.CsrMatrix.from_parts(v, ci, rp, 128 * 128, 128 * 128)
+    j1 = ca.JacobiPreconditioner(a1)
+    out["pressure_correction_128x128_jacobi_pcg"] = {
+        prec.name: krylov(a1, b1, j1, "cg", prec) for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED)}
+    log(f"2D pressure correction: {out['pressure_correction_128x128_jacobi_pcg']}")
+
+    # lid-driven cavity 512^2: momentum (nonsymmetric) and pinned pressure correction
+    (mrp, mci, mv, bm), (prp, pci, pv, bp) = P.config3(512)
+    am = ca.CsrMatrix.from_parts(mv, mci, mrp, 512 * 512, 512 * 512)
+    jm = ca.JacobiPreconditioner(am)
+    npress = bp.size
+    ap_ = ca.CsrMatrix.from_parts(pv, pci, prp, npress, npress)
+    amg_p = ca.AlgebraicMultigrid(ap_, ca.AMGConfig.smoothed_aggregation())
+    cav = {}
+    for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED):
+        cav[f"momentum_bicgstab_jacobi_{prec.name}"] = krylov(am, bm, jm, "bicgstab", prec)
+        cav[f"momentum_gmres30_jacobi_{prec.name}"] = krylov(am, bm, jm, "gmres", prec)
+        cav[f"pressure_amg_pcg_{prec.name}"] = krylov(ap_, bp, amg_p, "cg", prec)
+    out["lid_driven_cavity_512x512_re1000"] = cav
+    log(f"cavity: {cav}")
+
+    # 4096 Kirchhoff networks of 50..2000 nodes, one persistent launch
+    nets = NW.synth_networks()
+    batch = NW.channel_batch(nets)
+    xb = torch.zeros(batch.n_rows, dtype=torch.float64, device=dev)
+    netres = {"systems": nets.count, "rows": int(batch.n_rows)}
+    for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED):
+        cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(50000).with_precision(prec)
+
+        def run():
+            xb.zero_()
+            return batch.solve(xb, cfg)[1]
+
+        ms, reps = _time_solves(run, steps)
+        netres[prec.name] = {"ms": round(ms, 3), "systems_per_s": round(nets.count * 1e3 / ms, 1),
+                             "max_iterations": int(max(r.iterations for r in reps)),
+                             "all_converged": all(r.termination == ca.Termination.Converged for r in reps)}
+    out["networks_4096"] = netres
+    log(f"networks: {netres}")
+
+    # adjoint gradient of the outlet flux, 256^2 GRF permeability
+    prob = ADJ.FluxProblem(256, 256)
+    g = P.grf(256, 256)
+
+    def grad():
+        return ADJ.flux_and_gradient(prob, g)
+
+    ms, _ = _time_solves(grad, steps)
+    out["adjoint_flux_gradient_256x256"] = {"ms_including_amg_setup": round(ms, 3)}
+    log(f"adjoint: {out['adjoint_flux_gradient_256x256']}")
+    return out
 
 
 def a_residual(a, b, x, r) -> float:
@@ -198,13 +340,15 @@ def a_residual(a, b, x, r) -> float:
 
 
 def cpu_baseline(rp, ci, v, b_host, n, args, cfg):
-    """The CPU FP64 oracle solving the same system with the same AMG recipe
-    on one host core: setup excluded, solve from x = 0 to 1e-10."""
+    """The CPU FP64 oracle solving the same system with the same AMG recipe:
+    setup excluded, solve from x = 0 to 1e-10, on all host threads (at most
+    16, the host share of one GPU) and on one thread."""
     import numpy as np
 
     from oracle import oracle as O
 
-    log("CPU oracle baseline (one core)")
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    log(f"CPU oracle baseline ({threads} threads, then 1)")
     orc = O.Csr.from_arrays(rp, ci, v, n)
     t0 = time.perf_counter()
     amg = O.Amg(orc, coarsening="three_phase", theta=cfg.strength_threshold,
@@ -212,15 +356,20 @@ def cpu_baseline(rp, ci, v, b_host, n, args, cfg):
                 omega=cfg.relaxation_factor, coarse_solve="direct", sa_omega=cfg.smoothing_omega,
                 min_coarse=cfg.min_coarse_size, pre=cfg.pre_smooth_iterations, post=cfg.post_smooth_iterations)
     setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    x, rep = O.solve(orc, b_host, kind="cg", prec="amg", amg=amg, tol_rel=1e-10, max_it=1000)
-    t = time.perf_counter() - t0
-    rel = float(np.linalg.norm(b_host - orc.spmv(x)) / np.linalg.norm(b_host))
-    log(f"CPU: setup {setup:.2f}s, solve {t:.2f}s, {rep.iterations} iterations, residual {rel:.2e}")
-    return {"value": round(t * 1e3, 2), "unit": "ms", "cores": 1, "kind": "port",
+    timings = {}
+    for nt in (threads, 1):
+        O.set_threads(nt)
+        t0 = time.perf_counter()
+        x, rep = O.solve(orc, b_host, kind="cg", prec="amg", amg=amg, tol_rel=1e-10, max_it=1000)
+        timings[nt] = time.perf_counter() - t0
+        rel = float(np.linalg.norm(b_host - orc.spmv(x)) / np.linalg.norm(b_host))
+        log(f"CPU ({nt} threads): setup {setup:.2f}s, solve {timings[nt]:.2f}s, {rep.iterations} iterations, "
+            f"residual {rel:.2e}")
+    O.set_threads(1)
+    return {"value": round(timings[threads] * 1e3, 2), "unit": "ms", "cores": threads, "kind": "port",
             "sample": f"full FP64 AMG-PCG solve to 1e-10 of the same {n}-unknown system ({rep.iterations} "
-                      f"iterations, relative residual {rel:.2e}); AMG setup {setup:.1f} s excluded",
-            "iterations": rep.iterations}
+                      f"iterations, relative residual {rel:.2e}); AMG setup {setup:.1f} s (one thread) excluded",
+            "single_thread_ms": round(timings[1] * 1e3, 2), "iterations": rep.iterations}
 
 
 if __name__ == "__main__":

@@ -240,7 +240,7 @@ cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stor
     const DevCsr &d = devmat(m);
     if (format) *format = d.sell ? CFDLA_STORAGE_SELL64 : CFDLA_STORAGE_CSR_ROWBLOCK;
     if (stored_entries) *stored_entries = d.sell ? d.sell_entries : d.nnz;
-    if (units) *units = d.sell ? d.nslices : d.nrb;
+    if (units) *units = d.sell ? d.nslices : d.nrb64;
   });
 }
 

@@ -100,16 +100,22 @@ struct HostCsr {
 // slice with kSellPad offsets.
 //
 // Every other matrix runs the row-block kernel: rows are packed in order into
-// blocks of at most kRbRows rows and kRbCap nonzeros (a longer row gets a
-// block of its own); rbp holds the nrb + 1 first rows of the blocks.
+// blocks of at most kRbRows rows and rb_cap<T>() nonzeros (a longer row gets
+// a block of its own).  A partition per value type (the FP32 blocks hold
+// twice the nonzeros in the same LDS) is stored as nrb + 1 (first row, first
+// nonzero) pairs.
 constexpr int16_t kSellPad = INT16_MIN;
-constexpr int kRbCap = 2048;
 constexpr int kRbRows = 512;
+template <typename T>
+constexpr int rb_cap() {
+  return sizeof(T) == 8 ? 2048 : 4096;
+}
 struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
-  int nrb = 0;
-  DevBuf<int32_t> rp, ci, rbp;
+  int nrb64 = 0, nrb32 = 0;
+  int sell_chunk = 8;  // entries per unrolled step of the sliced-ELL kernel (8 or 16)
+  DevBuf<int32_t> rp, ci, rb64, rb32;
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;
   bool sell = false;
@@ -123,6 +129,8 @@ struct DevCsr {
   template <typename T> const T *vals() const;
   template <typename T> const T *sdinv() const;
   template <typename T> const T *sell_vals() const;
+  template <typename T> const int *rb() const { return sizeof(T) == 8 ? rb64.p : rb32.p; }
+  template <typename T> int nrb() const { return sizeof(T) == 8 ? nrb64 : nrb32; }
 };
 template <> inline const double *DevCsr::vals<double>() const { return v64.p; }
 template <> inline const float *DevCsr::vals<float>() const { return v32.p; }
@@ -131,7 +139,7 @@ template <> inline const float *DevCsr::sdinv<float>() const { return sdinv32.p;
 template <> inline const double *DevCsr::sell_vals<double>() const { return sell_v64.p; }
 template <> inline const float *DevCsr::sell_vals<float>() const { return sell_v32.p; }
 
-void build_row_blocks(const HostCsr &h, std::vector<int32_t> &rbp);
+void build_row_blocks(const HostCsr &h, int cap, std::vector<int32_t> &rb);
 
 // Device-resident scalars of one solve.  A reducing kernel writes block
 // partials; the single-block finalize kernel launched after it sums them into
@@ -177,14 +185,17 @@ enum FinalizeOp : int {
   OP_BI_RR = 12,       // BiCGSTAB: rr = red0; iters++
 };
 
-enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3 };
+enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4 };
 
 // ---- kernel launchers (kernels.hip) -------------------------------------
-// Vector-CSR product with epilogue:
-//   EPI_Y      y = A x
-//   EPI_RESID  y = b - A x
-//   EPI_JACOBI y = x + omega * dinv .* (b - A x)
-//   EPI_ADD    y = y + A x
+// Sparse product (sliced-ELL or row-block CSR) with epilogue:
+//   EPI_Y       y = A x
+//   EPI_RESID   y = b - A x
+//   EPI_JACOBI  y = x + omega * dinv .* (b - A x)
+//   EPI_ADD     y = y + A x
+//   EPI_JACOBI0 two Jacobi sweeps from x = 0 in one pass: with
+//               x1 = omega * dinv .* b formed on the fly at every gathered
+//               column (x is not read), y = x1 + omega * dinv .* (b - A x1)
 // When dot_op != OP_NONE, also reduces sum_i dotw_i * y_i (dotw defaults to
 // x when null) and runs dot_op.  Skipped entirely when *s->done != 0 and
 // check_done is set.

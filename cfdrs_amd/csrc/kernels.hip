@@ -177,12 +177,16 @@ struct EpiIn {
 
 template <typename T, int EPI, bool DOT>
 __device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, const T *__restrict__ b,
-                                             const T *__restrict__ dinv, const T *y, const T *dotw) {
+                                             const T *__restrict__ dinv, T omega, const T *y, const T *dotw) {
   EpiIn<T> e{T(0), T(0), T(0), T(0), T(0)};
-  if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI) e.b = b[row];
+  if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI0) e.b = b[row];
   if constexpr (EPI == EPI_JACOBI) {
     e.d = dinv[row];
     e.x = x[row];
+  }
+  if constexpr (EPI == EPI_JACOBI0) {
+    e.d = dinv[row];
+    e.x = omega * e.d * e.b;
   }
   if constexpr (EPI == EPI_ADD) e.y = y[row];
   if constexpr (DOT) {
@@ -197,36 +201,58 @@ __device__ __forceinline__ void epi_store(const EpiIn<T> &e, int row, T sum, T o
   T out;
   if constexpr (EPI == EPI_Y) out = sum;
   else if constexpr (EPI == EPI_RESID) out = e.b - sum;
-  else if constexpr (EPI == EPI_JACOBI) out = e.x + omega * e.d * (e.b - sum);
+  else if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) out = e.x + omega * e.d * (e.b - sum);
   else out = e.y + sum;
   if constexpr (DOT) dacc += (double)(dotw == y ? out : e.w) * (double)out;
   y[row] = out;
 }
 
+// The gathered operand of a product: x[c], or for EPI_JACOBI0 the first
+// Jacobi iterate omega * dinv[c] * b[c], evaluated as k_jacobi_first does.
+template <typename T, int EPI>
+__device__ __forceinline__ T gather(int c, const T *__restrict__ x, const T *__restrict__ b,
+                                    const T *__restrict__ dinv, T omega) {
+  if constexpr (EPI == EPI_JACOBI0) return omega * dinv[c] * b[c];
+  else return x[c];
+}
+
 template <typename T, int EPI, bool DOT>
-__global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int *__restrict__ rbp, const int *__restrict__ rp,
+__global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const T *__restrict__ av,
                                                    const T *__restrict__ x, const T *__restrict__ b,
                                                    const T *__restrict__ dinv, T omega, T *y, const T *dotw,
                                                    Scal *s, int check_done, double *partials) {
   if (check_done && s->done) return;
-  constexpr int U = kRbCap / kBlock;
-  __shared__ T prod[kRbCap];
+  constexpr int CAP = rb_cap<T>();
+  constexpr int U = CAP / kBlock;
+  constexpr int RPT = kRbRows / kBlock;  // rows per thread when a block has more rows than threads
+  __shared__ T prod[CAP];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
   const int chunk = (nrb + 7) >> 3;
-  const int q0 = xcd * chunk, q1 = min(nrb, q0 + chunk);
+  const int q1 = min(nrb, xcd * chunk + chunk);
   double dacc = 0.0;
-  for (int q = q0 + local; q < q1; q += per) {
-    const int r0 = rbp[q], r1 = rbp[q + 1];
-    const int e0 = rp[r0], ne = rp[r1] - e0;
-    const int nr = r1 - r0;
-    if (ne > kRbCap) {
+  int q = xcd * chunk + local;
+  // block descriptors (first row, first nonzero) of this and the next block
+  int2 d0 = make_int2(0, 0), d1 = make_int2(0, 0);
+  if (q < q1) {
+    d0 = rb[q];
+    d1 = rb[q + 1];
+  }
+  while (q < q1) {
+    const int qn = q + per;
+    int2 n0 = d0, n1 = d1;
+    if (qn < q1) {  // prefetched while this block is processed
+      n0 = rb[qn];
+      n1 = rb[qn + 1];
+    }
+    const int r0 = d0.x, e0 = d0.y, nr = d1.x - d0.x, ne = d1.y - d0.y;
+    if (ne > CAP) {
       // a single long row, summed block-strided from memory
-      const EpiIn<T> e = epi_load<T, EPI, DOT>(r0, x, b, dinv, y, dotw);
+      const EpiIn<T> e = epi_load<T, EPI, DOT>(r0, x, b, dinv, omega, y, dotw);
       T acc = T(0);
-      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * x[ci[e0 + k]];
+      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * gather<T, EPI>(ci[e0 + k], x, b, dinv, omega);
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
       if ((tid & 63) == 0) wsum[tid >> 6] = acc;
@@ -238,56 +264,68 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int *__restric
         epi_store<T, EPI, DOT>(e, r0, sum, omega, y, dotw, dacc);
       }
       __syncthreads();
-      continue;
-    }
-    // reduction layout: tpr threads per row when the rows fit the block
-    const bool wide = nr <= kBlock;
-    int tpr = 1;
-    if (wide)
-      while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
-    const int lane = tid & (tpr - 1);
-    const bool own = wide && tid / tpr < nr;
-    const int rl = wide ? min(tid / tpr, nr - 1) : 0;
-    // Loads and gathers are branch-free (clamped indices; the surplus lanes
-    // re-read the last entry or row), so the 2U entry loads, the row bounds
-    // and epilogue operands, then the U gathers are all in flight together.
-    const int ks = rp[r0 + rl] - e0;
-    const int kr = rp[r0 + rl + 1] - e0;
-    const EpiIn<T> e = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, y, dotw);
-    int c[U];
-    T v[U], xv[U];
-    if (ne > 0) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = min(tid + u * kBlock, ne - 1);
-        c[u] = ci[e0 + k];
-        v[u] = av[e0 + k];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = tid + u * kBlock;
-        if (k < ne) prod[k] = v[u] * xv[u];
-      }
-    }
-    __syncthreads();
-    if (wide) {
-      const int ke = own ? kr : ks;
-      T sum = T(0);
-      for (int k = ks + lane; k < ke; k += tpr) sum += prod[k];
-      for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-      if (own && lane == 0) epi_store<T, EPI, DOT>(e, r0 + rl, sum, omega, y, dotw, dacc);
     } else {
-      for (int r = tid; r < nr; r += kBlock) {
-        const EpiIn<T> er = epi_load<T, EPI, DOT>(r0 + r, x, b, dinv, y, dotw);
-        const int kb = rp[r0 + r] - e0, kq = rp[r0 + r + 1] - e0;
-        T sum = T(0);
-        for (int k = kb; k < kq; ++k) sum += prod[k];
-        epi_store<T, EPI, DOT>(er, r0 + r, sum, omega, y, dotw, dacc);
+      // Reduction layout: with nr <= kBlock rows, tpr threads (a power of two
+      // up to 64) per row; otherwise up to RPT rows per thread.  Row bounds and
+      // epilogue operands are loaded first and all loads are branch-free
+      // (clamped indices; surplus lanes re-read the last entry or row), so
+      // they, the 2U entry loads and then the U gathers are in flight together.
+      const bool wide = nr <= kBlock;
+      int tpr = 1;
+      if (wide)
+        while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
+      const int lane = tid & (tpr - 1);
+      int ks[RPT], kr[RPT];
+      EpiIn<T> e[RPT];
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int rl = wide ? min(tid / tpr, nr - 1) : min(tid + j * kBlock, nr - 1);
+        if (j == 0 || !wide) {
+          ks[j] = rp[r0 + rl] - e0;
+          kr[j] = rp[r0 + rl + 1] - e0;
+          e[j] = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, omega, y, dotw);
+        }
       }
+      if (ne > 0) {
+        int c[U];
+        T v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = min(tid + u * kBlock, ne - 1);
+          c[u] = ci[e0 + k];
+          v[u] = av[e0 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = gather<T, EPI>(c[u], x, b, dinv, omega);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = tid + u * kBlock;
+          if (k < ne) prod[k] = v[u] * xv[u];
+        }
+      }
+      __syncthreads();
+      if (wide) {
+        const bool own = tid / tpr < nr;
+        const int ke = own ? kr[0] : ks[0];
+        T sum = T(0);
+        for (int k = ks[0] + lane; k < ke; k += tpr) sum += prod[k];
+        for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if (own && lane == 0) epi_store<T, EPI, DOT>(e[0], r0 + tid / tpr, sum, omega, y, dotw, dacc);
+      } else {
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          if (tid + j * kBlock < nr) {
+            T sum = T(0);
+            for (int k = ks[j]; k < kr[j]; ++k) sum += prod[k];
+            epi_store<T, EPI, DOT>(e[j], r0 + tid + j * kBlock, sum, omega, y, dotw, dacc);
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    d0 = n0;
+    d1 = n1;
+    q = qn;
   }
   if constexpr (DOT) {
     double dv[1] = {dacc};
@@ -304,9 +342,11 @@ inline int xcd_grid(int units, int units_per_block) {
 template <typename T, int EPI, bool DOT>
 void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                    Scal *s, bool check_done, int op, RedWs &ws) {
-  const int blocks = xcd_grid(A.nrb, 1);
-  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb, A.rbp.p, A.rp.p,
-                     A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p);
+  const int nrb = A.nrb<T>();
+  const int blocks = xcd_grid(nrb, 1);
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), nrb,
+                     reinterpret_cast<const int2 *>(A.rb<T>()), A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega,
+                     y, dotw, s, check_done ? 1 : 0, ws.partials.p);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
@@ -316,7 +356,7 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
 // so the slices are split into 8 contiguous ranges and XCD q walks range q:
 // the x gathers of neighbouring rows then hit the L2 of the XCD that owns
 // them.  The grid is a multiple of 8 blocks.
-template <typename T, int EPI, bool DOT>
+template <typename T, int EPI, bool DOT, int CH>
 __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *__restrict__ sp,
                                                  const short *__restrict__ dl, const T *__restrict__ av,
                                                  const T *__restrict__ x, const T *__restrict__ b,
@@ -337,27 +377,27 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *
     const int w = (sp[sl + 1] - base) >> 6;
     const short *dd = dl + base + lane;
     const T *vv = av + base + lane;
-    const EpiIn<T> e = epi_load<T, EPI, DOT>(row, x, b, dinv, y, dotw);
+    const EpiIn<T> e = epi_load<T, EPI, DOT>(row, x, b, dinv, omega, y, dotw);
     // Branch-free: indices clamped to the slice, padding gathers x[row] and
     // is masked out of the sum, so all loads of a chunk are in flight at once.
     T sum = T(0);
-    for (int k0 = 0; k0 < w; k0 += 8) {
-      int d[8];
-      bool ok[8];
-      T v[8], xv[8];
+    for (int k0 = 0; k0 < w; k0 += CH) {
+      int d[CH];
+      bool ok[CH];
+      T v[CH], xv[CH];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < CH; ++u) {
         const int k = min(k0 + u, w - 1);
         d[u] = dd[k * 64];
         v[u] = vv[k * 64];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < CH; ++u) {
         ok[u] = k0 + u < w && d[u] != kSellPad;
-        xv[u] = x[row + (ok[u] ? d[u] : 0)];
+        xv[u] = gather<T, EPI>(row + (ok[u] ? d[u] : 0), x, b, dinv, omega);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) sum += ok[u] ? v[u] * xv[u] : T(0);
+      for (int u = 0; u < CH; ++u) sum += ok[u] ? v[u] * xv[u] : T(0);
     }
     epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc);
   }
@@ -371,9 +411,14 @@ template <typename T, int EPI, bool DOT>
 void launch_sell(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                  Scal *s, bool check_done, int op, RedWs &ws) {
   const int blocks = xcd_grid(A.nslices, kBlock / 64);
-  hipLaunchKernelGGL((k_sell<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices, A.sell_ptr.p,
-                     A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
-                     ws.partials.p);
+  if (A.sell_chunk == 16)
+    hipLaunchKernelGGL((k_sell<T, EPI, DOT, 16>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices,
+                       A.sell_ptr.p, A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
+                       check_done ? 1 : 0, ws.partials.p);
+  else
+    hipLaunchKernelGGL((k_sell<T, EPI, DOT, 8>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices,
+                       A.sell_ptr.p, A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
+                       check_done ? 1 : 0, ws.partials.p);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
@@ -602,6 +647,7 @@ void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, 
     CFDLA_EPI_CASE(EPI_RESID)
     CFDLA_EPI_CASE(EPI_JACOBI)
     CFDLA_EPI_CASE(EPI_ADD)
+    CFDLA_EPI_CASE(EPI_JACOBI0)
     default:
       throw Error(CFDLA_ERR_INVALID_ARG, "unknown CSR epilogue");
   }

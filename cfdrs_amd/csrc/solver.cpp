@@ -132,19 +132,23 @@ struct Poller {
 // ---------------------------------------------------------------------------
 // Device CSR
 // ---------------------------------------------------------------------------
-void build_row_blocks(const HostCsr &h, std::vector<int32_t> &rbp) {
-  rbp.assign(1, 0);
+void build_row_blocks(const HostCsr &h, int cap, std::vector<int32_t> &rb) {
+  rb.assign({0, 0});
   int64_t rows = 0, nz = 0;
   for (int64_t r = 0; r < h.n; ++r) {
     const int64_t len = h.rp[r + 1] - h.rp[r];
-    if (rows > 0 && (nz + len > kRbCap || rows == kRbRows)) {
-      rbp.push_back((int32_t)r);
+    if (rows > 0 && (nz + len > cap || rows == kRbRows)) {
+      rb.push_back((int32_t)r);
+      rb.push_back((int32_t)h.rp[r]);
       rows = nz = 0;
     }
     rows += 1;
     nz += len;
   }
-  if (rows > 0) rbp.push_back((int32_t)h.n);
+  if (rows > 0) {
+    rb.push_back((int32_t)h.n);
+    rb.push_back((int32_t)h.rp[h.n]);
+  }
 }
 
 namespace {
@@ -160,11 +164,13 @@ void smoother_dinv(const HostCsr &h, std::vector<double> &d) {
 }  // namespace
 
 namespace {
-// Sliced-ELL layout when every offset fits in int16, rows are short and the
-// padding stays under 20 %.
+// Sliced-ELL layout for square matrices of at least 4096 rows when every
+// offset fits in int16, rows average at most 64 nonzeros and the padding
+// stays under 35 % (7-point stencils: ~0.2 %; the first smoothed-aggregation
+// Galerkin level of a 3D Poisson problem: ~28 %).
 bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t> &delta, std::vector<double> &val) {
-  if (h.n != h.m || h.n == 0) return false;
-  if ((double)h.nnz() > 16.0 * (double)h.n) return false;
+  if (h.n != h.m || h.n < 4096) return false;
+  if ((double)h.nnz() > 64.0 * (double)h.n) return false;
   const int64_t nslices = (h.n + 63) / 64;
   ptr.assign((size_t)nslices + 1, 0);
   int64_t total = 0;
@@ -181,7 +187,7 @@ bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t
     if (total >= INT32_MAX) return false;
     ptr[s + 1] = (int32_t)total;
   }
-  if ((double)total > 1.2 * (double)h.nnz() + 64.0 * 16.0) return false;
+  if ((double)total > 1.35 * (double)h.nnz() + 64.0 * 16.0) return false;
   delta.assign((size_t)total, kSellPad);
   val.assign((size_t)total, 0.0);
   for (int64_t s = 0; s < nslices; ++s)
@@ -209,9 +215,14 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   for (int64_t i = 0; i <= h.n; ++i) rp32[i] = (int32_t)h.rp[i];
   rp.upload(rp32.data(), rp32.size());
   std::vector<int32_t> blocks;
-  build_row_blocks(h, blocks);
-  nrb = (int)blocks.size() - 1;
-  rbp.upload(blocks.data(), blocks.size());
+  build_row_blocks(h, rb_cap<double>(), blocks);
+  nrb64 = (int)blocks.size() / 2 - 1;
+  rb64.upload(blocks.data(), blocks.size());
+  if (want_fp32) {
+    build_row_blocks(h, rb_cap<float>(), blocks);
+    nrb32 = (int)blocks.size() / 2 - 1;
+    rb32.upload(blocks.data(), blocks.size());
+  }
   ci.upload(h.ci.data(), h.ci.size());
   v64.upload(h.v.data(), h.v.size());
   std::vector<float> f;
@@ -236,6 +247,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   if (sell) {
     nslices = (int)(sp.size() - 1);
     sell_entries = sp.back();
+    sell_chunk = (double)sell_entries > 12.0 * 64.0 * (double)nslices ? 16 : 8;
     sell_ptr.upload(sp.data(), sp.size());
     sell_delta.upload(sd.data(), sd.size());
     sell_v64.upload(sv.data(), sv.size());
@@ -252,6 +264,15 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
 // ---------------------------------------------------------------------------
 // Preconditioners
 // ---------------------------------------------------------------------------
+void Precond::apply_dot(int n, const double *r, double *z, Scal *s, bool check_done, int dot_op) {
+  apply(r, z, s, check_done);
+  dot2<double>(n, r, z, nullptr, nullptr, s, check_done, dot_op, ctx().ws);
+}
+void Precond::apply_dot(int n, const float *r, float *z, Scal *s, bool check_done, int dot_op) {
+  apply(r, z, s, check_done);
+  dot2<float>(n, r, z, nullptr, nullptr, s, check_done, dot_op, ctx().ws);
+}
+
 JacobiPrecond::JacobiPrecond(const HostCsr &a) {
   n = (int)a.n;
   std::vector<double> inv((size_t)a.n, 1.0);
@@ -342,7 +363,7 @@ void AmgPrecond::update_values(const double *values) {
 }
 
 template <typename T>
-void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk) {
+void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) {
   DevLevel &L = lv[(size_t)l];
   RedWs &ws = ctx().ws;
   const int n = L.A.n;
@@ -351,6 +372,7 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk) {
   const T *dinv = L.A.sdinv<T>();
   if (coarsest && cfg.coarse_solver == 1) {
     dense_gemv<T>(n, coarse_inv.p, b, x, s, chk);
+    if (dot_op != OP_NONE) dot2<T>(n, b, x, nullptr, nullptr, s, chk, dot_op, ctx().ws);
     return;
   }
   // amg.rs:423-503 with Jacobi: pre-smoothing from the zero guess, then on
@@ -361,9 +383,18 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk) {
   // number of buffer flips after the first sweep; the iterate must end in x
   const int flips = (down > 0 ? down - 1 : 0) + up;
   int cur = (flips % 2 == 0) ? 0 : 1;
-  if (down > 0) jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
-  else fill_vec<T>(n, T(0), buf[cur]);
-  for (int k = 1; k < down; ++k) {
+  int k = 1;
+  if (down >= 2) {
+    // the zero-guess sweep and the next one in a single pass
+    csr_apply<T>(L.A, EPI_JACOBI0, b, b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
+    cur ^= 1;
+    k = 2;
+  } else if (down == 1) {
+    jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
+  } else {
+    fill_vec<T>(n, T(0), buf[cur]);
+  }
+  for (; k < down; ++k) {
     csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
     cur ^= 1;
   }
@@ -371,18 +402,25 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk) {
     DevLevel &C = lv[(size_t)l + 1];
     csr_apply<T>(L.A, EPI_RESID, buf[cur], b, nullptr, 0.0, L.r<T>(), s, chk, OP_NONE, nullptr, ws);
     csr_apply<T>(L.R, EPI_Y, L.r<T>(), nullptr, nullptr, 0.0, C.b<T>(), s, chk, OP_NONE, nullptr, ws);
-    vcycle<T>(l + 1, C.b<T>(), C.x<T>(), s, chk);
+    vcycle<T>(l + 1, C.b<T>(), C.x<T>(), s, chk, OP_NONE);
     csr_apply<T>(L.P, EPI_ADD, C.x<T>(), nullptr, nullptr, 0.0, buf[cur], s, chk, OP_NONE, nullptr, ws);
-    for (int k = 0; k < up; ++k) {
-      csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
+    for (int j = 0; j < up; ++j) {
+      // the last sweep writes x (the flip count fixes the parity) and, when
+      // asked, also reduces b . x and runs dot_op
+      const bool last = j + 1 == up && (cur ^ 1) == 0;
+      const int op = last ? dot_op : OP_NONE;
+      csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, op, op != OP_NONE ? b : nullptr,
+                   ws);
+      if (last) dot_op = OP_NONE;
       cur ^= 1;
     }
   }
   if (cur != 0) copy_vec<T>(n, buf[cur], x, s, chk);
+  if (dot_op != OP_NONE) dot2<T>(n, b, x, nullptr, nullptr, s, chk, dot_op, ws);
 }
 
-template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool);
-template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool);
+template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool, int);
+template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool, int);
 
 // ---------------------------------------------------------------------------
 // Krylov drivers
@@ -429,8 +467,7 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
   }
   const T *jd = jdinv<T>(M);
   auto start_direction = [&]() {
-    M->apply(r, z, s, false);
-    dot2<T>(n, r, z, nullptr, nullptr, s, false, OP_SET_RZ, ws);
+    M->apply_dot(n, r, z, s, false, OP_SET_RZ);
     copy_vec<T>(n, z, p, s, false);
   };
   start_direction();
@@ -448,8 +485,7 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
           cg_update<T>(n, p, q, x, r, jd, z, s, ws);
         } else {
           cg_update<T>(n, p, q, x, r, nullptr, nullptr, s, ws);
-          M->apply(r, z, s, true);
-          dot2<T>(n, r, z, nullptr, nullptr, s, true, OP_CG_BETA, ws);
+          M->apply_dot(n, r, z, s, true, OP_CG_BETA);
         }
         cg_direction<T>(n, z, p, s);
       }

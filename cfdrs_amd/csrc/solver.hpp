@@ -29,6 +29,9 @@ struct Precond {
   virtual int kind() const = 0;  // 0 identity, 1 Jacobi, 2 AMG
   virtual void apply(const double *r, double *z, Scal *s, bool check_done) = 0;
   virtual void apply(const float *r, float *z, Scal *s, bool check_done) = 0;
+  // z = M r, then r . z reduced and dot_op run on the device scalars.
+  virtual void apply_dot(int n, const double *r, double *z, Scal *s, bool check_done, int dot_op);
+  virtual void apply_dot(int n, const float *r, float *z, Scal *s, bool check_done, int dot_op);
   virtual const double *jacobi_dinv64() const { return nullptr; }
   virtual const float *jacobi_dinv32() const { return nullptr; }
   virtual void ensure_fp32() {}
@@ -74,13 +77,22 @@ struct AmgPrecond : Precond {
   int kind() const override { return 2; }
   void apply(const double *r, double *z, Scal *s, bool check_done) override {
     ensure_device();
-    vcycle<double>(0, r, z, s, check_done);
+    vcycle<double>(0, r, z, s, check_done, OP_NONE);
   }
   void apply(const float *r, float *z, Scal *s, bool check_done) override {
     ensure_device();
-    vcycle<float>(0, r, z, s, check_done);
+    vcycle<float>(0, r, z, s, check_done, OP_NONE);
   }
-  template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done);
+  // The dot product rides on the last post-smoothing sweep of level 0.
+  void apply_dot(int, const double *r, double *z, Scal *s, bool check_done, int dot_op) override {
+    ensure_device();
+    vcycle<double>(0, r, z, s, check_done, dot_op);
+  }
+  void apply_dot(int, const float *r, float *z, Scal *s, bool check_done, int dot_op) override {
+    ensure_device();
+    vcycle<float>(0, r, z, s, check_done, dot_op);
+  }
+  template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done, int dot_op);
 };
 
 struct SolveOut {

@@ -82,7 +82,13 @@ double csr_value(const Csr &a, i64 r, i64 c) {
   return 0.0;
 }
 
+// Threads for the solve-phase loops (row loops of products and sweeps,
+// vector updates, dot products).  1 (the default) keeps every loop serial and
+// every sum in index order; more threads reorder only the dot-product sums.
+int g_threads = 1;
+
 void spmv(const Csr &a, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
   for (i64 i = 0; i < a.n; ++i) {
     double s = 0.0;
     for (i64 k = a.rp[i]; k < a.rp[i + 1]; ++k) s += a.v[k] * x[a.ci[k]];
@@ -92,7 +98,13 @@ void spmv(const Csr &a, const double *x, double *y) {
 
 double dot(const std::vector<double> &a, const std::vector<double> &b) {
   double s = 0.0;
-  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  const i64 n = (i64)a.size();
+  if (g_threads > 1) {
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(g_threads)
+    for (i64 i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+  }
+  for (i64 i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
 double nrm2(const std::vector<double> &a) { return std::sqrt(dot(a, a)); }
@@ -435,6 +447,7 @@ void jacobi(const Csr &a, double *x, const double *b, i64 iters, double w) {
   std::vector<double> xo((size_t)a.n);
   for (i64 it = 0; it < iters; ++it) {
     std::copy(x, x + a.n, xo.begin());
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
     for (i64 i = 0; i < a.n; ++i) {
       double sum = 0.0, diag = 0.0;
       for (i64 k = a.rp[i]; k < a.rp[i + 1]; ++k) {
@@ -606,12 +619,14 @@ void v_cycle(const Amg &h, size_t l, const double *b, double *x) {
     smooth(h.cfg, lv.A, x, b, h.cfg.pre);
     std::vector<double> ax((size_t)n), r((size_t)n);
     spmv(lv.A, x, ax.data());
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
     for (i64 i = 0; i < n; ++i) r[i] = b[i] - ax[i];
     i64 nc = lv.R.n;
     std::vector<double> rc((size_t)nc), xc((size_t)nc, 0.0), e((size_t)n);
     spmv(lv.R, r.data(), rc.data());
     v_cycle(h, l + 1, rc.data(), xc.data());
     spmv(lv.P, xc.data(), e.data());
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
     for (i64 i = 0; i < n; ++i) x[i] += e[i];
     smooth(h.cfg, lv.A, x, b, h.cfg.post);
   } else if (h.cfg.coarse_solve == CS_DIRECT) {
@@ -694,6 +709,7 @@ Report pcg(const Csr &a, const Prec &M, const double *b, double *x, double tol_a
     double pq = dot(p, q);
     if (!(pq > 0.0)) { rep.iterations = it - 1; rep.final_residual_norm = rn; rep.termination = T_NONPOS; return rep; }
     double alpha = rz / pq;
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
     for (i64 i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
     rn = nrm2(r);
     rep.iterations = it;
@@ -712,6 +728,7 @@ Report pcg(const Csr &a, const Prec &M, const double *b, double *x, double tol_a
     if (rz_new == 0.0 || !std::isfinite(rz_new)) { rep.final_residual_norm = rn; rep.termination = T_BREAKDOWN; return rep; }
     double beta = rz_new / rz;
     rz = rz_new;
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
     for (i64 i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
   }
   rep.final_residual_norm = nrm2(true_residual(a, b, x));
@@ -1240,6 +1257,7 @@ void orc_solve(void *h, int kind, int prec, void *amg, const double *b, double *
   out->termination = r.termination;
 }
 int64_t orc_restart_ladder(int64_t requested) { return restart_ladder(requested); }
+void orc_set_threads(int32_t n) { g_threads = n < 1 ? 1 : n; }
 
 void orc_jacobi_inv_diag(void *h, double *out) {
   std::vector<double> d = jacobi_inv_diag(*static_cast<Csr *>(h));

@@ -46,6 +46,11 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
 
 
+def set_threads(n: int) -> None:
+    """OpenMP threads for the solve-phase loops (1 = serial, index-order sums)."""
+    lib().orc_set_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -78,6 +83,7 @@ def lib():
             "orc_solve": (None, [vp, C.c_int, C.c_int, vp, F64P, F64P, C.c_double, C.c_double,
                                  C.c_int64, C.c_int64, C.POINTER(Report)]),
             "orc_restart_ladder": (C.c_int64, [C.c_int64]),
+            "orc_set_threads": (None, [C.c_int32]),
             "orc_jacobi_inv_diag": (None, [vp, F64P]),
             "orc_dense_solve": (C.c_int, [vp, F64P, F64P]),
             "orc_assemble_pressure_correction": (vp, [C.c_int64, C.c_int64, C.c_double, C.c_double, I32P]),
