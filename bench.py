@@ -28,6 +28,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the roofline kernel (FP64 y = A x on the 128^3
+# operator), from two rocprofv3 counter passes over `bench.py --spmv-only`:
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors that make the counters
+# read 512 MiB each for a 512 MiB device copy
+# (profiles/k_sell_spmv_varcoef128.md).
+SPMV_FP64_TRAFFIC_BYTES = round((90927.0 * 2.0 + 16384.0) * 1024)
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -57,11 +64,14 @@ def main() -> None:
     ap.add_argument("--min-coarse", type=int, default=None, help="stop coarsening at this many rows")
     ap.add_argument("--unsmoothed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--spmv-launches", type=int, default=50)
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV roofline (and copy calibration): the run profiled for byte counters")
+    ap.add_argument("--spmv-launches", type=int, default=200)
+    ap.add_argument("--spmv-copies", type=int, default=4, help="device copies of the operator rotated in the SpMV timing")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary workloads (2D Poisson, cavity, networks, adjoint)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run 3 device-to-device copies of 512 MiB (scales rocprofv3 byte counters)")
+                    help="also time device-to-device copies of 512 MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is described, not run",
@@ -82,6 +92,16 @@ def main() -> None:
     rp, ci, v, b_host = P.config2(nn)
     nnz = ci.size
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    if args.spmv_only:
+        bind_torch_stream()
+        spmv = spmv_roofline(a, (v, ci, rp), n, nnz, b_host, args)
+        copy = copy_calibration(args.copy_calibration)
+        print(json.dumps({"spmv_fp64_us": round(spmv[8]["us"], 3), "spmv_fp64_gbs": round(spmv[8]["gbs"], 1),
+                          "spmv_fp32_us": round(spmv[4]["us"], 3), "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
+                          "algorithmic_bytes_fp64": spmv[8]["bytes"], "algorithmic_bytes_fp32": spmv[4]["bytes"],
+                          "copy_calibration": copy}), flush=True)
+        return
+
     cfg = ca.AMGConfig.smoothed_aggregation(args.theta, args.omega, smoothed=not args.unsmoothed)
     cfg.pre_smooth_iterations = cfg.post_smooth_iterations = args.nu
     cfg.smoothing_omega = args.sa_omega
@@ -126,53 +146,17 @@ def main() -> None:
     err = float(torch.linalg.norm(results["FP32_REFINED"]["x"] - results["FP64"]["x"]) /
                 torch.linalg.norm(results["FP64"]["x"]))
 
-    # SpMV roofline: FP64 and FP32 y = A x on the fine operator, HIP events
-    # recorded on the stream the library launches on (torch's current stream).
-    # "cold": each timed launch follows a 1 GiB scrub that evicts the matrix
-    # from L2 and the 256 MB Infinity Cache, as inside a solve, where the
-    # V-cycle streams other operators between two products; "warm": launches
-    # back to back, the matrix partly cache-resident.
-    spmv = {}
-    storage = a.storage()
-    kernel = (f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)"
-              if storage["format"] == "sell64" else f"k_csr_rb ({storage['units']} row blocks)")
-    scrub = torch.empty(1 << 27, dtype=torch.float64, device=dev)
-    for dt, s in ((torch.float64, 8), (torch.float32, 4)):
-        xv = torch.from_numpy(b_host).to(dev).to(dt)
-        yv = torch.empty_like(xv)
-        for _ in range(3):
-            a.spmv_into(xv, yv)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(args.spmv_launches):
-            a.spmv_into(xv, yv)
-        e1.record()
-        e1.synchronize()
-        warm = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
-        cold = 0.0
-        for _ in range(args.spmv_launches):
-            scrub.fill_(1.0)
-            e0.record()
-            a.spmv_into(xv, yv)
-            e1.record()
-            e1.synchronize()
-            cold += e0.elapsed_time(e1) * 1e3
-        cold /= args.spmv_launches
-        nbytes = spmv_bytes(n, nnz, s, storage)
-        spmv[s] = dict(us=cold, bytes=nbytes, gbs=nbytes / (cold * 1e-6) / 1e9, warm_us=warm,
-                       warm_gbs=nbytes / (warm * 1e-6) / 1e9)
-        log(f"SpMV fp{8 * s}: cold {cold:.2f} us ({spmv[s]['gbs']:.1f} GB/s), warm {warm:.2f} us "
-            f"({spmv[s]['warm_gbs']:.1f} GB/s), {nbytes / 1e6:.2f} MB algorithmic")
-    del scrub
+    # SpMV roofline: FP64 and FP32 y = A x on the fine operator, timed with
+    # HIP events recorded on the stream the library launches on (torch's
+    # current stream, bound above).  The launches run back to back and rotate
+    # over --spmv-copies device copies of the operator and of x/y, so each
+    # launch streams a matrix last touched copies-1 launches earlier and the
+    # working set (4 x 180 MB for FP64) exceeds the 256 MB Infinity Cache,
+    # as inside a solve, where the V-cycle streams the other levels between
+    # two fine-level products.
+    spmv = spmv_roofline(a, (v, ci, rp), n, nnz, b_host, args)
 
-    if args.copy_calibration:
-        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy calibration: 3 x 536870912 bytes read and written")
-        del src, dst
+    copy = copy_calibration(args.copy_calibration)
 
     others = None if args.no_configs else secondary_configs(args)
 
@@ -208,19 +192,92 @@ def main() -> None:
         "refined_vs_fp64_solution_rel_diff": err,
         "spmv_fp64_gbs": round(spmv[8]["gbs"], 1),
         "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
-        "spmv_fp64_warm_gbs": round(spmv[8]["warm_gbs"], 1),
-        "spmv_fp32_warm_gbs": round(spmv[4]["warm_gbs"], 1),
+        "spmv_fp32_us": round(spmv[4]["us"], 3),
+        "copy_calibration": copy,
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "FP64 y = A x, " + kernel,
-                     "avg_us": round(spmv[8]["us"], 3), "timing": "cold caches (1 GiB scrub before each launch)",
+                     "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_FP64_TRAFFIC_BYTES if nn == 128 else None,
+                     "kernel": "FP64 y = A x, " + spmv["kernel"],
+                     "avg_us": round(spmv[8]["us"], 3),
+                     "timing": f"HIP events over {args.spmv_launches} back-to-back launches rotating over "
+                               f"{args.spmv_copies} device copies of the operator",
                      "algorithmic_bytes": spmv[8]["bytes"],
-                     "csr_equivalent_gbs": round(spmv_bytes(n, nnz, 8, {"format": "csr", "units": 0}) /
-                                                 (spmv[8]["us"] * 1e-6) / 1e9, 1)},
+                     "csr_equivalent_bytes": spmv[8]["csr_bytes"],
+                     "csr_equivalent_gbs": round(spmv[8]["csr_bytes"] / (spmv[8]["us"] * 1e-6) / 1e9, 1)},
         "cpu_baseline": cpu,
         "other_workloads": others,
     }
     print(json.dumps(line), flush=True)
+
+
+def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
+    """Average duration and bandwidth of y = A x on the fine operator, FP64
+    and FP32.  K launches are enqueued back to back between two HIP events on
+    the library's stream, launch k on copy k % copies of the operator (and of
+    x and y), preceded by one untimed pass over all copies."""
+    import ctypes as C
+
+    import torch
+
+    import cfdrs_amd as ca
+    from cfdrs_amd import _native as N
+
+    v, ci, rp = parts
+    dev = torch.device("cuda:0")
+    mats = [a] + [ca.CsrMatrix.from_parts(v, ci, rp, n, n) for _ in range(args.spmv_copies - 1)]
+    storage = a.storage()
+    kernel = (f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)"
+              if storage["format"] == "sell64" else f"k_csr_rb ({storage['units']} row blocks)")
+    fn = N.lib().cfdla_spmv_device
+    out = {"kernel": kernel}
+    for dt, s, code in ((torch.float64, 8, N.DT_F64), (torch.float32, 4, N.DT_F32)):
+        xs = [torch.from_numpy(b_host).to(dev).to(dt) for _ in mats]
+        ys = [torch.empty_like(xs[0]) for _ in mats]
+        calls = [(m.handle, code, C.c_void_p(xv.data_ptr()), C.c_void_p(yv.data_ptr()))
+                 for m, xv, yv in zip(mats, xs, ys)]
+        for c in calls:
+            if fn(*c) != N.OK:
+                raise RuntimeError(N.lib().cfdla_last_error().decode())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(args.spmv_launches):
+            fn(*calls[k % len(calls)])
+        e1.record()
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
+        nbytes = spmv_bytes(n, nnz, s, storage)
+        csr = spmv_bytes(n, nnz, s, {"format": "csr", "units": -1})  # nnz (s + 4) + n (2 s + 4) + 4
+        out[s] = dict(us=us, bytes=nbytes, gbs=nbytes / (us * 1e-6) / 1e9, csr_bytes=csr)
+        log(f"SpMV fp{8 * s}: {us:.2f} us/launch ({out[s]['gbs']:.1f} GB/s of {nbytes / 1e6:.2f} MB in the device "
+            f"format; CSR model {csr / 1e6:.2f} MB -> {csr / (us * 1e-6) / 1e9:.1f} GB/s)")
+        del xs, ys
+    for m in mats[1:]:
+        m.close()
+    return out
+
+
+def copy_calibration(enabled: bool):
+    """Device-to-device copy of 512 MiB, 3 launches timed with HIP events: the
+    bandwidth a plain streaming kernel reaches, and the known-size launches
+    that scale the profiler's byte counters."""
+    if not enabled:
+        return None
+    import torch
+
+    src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / 3
+    nbytes = 2 * src.numel() * 8
+    log(f"copy calibration: {src.numel() * 8} bytes read and written per copy, {us:.1f} us each, "
+        f"{nbytes / (us * 1e-6) / 1e9:.1f} GB/s")
+    return {"bytes_per_launch": nbytes, "us": round(us, 2), "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)}
 
 
 def _time_solves(fn, steps: int, warmup: int = 1):
