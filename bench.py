@@ -43,6 +43,9 @@ def log(msg: str) -> None:
 def spmv_bytes(n: int, nnz: int, s: int, storage: dict) -> int:
     """Bytes one y = A x must move in the matrix's device format, x read once
     and y written once (s = bytes per value)."""
+    if storage["format"] == "dia":
+        # stored diagonals, no indices (the lower half of a symmetric matrix is re-read from the upper)
+        return storage["stored_entries"] * s + 2 * n * s
     if storage["format"] == "sell64":
         # padded slots: value + int16 column offset; one slice pointer per slice
         return storage["stored_entries"] * (s + 2) + (storage["units"] + 1) * 4 + 2 * n * s
@@ -58,7 +61,10 @@ def main() -> None:
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--theta", type=float, default=0.25)
     ap.add_argument("--omega", type=float, default=0.8)
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=1e-5,
+                    help="inner tolerance of restarted refinement (--reliable-interval -1)")
+    ap.add_argument("--reliable-interval", type=int, default=0,
+                    help="FP32-refined CG: iterations between FP64 residual replacements (0 = library default)")
     ap.add_argument("--nu", type=int, default=2, help="pre- and post-smoothing sweeps")
     ap.add_argument("--sa-omega", type=float, default=2.0 / 3.0, help="prolongator smoothing weight")
     ap.add_argument("--min-coarse", type=int, default=None, help="stop coarsening at this many rows")
@@ -119,8 +125,8 @@ def main() -> None:
 
     def solve(precision):
         x.zero_()
-        c = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(1000).with_precision(precision,
-                                                                                         args.inner_tol)
+        c = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(1000)
+             .with_precision(precision, args.inner_tol).with_reliable_interval(args.reliable_interval))
         return ca.cg_preconditioned(a, b, amg, x, c)
 
     results = {}
@@ -226,8 +232,9 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
     dev = torch.device("cuda:0")
     mats = [a] + [ca.CsrMatrix.from_parts(v, ci, rp, n, n) for _ in range(args.spmv_copies - 1)]
     storage = a.storage()
-    kernel = (f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)"
-              if storage["format"] == "sell64" else f"k_csr_rb ({storage['units']} row blocks)")
+    kernel = {"dia": f"k_dia (symmetric diagonal storage, {storage['units']} stored diagonals x {n} rows)",
+              "sell64": f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)",
+              "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)"}[storage["format"]]
     fn = N.lib().cfdla_spmv_device
     out = {"kernel": kernel}
     for dt, s, code in ((torch.float64, 8, N.DT_F64), (torch.float32, 4, N.DT_F32)):

@@ -39,7 +39,8 @@ class SolverConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("restart", C.c_int32), ("restart_ladder", C.c_int32),
                 ("precision", C.c_int32), ("tol_abs", C.c_double), ("tol_rel", C.c_double),
                 ("max_iter", C.c_int64), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("reliable_interval", C.c_int32), ("reserved", C.c_int32),
+                ("reliable_delta", C.c_double)]
 
 
 class Report(C.Structure):

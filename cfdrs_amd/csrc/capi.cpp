@@ -238,6 +238,12 @@ cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stor
   return guarded([&] {
     require_ptr(m, "matrix");
     const DevCsr &d = devmat(m);
+    if (d.dia) {
+      if (format) *format = CFDLA_STORAGE_DIA;
+      if (stored_entries) *stored_entries = (int64_t)d.dia_desc.nd * d.n;
+      if (units) *units = d.dia_desc.nd;
+      return;
+    }
     if (format) *format = d.sell ? CFDLA_STORAGE_SELL64 : CFDLA_STORAGE_CSR_ROWBLOCK;
     if (stored_entries) *stored_entries = d.sell ? d.sell_entries : d.nnz;
     if (units) *units = d.sell ? d.nslices : d.nrb64;

@@ -110,6 +110,23 @@ template <typename T>
 constexpr int rb_cap() {
   return sizeof(T) == 8 ? 2048 : 4096;
 }
+// Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
+// one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
+// 90 % nonzeros.  Values are stored diagonal-major, dia_v[k * n + i] =
+// a(i, i + off[k]) (zero where the diagonal leaves the matrix or the entry is
+// absent), so lane i of a wavefront reads consecutive addresses on every
+// diagonal and no column index is stored.  A symmetric matrix (equal offset
+// sets on both sides, a(i, j) == a(j, i) bit for bit) stores only the main
+// diagonal and the upper ones, off[0] = 0 < off[1] < ...; the lower entry
+// a(i, i - d) is read as the upper one at row i - d.  General matrices store
+// all diagonals, offsets ascending.
+constexpr int kDiaMax = 9;
+struct DiaDesc {
+  int nd = 0;           // stored diagonals
+  int sym = 0;          // 1: main + upper diagonals of a symmetric matrix
+  int off[kDiaMax] = {};
+};
+
 struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
@@ -125,10 +142,15 @@ struct DevCsr {
   DevBuf<int16_t> sell_delta; // col - row, kSellPad for padding
   DevBuf<double> sell_v64;
   DevBuf<float> sell_v32;
+  bool dia = false;
+  DiaDesc dia_desc;
+  DevBuf<double> dia_v64;
+  DevBuf<float> dia_v32;
   void upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell = true);
   template <typename T> const T *vals() const;
   template <typename T> const T *sdinv() const;
   template <typename T> const T *sell_vals() const;
+  template <typename T> const T *dia_vals() const;
   template <typename T> const int *rb() const { return sizeof(T) == 8 ? rb64.p : rb32.p; }
   template <typename T> int nrb() const { return sizeof(T) == 8 ? nrb64 : nrb32; }
 };
@@ -138,6 +160,9 @@ template <> inline const double *DevCsr::sdinv<double>() const { return sdinv64.
 template <> inline const float *DevCsr::sdinv<float>() const { return sdinv32.p; }
 template <> inline const double *DevCsr::sell_vals<double>() const { return sell_v64.p; }
 template <> inline const float *DevCsr::sell_vals<float>() const { return sell_v32.p; }
+template <> inline const double *DevCsr::dia_vals<double>() const { return dia_v64.p; }
+template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32.p; }
+
 
 void build_row_blocks(const HostCsr &h, int cap, std::vector<int32_t> &rb);
 
@@ -149,12 +174,16 @@ struct Scal {
   double red[8];
   double rz, alpha, beta, thr2, rr, pq;
   double rho, omega, rn0;
-  double hn;      // GMRES: norm of the newest Arnoldi vector
-  double pad[2];
-  int32_t done;   // 0 running, 1 residual estimate below threshold, 2 stopped (term), 3 see BiCGSTAB
+  double hn;          // GMRES: norm of the newest Arnoldi vector
+  double thr_final2;  // reliable-update CG: squared FP64 stopping threshold
+  double rr_true;     // reliable-update CG: ||b - A x||^2 at the latest FP64 residual replacement
+  int32_t done;   // 0 running, 1 residual estimate below threshold, 2 stopped (term), 3 see BiCGSTAB,
+                  // 4 FP64 residual below the threshold (reliable-update CG)
   int32_t iters;  // total iterations (Arnoldi steps for GMRES)
   int32_t term;
   int32_t kstep;  // GMRES: Arnoldi steps in the current cycle
+  int32_t nupd;   // reliable-update CG: FP64 residual replacements performed
+  int32_t restart_dir;  // reliable-update CG: the next OP_CG_BETA restarts the direction (beta = 0)
 };
 
 // Per-stream reduction workspace: block partials and an arrival counter that
@@ -166,7 +195,8 @@ struct RedWs {
 };
 
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxBlocks = 8192;  // capacity of the block partials (grid of any reducing kernel)
+constexpr int kGridCap = 2048;    // grid of the grid-stride vector and row-block kernels
 constexpr int kMaxRed = 8;
 
 enum FinalizeOp : int {
@@ -183,12 +213,25 @@ enum FinalizeOp : int {
   OP_BI_SNORM = 10,    // BiCGSTAB: ||s||^2 = red0
   OP_BI_OMEGA = 11,    // BiCGSTAB: omega = (t.s)/(t.t), red0 = t.s, red1 = t.t
   OP_BI_RR = 12,       // BiCGSTAB: rr = red0; iters++
+  OP_RELIABLE = 13,    // rr_true = rr = red0 (FP64 residual); done = 4 if rr <= thr_final2, else 0;
+                       // restart_dir if it exceeds 4x the recursive rr it replaces
 };
 
 enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4 };
 
 // ---- kernel launchers (kernels.hip) -------------------------------------
-// Sparse product (sliced-ELL or row-block CSR) with epilogue:
+// Diagonal-format product with epilogue (dia.hip); A.dia must hold and the
+// stored diagonal count must be one of the instantiated shapes (dia_supported).
+bool dia_supported(const DiaDesc &d);
+template <typename T, int EPI, bool DOT>
+void launch_dia(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
+                Scal *s, bool check_done, int op, RedWs &ws);
+// Single-block kernel that sums the block partials of a reducing kernel and
+// runs the finalize op (kernels.hip).
+void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done);
+int xcd_grid(int units, int units_per_block, int cap = kGridCap);
+
+// Sparse product (diagonal, sliced-ELL or row-block CSR) with epilogue:
 //   EPI_Y       y = A x
 //   EPI_RESID   y = b - A x
 //   EPI_JACOBI  y = x + omega * dinv .* (b - A x)
@@ -227,6 +270,10 @@ template <typename T>
 void fill_vec(int n, T value, T *dst);
 void f64_to_f32_scaled(int n, const double *src, double scale, float *dst);
 void axpy_f32_to_f64(int n, double a, const float *x, double *y);
+// Residual replacement of the reliable-update CG, skipped when check_done
+// and s->done != 0:  x += (double) xc, xc = 0  /  dst = (float) src.
+void fold_correction(int n, float *xc, double *x, Scal *s, bool check_done);
+void f64_to_f32(int n, const double *src, float *dst, Scal *s, bool check_done);
 
 // x = Ainv * b (dense row-major FP64 inverse), b/x of type T.
 template <typename T>
@@ -247,6 +294,7 @@ void axpy_scal(int n, const T *x, T *y, Scal *s, int which);  // y += s->{alpha}
 void scal_reset(Scal *s, double thr2);
 void scal_set_done(Scal *s, int value);
 void scal_bicg_restart(Scal *s);  // rho = alpha = omega = 1, done = 0
+void scal_set_final(Scal *s, double thr_final2);  // reliable-update CG stopping threshold
 
 // Coefficient gradient of the 2D variable-coefficient Poisson operator.
 void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, const double *dir_val,

@@ -9,6 +9,7 @@
 // not depend on block scheduling, so runs are bitwise reproducible.
 #include <cmath>
 
+#include "epilogue.hpp"
 #include "internal.hpp"
 #include "reduce.hpp"
 
@@ -50,7 +51,8 @@ __device__ void finalize(Scal *s, int op) {
         s->term = CFDLA_TERM_BREAKDOWN;
         s->done = 2;
       } else {
-        s->beta = rzn / s->rz;
+        s->beta = s->restart_dir ? 0.0 : rzn / s->rz;
+        s->restart_dir = 0;
         s->rz = rzn;
       }
     } break;
@@ -104,6 +106,24 @@ __device__ void finalize(Scal *s, int op) {
       double ts = s->red[0], tt = s->red[1];
       s->omega = tt > 0.0 ? ts / tt : 0.0;
     } break;
+    case OP_RELIABLE: {
+      // FP64 residual of the accumulated FP64 solution (reliable-update CG)
+      // A replaced residual far above the recursive one it replaces (the
+      // FP32 operator's rounding caps what one cycle can gain) would make
+      // beta = r.z / rz blow up and wreck the direction: restart it.
+      double rr = s->red[0];
+      const double rec = s->rr;
+      s->rr = rr;
+      s->rr_true = rr;
+      s->nupd += 1;
+      if (!isfinite(rr)) {
+        s->term = CFDLA_TERM_BREAKDOWN;
+        s->done = 2;
+      } else {
+        s->done = rr <= s->thr_final2 ? 4 : 0;
+        s->restart_dir = rr > 4.0 * rec ? 1 : 0;
+      }
+    } break;
     case OP_BI_RR: {
       double rr = s->red[0];
       s->rr = rr;
@@ -142,6 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Scal *s, int op, int nblock
   }
 }
 
+}  // namespace
+
 void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done) {
   if (nv == 1)
     hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, stream(), s, op, nblocks, ws.partials.p,
@@ -150,6 +172,15 @@ void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done) {
     hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kBlock), 0, stream(), s, op, nblocks, ws.partials.p,
                        check_done ? 1 : 0);
 }
+
+// Grid of a multiple of 8 blocks (one per XCD round-robin), at most cap.
+int xcd_grid(int units, int units_per_block, int cap) {
+  int blocks = (units + units_per_block - 1) / units_per_block;
+  blocks = std::max(1, std::min(blocks, cap));
+  return (blocks + 7) & ~7;
+}
+
+namespace {
 
 // ---- row-block CSR product ----------------------------------------------------
 // The host packs rows in order into blocks of at most kRbRows rows and kRbCap
@@ -165,57 +196,6 @@ void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done) {
 // Blocks are dealt to the 8 XCDs round-robin; XCD q walks the q-th
 // contiguous range of row blocks, so neighbouring rows share its L2.
 //
-// y and dotw carry no __restrict__: the self dot products (r . r with
-// y = dotw = r) alias them, and the store of y[row] must precede the load of
-// dotw[row].
-// Epilogue operands are loaded before the row sum is known, so their loads
-// overlap the entry loads instead of following the reduction.
-template <typename T>
-struct EpiIn {
-  T b, d, x, y, w;
-};
-
-template <typename T, int EPI, bool DOT>
-__device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, const T *__restrict__ b,
-                                             const T *__restrict__ dinv, T omega, const T *y, const T *dotw) {
-  EpiIn<T> e{T(0), T(0), T(0), T(0), T(0)};
-  if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI0) e.b = b[row];
-  if constexpr (EPI == EPI_JACOBI) {
-    e.d = dinv[row];
-    e.x = x[row];
-  }
-  if constexpr (EPI == EPI_JACOBI0) {
-    e.d = dinv[row];
-    e.x = omega * e.d * e.b;
-  }
-  if constexpr (EPI == EPI_ADD) e.y = y[row];
-  if constexpr (DOT) {
-    if (dotw != y) e.w = dotw[row];
-  }
-  return e;
-}
-
-template <typename T, int EPI, bool DOT>
-__device__ __forceinline__ void epi_store(const EpiIn<T> &e, int row, T sum, T omega, T *y, const T *dotw,
-                                          double &dacc) {
-  T out;
-  if constexpr (EPI == EPI_Y) out = sum;
-  else if constexpr (EPI == EPI_RESID) out = e.b - sum;
-  else if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) out = e.x + omega * e.d * (e.b - sum);
-  else out = e.y + sum;
-  if constexpr (DOT) dacc += (double)(dotw == y ? out : e.w) * (double)out;
-  y[row] = out;
-}
-
-// The gathered operand of a product: x[c], or for EPI_JACOBI0 the first
-// Jacobi iterate omega * dinv[c] * b[c], evaluated as k_jacobi_first does.
-template <typename T, int EPI>
-__device__ __forceinline__ T gather(int c, const T *__restrict__ x, const T *__restrict__ b,
-                                    const T *__restrict__ dinv, T omega) {
-  if constexpr (EPI == EPI_JACOBI0) return omega * dinv[c] * b[c];
-  else return x[c];
-}
-
 template <typename T, int EPI, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const T *__restrict__ av,
@@ -333,12 +313,6 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   }
 }
 
-inline int xcd_grid(int units, int units_per_block) {
-  int blocks = (units + units_per_block - 1) / units_per_block;
-  blocks = std::max(1, std::min(blocks, kMaxBlocks));
-  return (blocks + 7) & ~7;
-}
-
 template <typename T, int EPI, bool DOT>
 void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                    Scal *s, bool check_done, int op, RedWs &ws) {
@@ -425,6 +399,10 @@ void launch_sell(const DevCsr &A, const T *x, const T *b, const T *dinv, double 
 template <typename T, int EPI, bool DOT>
 void launch_csr_e(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                   Scal *s, bool check_done, int op, RedWs &ws) {
+  if (A.dia && A.dia_vals<T>() != nullptr) {
+    launch_dia<T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws);
+    return;
+  }
   if (A.sell && A.sell_vals<T>() != nullptr) {
     launch_sell<T, EPI, DOT>(A, x, b, dinv, omega, y, dotw, s, check_done, op, ws);
     return;
@@ -510,6 +488,21 @@ __global__ __launch_bounds__(kBlock) void k_axpy_f32_to_f64(int n, double a, con
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += a * (double)x[i];
 }
 
+__global__ __launch_bounds__(kBlock) void k_fold_correction(int n, float *__restrict__ xc, double *__restrict__ x,
+                                                            Scal *s, int check_done) {
+  if (check_done && s->done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] += (double)xc[i];
+    xc[i] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32(int n, const double *__restrict__ src,
+                                                       float *__restrict__ dst, Scal *s, int check_done) {
+  if (check_done && s->done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = (float)src[i];
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double *__restrict__ a,
                                                        const T *__restrict__ b, T *__restrict__ x, Scal *s,
@@ -578,18 +571,24 @@ __global__ __launch_bounds__(kBlock) void k_axpy_alpha(int n, const T *__restric
 __global__ void k_scal_reset(Scal *s, double thr2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     for (int i = 0; i < 8; ++i) s->red[i] = 0.0;
-    s->rz = s->beta = s->rr = s->pq = s->hn = 0.0;
+    s->rz = s->beta = s->rr = s->pq = s->hn = s->rr_true = 0.0;
     s->rho = s->omega = s->alpha = 1.0;
-    s->thr2 = thr2;
+    s->thr2 = s->thr_final2 = thr2;
     s->done = 0;
     s->iters = 0;
     s->kstep = 0;
+    s->nupd = 0;
+    s->restart_dir = 0;
     s->term = CFDLA_TERM_MAX_ITERATIONS;
   }
 }
 
 __global__ void k_scal_set_done(Scal *s, int value) {
   if (threadIdx.x == 0 && blockIdx.x == 0) s->done = value;
+}
+
+__global__ void k_scal_set_final(Scal *s, double thr_final2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->thr_final2 = thr_final2;
 }
 
 __global__ void k_scal_bicg_restart(Scal *s) {
@@ -719,6 +718,18 @@ void axpy_f32_to_f64(int n, double a, const float *x, double *y) {
   CFDLA_HIP(hipGetLastError());
 }
 
+void fold_correction(int n, float *xc, double *x, Scal *s, bool check_done) {
+  hipLaunchKernelGGL(k_fold_correction, dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, xc, x, s,
+                     check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void f64_to_f32(int n, const double *src, float *dst, Scal *s, bool check_done) {
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, src, dst, s,
+                     check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+
 template <typename T>
 void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done) {
   int blocks = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
@@ -765,6 +776,11 @@ void scal_reset(Scal *s, double thr2) {
 
 void scal_set_done(Scal *s, int value) {
   hipLaunchKernelGGL(k_scal_set_done, dim3(1), dim3(64), 0, stream(), s, value);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void scal_set_final(Scal *s, double thr_final2) {
+  hipLaunchKernelGGL(k_scal_set_final, dim3(1), dim3(64), 0, stream(), s, thr_final2);
   CFDLA_HIP(hipGetLastError());
 }
 

@@ -52,6 +52,8 @@ __device__ __forceinline__ void sum_partials(int nblocks, const double *partials
   double acc[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  // unrolled so up to 8 loads per value are in flight (same summation order)
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblocks; b += kBlock) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] += partials[i * kMaxBlocks + b];
@@ -74,7 +76,7 @@ __device__ __forceinline__ void sum_partials(int nblocks, const double *partials
 inline int blocks_for(int64_t work_items, int per_block) {
   int64_t b = (work_items + per_block - 1) / per_block;
   if (b < 1) b = 1;
-  if (b > kMaxBlocks) b = kMaxBlocks;
+  if (b > kGridCap) b = kGridCap;
   return (int)b;
 }
 

@@ -205,6 +205,73 @@ bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t
 }
 }  // namespace
 
+namespace {
+// Diagonal layout (see DiaDesc) for square matrices of at least 4096 rows
+// whose nonzeros lie on few diagonals filled to at least 90 %.  Returns the
+// stored diagonals diagonal-major in val (nd x n).
+bool build_dia(const HostCsr &h, DiaDesc &d, std::vector<double> &val) {
+  if (h.n != h.m || h.n < 4096 || h.nnz() == 0) return false;
+  const int64_t n = h.n;
+  std::vector<int64_t> offs;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
+      const int64_t o = (int64_t)h.ci[k] - i;
+      if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+        offs.push_back(o);
+        if (offs.size() > (size_t)2 * kDiaMax) return false;
+      }
+    }
+  std::sort(offs.begin(), offs.end());
+  const int64_t nall = (int64_t)offs.size();
+  if ((double)h.nnz() < 0.9 * (double)n * (double)nall) return false;
+  if (nall * n >= INT32_MAX) return false;
+  std::vector<double> full((size_t)(nall * n), 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
+      const int64_t kd = std::lower_bound(offs.begin(), offs.end(), (int64_t)h.ci[k] - i) - offs.begin();
+      full[(size_t)(kd * n + i)] = h.v[k];
+    }
+  auto index_of = [&](int64_t o) -> int64_t {
+    auto it = std::lower_bound(offs.begin(), offs.end(), o);
+    return it != offs.end() && *it == o ? it - offs.begin() : -1;
+  };
+  bool sym = true;
+  for (int64_t kd = 0; kd < nall && sym; ++kd) {
+    const int64_t o = offs[(size_t)kd];
+    if (o <= 0) continue;
+    const int64_t km = index_of(-o);
+    if (km < 0) {
+      sym = false;
+      break;
+    }
+    for (int64_t i = 0; i + o < n; ++i)
+      if (!(full[(size_t)(kd * n + i)] == full[(size_t)(km * n + i + o)])) {
+        sym = false;
+        break;
+      }
+  }
+  std::vector<int64_t> keep;
+  if (sym) {
+    keep.push_back(index_of(0));  // -1: no main diagonal, stored as zeros
+    for (int64_t kd = 0; kd < nall; ++kd)
+      if (offs[(size_t)kd] > 0) keep.push_back(kd);
+  } else {
+    for (int64_t kd = 0; kd < nall; ++kd) keep.push_back(kd);
+  }
+  if ((int64_t)keep.size() > kDiaMax) return false;
+  d = DiaDesc{};
+  d.nd = (int)keep.size();
+  d.sym = sym ? 1 : 0;
+  for (int j = 0; j < d.nd; ++j) d.off[j] = keep[(size_t)j] < 0 ? 0 : (int)offs[(size_t)keep[(size_t)j]];
+  if (!dia_supported(d)) return false;
+  val.assign((size_t)(d.nd * n), 0.0);
+  for (int j = 0; j < d.nd; ++j)
+    if (keep[(size_t)j] >= 0)
+      std::copy(full.begin() + keep[(size_t)j] * n, full.begin() + (keep[(size_t)j] + 1) * n, val.begin() + j * n);
+  return true;
+}
+}  // namespace
+
 void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell) {
   CFDLA_REQUIRE(h.n < INT32_MAX && h.m < INT32_MAX && h.nnz() < INT32_MAX, CFDLA_ERR_INVALID_ARG,
                 "matrix too large for 32-bit device indices");
@@ -240,10 +307,21 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     for (size_t k = 0; k < d.size(); ++k) df[k] = (float)d[k];
     sdinv32.upload(df.data(), df.size());
   }
+  std::vector<double> dvals;
+  dia = allow_sell && build_dia(h, dia_desc, dvals);
+  if (dia) {
+    dia_v64.upload(dvals.data(), dvals.size());
+    if (want_fp32) {
+      std::vector<float> df32(dvals.size());
+      for (size_t k = 0; k < df32.size(); ++k) df32[k] = (float)dvals[k];
+      dia_v32.upload(df32.data(), df32.size());
+      sync_stream();
+    }
+  }
   std::vector<int32_t> sp;
   std::vector<int16_t> sd;
   std::vector<double> sv;
-  sell = allow_sell && build_sell(h, sp, sd, sv);
+  sell = allow_sell && !dia && build_sell(h, sp, sd, sv);
   if (sell) {
     nslices = (int)(sp.size() - 1);
     sell_entries = sp.back();
@@ -384,12 +462,16 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) 
   const int flips = (down > 0 ? down - 1 : 0) + up;
   int cur = (flips % 2 == 0) ? 0 : 1;
   int k = 1;
-  if (down >= 2) {
+  // Fusing the zero-guess sweep into the next one gathers dinv and b at
+  // every column instead of x: a win on stencil rows (<= 8 entries), a loss
+  // on the long rows of Galerkin levels (measured 44.7 us fused against
+  // 26.2 + ~3 us unfused on the first coarse level of the 128^3 problem).
+  if (down >= 2 && L.A.nnz <= 8 * (int64_t)n) {
     // the zero-guess sweep and the next one in a single pass
     csr_apply<T>(L.A, EPI_JACOBI0, b, b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
     cur ^= 1;
     k = 2;
-  } else if (down == 1) {
+  } else if (down >= 1) {
     jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
   } else {
     fill_vec<T>(n, T(0), buf[cur]);
@@ -526,6 +608,137 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
   return o;
 }
 
+// FP32-inner PCG with reliable updates (residual replacement): the FP32
+// recurrence runs uninterrupted -- no restart, so the Krylov space and its
+// convergence rate survive -- while its solution increments accumulate in an
+// FP32 correction xc that is folded into the FP64 solution, and the FP32
+// residual is replaced by the FP64 residual b - A x (Sleijpen & van der
+// Vorst 1996; van der Vorst & Ye 2000; Clark et al., Comput. Phys. Commun.
+// 181 (2010) 1517, mixed-precision CG with reliable updates):
+//   * interval > 0: every `interval` iterations, between the residual
+//     update of an iteration and its preconditioner application, enqueued
+//     with the iteration (skipped once the solve has stopped);
+//   * delta > 0: in the next enqueued iteration once the lagged host poll
+//     shows the recursive residual at or below delta times the latest FP64
+//     residual (the reduction criterion of Clark et al.);
+//   * when the FP32 recursive residual passes the FP64 threshold (done = 1,
+//     seen by the lagged host poll), to confirm convergence.
+// A replacement leaves rz and p in place: z = M r_new, beta = r_new.z / rz,
+// p = z + beta p continue the iteration -- unless the FP64 residual comes out
+// above twice the recursive one it replaces, when beta = 0 restarts the
+// direction (the FP32 operator's rounding bounds the true-residual gain of
+// one cycle near eps32 * cond(A); past that the recursive residual is
+// fiction and beta would blow up).  The FP64 residual decides
+// convergence (OP_RELIABLE: done = 4), so CONVERGED is always backed by an
+// explicit FP64 residual.
+SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, double thr, int64_t max_it,
+                      int interval, double delta, int chunk) {
+  CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
+  Context &c = ctx();
+  RedWs &ws = c.ws;
+  const int n = A.n;
+  double *r64 = c.dv_at(8, n);
+  float *r = c.fv_at(0, n), *z = c.fv_at(1, n), *p = c.fv_at(2, n), *q = c.fv_at(3, n), *xc = c.fv_at(4, n);
+  Scal *s = c.s(0);
+  SolveOut o;
+  o.thr = thr;
+  scal_reset(s, thr * thr);
+  o.rn0 = o.rn = true_residual<double>(A, b, x, r64, s);
+  if (o.rn0 <= thr) {
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
+  f64_to_f32(n, r64, r, s, false);
+  fill_vec<float>(n, 0.0f, xc);
+  const float *jd = jdinv<float>(M);
+  // z = M r (+ rz) and p = z
+  M->apply_dot(n, r, z, s, false, OP_SET_RZ);
+  copy_vec<float>(n, z, p, s, false);
+  if (chunk <= 0) chunk = auto_chunk(M, n);
+  // FP64 residual replacement; check_done skips it once the solve stopped
+  auto replace = [&](bool check_done) {
+    fold_correction(n, xc, x, s, check_done);
+    csr_apply<double>(A, EPI_RESID, x, b, nullptr, 0.0, r64, s, check_done, OP_RELIABLE, r64, ws);
+    f64_to_f32(n, r64, r, s, check_done);
+  };
+  // the preconditioner half of an iteration: z = M r, beta, p = z + beta p
+  auto direction = [&]() {
+    if (jd) {
+      jacobi_first<float>(n, jd, 1.0, r, z, s, true);
+      dot2<float>(n, r, z, nullptr, nullptr, s, true, OP_CG_BETA, ws);
+    } else {
+      M->apply_dot(n, r, z, s, true, OP_CG_BETA);
+    }
+    cg_direction<float>(n, z, p, s);
+  };
+  int64_t enq = 0;
+  int32_t replacements = 0;  // enqueued, including those the device skips after stopping
+  bool want = false;         // a reduction by delta was seen: replace in the next step
+  Poller poll(s);
+  for (;;) {
+    const Scal *h = nullptr;
+    if (enq < max_it) {
+      const int64_t steps = std::min<int64_t>(chunk, max_it - enq);
+      for (int64_t k = 0; k < steps; ++k) {
+        csr_apply<float>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        const bool rep = want || (interval > 0 && (enq + k + 1) % interval == 0);
+        if (rep) ++replacements;
+        want = false;
+        if (jd && !rep) {
+          // xc += alpha p, r -= alpha q, z = D^-1 r, rr and rz -> beta in one pass
+          cg_update<float>(n, p, q, xc, r, jd, z, s, ws);
+          cg_direction<float>(n, z, p, s);
+        } else {
+          cg_update<float>(n, p, q, xc, r, nullptr, nullptr, s, ws);  // xc += alpha p, r -= alpha q, rr
+          if (rep) replace(true);
+          direction();
+        }
+      }
+      enq += steps;
+      poll.post();
+      h = poll.lagged();
+    } else {
+      h = poll.latest();
+    }
+    if (!h) continue;
+    if (h->done == 0 && enq < max_it) {
+      // Reduction trigger (Clark et al. 2010): once every replacement enqueued
+      // so far shows in the polled state, replace again when the recursive
+      // residual has dropped by delta since the latest FP64 residual.
+      const double ref = h->nupd > 0 ? h->rr_true : o.rn0 * o.rn0;
+      if (delta > 0.0 && h->nupd >= replacements && h->rr <= delta * delta * ref) want = true;
+      continue;
+    }
+    if (h->done == 1 && enq < max_it) {
+      // the FP32 residual passed the threshold: confirm with the FP64 one
+      // and continue the iteration from it if it is not there yet
+      ++replacements;
+      replace(false);
+      direction();
+      poll = Poller(s);
+      poll.post();  // the replaced state, so latest() is defined once the budget is spent
+      continue;
+    }
+    sync_stream();
+    Scal now = c.read(s);
+    o.iters = now.iters;
+    if (now.done == 4) {
+      o.term = CFDLA_TERM_CONVERGED;
+      o.outer = now.nupd;
+      o.rn = std::sqrt(now.rr_true);
+      return o;
+    }
+    // stopped (breakdown, curvature) or out of budget: fold what the FP32
+    // recurrence has and report the FP64 residual
+    fold_correction(n, xc, x, s, false);
+    o.rn = true_residual<double>(A, b, x, r64, s);
+    o.outer = now.nupd + 1;
+    if (now.done == 2) o.term = now.term;
+    else o.term = o.rn <= thr ? CFDLA_TERM_CONVERGED : CFDLA_TERM_MAX_ITERATIONS;
+    return o;
+  }
+}
+
 // Right-preconditioned BiCGSTAB (Saad 2003, Alg. 7.7).
 template <typename T>
 SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t max_it, int chunk,
@@ -615,6 +828,10 @@ SolveOut run_fp64(const DevCsr &A, Precond *M, const double *b, double *x, doubl
 SolveOut run_refined(const DevCsr &A, Precond *M, const double *b, double *x, double thr,
                      const cfdla_solver_config &cfg) {
   CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
+  if (cfg.kind == CFDLA_CG && cfg.reliable_interval >= 0)
+    return pcg_reliable(A, M, b, x, thr, cfg.max_iter, cfg.reliable_interval,
+                        cfg.reliable_interval == 0 ? (cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-2) : 0.0,
+                        cfg.check_every);
   Context &c = ctx();
   const int n = A.n;
   double *r64 = c.dv_at(8, n);
@@ -674,6 +891,7 @@ SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, c
   CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
                 "unknown precision");
   CFDLA_REQUIRE(cfg.max_iter >= 1, CFDLA_ERR_CONFIG, "max_iter must be >= 1");
+  CFDLA_REQUIRE(cfg.reliable_interval >= -1, CFDLA_ERR_CONFIG, "reliable_interval must be >= -1");
   CFDLA_REQUIRE(std::isfinite(cfg.tol_abs) && std::isfinite(cfg.tol_rel) && cfg.tol_abs >= 0.0 && cfg.tol_rel >= 0.0,
                 CFDLA_ERR_CONFIG, "tolerances must be finite and non-negative");
   std::unique_ptr<IdentityPrecond> ident;

@@ -79,6 +79,8 @@ class IterativeSolverConfig:
     inner_tolerance: float = 1e-5
     max_outer: int = 30
     check_every: int = 0
+    reliable_interval: int = 0
+    reliable_delta: float = 0.0
 
     @classmethod
     def new(cls, tolerance: float) -> "IterativeSolverConfig":
@@ -98,10 +100,23 @@ class IterativeSolverConfig:
             self.inner_tolerance = inner_tolerance
         return self
 
+    def with_reliable_interval(self, k: int) -> "IterativeSolverConfig":
+        """FP32_REFINED CG: FP64 residual replacement every k iterations
+        (0 = when the residual has dropped by ``reliable_delta``); -1 selects
+        restarted refinement."""
+        self.reliable_interval = int(k)
+        return self
+
+    def with_reliable_delta(self, delta: float) -> "IterativeSolverConfig":
+        """FP32_REFINED CG: residual reduction between FP64 residual
+        replacements (0 = library default, 1e-2)."""
+        self.reliable_delta = float(delta)
+        return self
+
     def native(self, kind: int, restart: int = 30, ladder: bool = True) -> N.SolverConfig:
         return N.SolverConfig(kind, restart, 1 if ladder else 0, int(self.precision), self.absolute_tolerance,
                               self.tolerance, self.max_iterations, self.inner_tolerance, self.max_outer,
-                              self.check_every)
+                              self.check_every, self.reliable_interval, 0, self.reliable_delta)
 
 
 def restart_width(requested: int) -> int:

@@ -108,11 +108,13 @@ class CsrMatrix:
         N.call("cfdla_spmv_device", self.handle, dt, x.data_ptr(), y.data_ptr())
 
     def storage(self) -> dict:
-        """Device storage of the matrix: format ("sell64" or "csr_rowblock"),
-        stored entries (padded slots for sell64) and slices / row blocks."""
+        """Device storage of the matrix: format ("dia", "sell64" or
+        "csr_rowblock"), stored entries (diagonals x rows for dia, padded
+        slots for sell64, nnz for csr) and stored diagonals / slices / row
+        blocks."""
         fmt, ent, units = C.c_int32(), C.c_int64(), C.c_int64()
         N.call("cfdla_matrix_storage", self.handle, C.byref(fmt), C.byref(ent), C.byref(units))
-        return {"format": "sell64" if fmt.value == 1 else "csr_rowblock", "stored_entries": ent.value,
+        return {"format": {0: "csr_rowblock", 1: "sell64", 2: "dia"}[fmt.value], "stored_entries": ent.value,
                 "units": units.value}
 
     def diagonal(self):

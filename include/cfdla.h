@@ -82,9 +82,15 @@ typedef struct {
   double tol_abs;
   double tol_rel;
   int64_t max_iter;        /* total (inner) iteration budget */
-  double inner_tol;        /* FP32_REFINED: relative tolerance of each inner solve */
-  int32_t max_outer;       /* FP32_REFINED: refinement step budget */
+  double inner_tol;        /* FP32_REFINED restarted refinement: relative tolerance of each inner solve */
+  int32_t max_outer;       /* FP32_REFINED restarted refinement: refinement step budget */
   int32_t check_every;     /* host convergence poll interval in iterations (0 = auto) */
+  int32_t reliable_interval; /* FP32_REFINED CG runs one uninterrupted FP32 PCG whose residual is replaced
+                              * by the FP64 residual at convergence and: 0 = when it has dropped by
+                              * reliable_delta since the last replacement; k > 0 = every k iterations.
+                              * -1 = restarted refinement (inner_tol, max_outer) as for BiCGSTAB/GMRES */
+  int32_t reserved;
+  double reliable_delta;   /* residual reduction between replacements for reliable_interval 0 (<= 0: 1e-2) */
 } cfdla_solver_config;
 
 /* athena SolveReport {iterations, initial_residual_norm,
@@ -170,13 +176,17 @@ cfdla_status cfdla_matrix_create_device(int64_t nrows, int64_t ncols, int64_t nn
 cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values);
 cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, int64_t *nnz);
 /* Device storage chosen for the matrix (uploads it if needed): *format is
+ * CFDLA_STORAGE_DIA (stencil matrices: diagonal-major values, no column
+ * indices; symmetric ones store the main and upper diagonals only;
+ * *stored_entries = stored diagonals x rows, *units = stored diagonals),
  * CFDLA_STORAGE_SELL64 (64-row slices, int16 column offsets; *stored_entries
- * counts the padded slots) or CFDLA_STORAGE_CSR_ROWBLOCK (int32 columns;
- * *stored_entries = nnz).  *units = slices or row blocks.  No counterpart in
- * the reference, whose CSR lives in host memory; used to state the bytes a
- * product moves. */
+ * counts the padded slots, *units = slices) or CFDLA_STORAGE_CSR_ROWBLOCK
+ * (int32 columns; *stored_entries = nnz, *units = row blocks).  No
+ * counterpart in the reference, whose CSR lives in host memory; used to state
+ * the bytes a product moves. */
 #define CFDLA_STORAGE_CSR_ROWBLOCK 0
 #define CFDLA_STORAGE_SELL64 1
+#define CFDLA_STORAGE_DIA 2
 cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stored_entries, int64_t *units);
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
 

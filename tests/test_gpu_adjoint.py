@@ -2,7 +2,8 @@
 correlation length 16 cells, sigma 1) from the adjoint AMG-PCG solve and the
 coefficient-gradient kernel, checked against the CPU oracle's adjoint
 gradient (relative 1e-6, both solves at >= 1e-10) and against central finite
-differences of GPU forward solves on 32 random directions (relative 1e-5),
+differences of GPU forward solves on 32 random directions (relative 1e-5
+plus a floor of 1e-8 |grad| |d| for nearly cancelling directions),
 and through torch.autograd."""
 import numpy as np
 import pytest
@@ -45,7 +46,11 @@ def test_gradient_matches_central_differences(setup):
         jp = ADJ.flux_and_gradient(prob, g + h * d)[0]
         jm = ADJ.flux_and_gradient(prob, g - h * d)[0]
         fd = (jp - jm) / (2 * h)
-        assert abs(fd - grad @ d) <= 1e-5 * abs(fd), (d_i, fd, grad @ d)
+        # relative 1e-5, with a floor of 1e-8 |grad| |d| (the Cauchy-Schwarz
+        # scale of grad . d) for directions where the derivative nearly
+        # cancels: the O(h^2) truncation error does not shrink with it
+        tol = 1e-5 * abs(fd) + 1e-8 * np.linalg.norm(grad) * np.linalg.norm(d)
+        assert abs(fd - grad @ d) <= tol, (d_i, fd, grad @ d)
 
 
 def test_torch_autograd_function(setup):

@@ -47,8 +47,8 @@ def test_spmv_integer_bit_exact(row_len):
 
 @pytest.mark.parametrize("shape", [(37, 1, 1), (64, 64, 1), (33, 17, 9), (128, 128, 128)])
 def test_spmv_stencil_integer_bit_exact(shape):
-    # stencil matrices take the sliced-ELL kernel (int16 column offsets);
-    # integer values and x make every partial sum exact
+    # stencil matrices of >= 4096 rows take the diagonal kernel, smaller ones
+    # the row-block kernel; integer values and x make every partial sum exact
     nx, ny, nz = shape
     n = nx * ny * nz
     k = np.round(P.uniform(3, n, 1.0, 9.0))
@@ -57,6 +57,67 @@ def test_spmv_stencil_integer_bit_exact(shape):
     x = np.round(P.uniform(4, n, -20.0, 20.0))
     y = ca.CsrMatrix.from_parts(v, ci, rp, n, n).spmv(x)
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n).spmv(x))
+
+
+def test_dia_storage_selected():
+    # stencil operators take the diagonal layout: the symmetric 7-point
+    # operator stores its main and 3 upper diagonals, the nonsymmetric
+    # 5-point upwind momentum operator all 5; scattered matrices do not
+    rp, ci, v, _ = P.config2(24)
+    n = 24 ** 3
+    st = ca.CsrMatrix.from_parts(v, ci, rp, n, n).storage()
+    assert st == {"format": "dia", "stored_entries": 4 * n, "units": 4}
+    orc = _momentum()
+    mrp, mci, mv = orc.arrays()
+    m = mrp.size - 1
+    assert ca.CsrMatrix.from_parts(mv, mci, mrp, m, m).storage()["format"] == "dia"
+    assert ca.CsrMatrix.from_parts(mv, mci, mrp, m, m).storage()["units"] == 5
+    rp, ci, v = random_csr(5000, 5000, 7, seed=2)
+    assert ca.CsrMatrix.from_parts(v, ci, rp, 5000, 5000).storage()["format"] != "dia"
+
+
+def test_spmv_dia_nonsymmetric_integer_bit_exact():
+    orc = _momentum()
+    rp, ci, v = orc.arrays()
+    n = rp.size - 1
+    v = np.round(v * 64.0)  # integer-valued, same (nonsymmetric) pattern
+    x = np.round(P.uniform(5, n, -30.0, 30.0))
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    assert a.storage()["format"] == "dia"
+    assert np.array_equal(a.spmv(x), O.Csr.from_arrays(rp, ci, v, n).spmv(x))
+
+
+def test_spmv_dia_partial_diagonals_and_absent_entries():
+    # 5-point grid Laplacian on 64 x 64 with 4 % of the neighbour couplings
+    # removed symmetrically: the diagonals stay >= 90 % full, so the layout is
+    # kept, and the zeros it stores for absent entries must not reach y --
+    # with x = inf at the end of a grid row, the next row start (one
+    # diagonal step away, not a neighbour) stays finite, as in CSR
+    nn = 64
+    d = np.zeros((nn * nn, nn * nn))
+    rng = np.random.default_rng(8)
+    for i in range(nn):
+        for j in range(nn):
+            p = i * nn + j
+            d[p, p] = 40.0
+            for q in ((i + 1) * nn + j if i + 1 < nn else -1, p + 1 if j + 1 < nn else -1):
+                if q >= 0 and rng.random() >= 0.04:
+                    w = -float(rng.integers(1, 9))
+                    d[p, q] = d[q, p] = w
+    from scipy import sparse as S
+    sp = S.csr_matrix(d)
+    rp, ci, v = sp.indptr.astype(np.int64), sp.indices.astype(np.int64), sp.data
+    n = nn * nn
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    assert a.storage() == {"format": "dia", "stored_entries": 3 * n, "units": 3}
+    x = np.round(P.uniform(6, n, -30.0, 30.0))
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    assert np.array_equal(a.spmv(x), orc.spmv(x))
+    x[nn - 1] = np.inf
+    y = a.spmv(x)
+    yo = orc.spmv(x)
+    assert np.isfinite(y[nn]) and y[nn] == yo[nn]
+    assert np.array_equal(np.isfinite(y), np.isfinite(yo))
 
 
 def test_spmv_fp32_device_tensor():
@@ -296,3 +357,76 @@ def test_amg_recompute_then_solve():
     rep = ca.cg_preconditioned(a, b, amg, x, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500))
     assert rep.termination == ca.Termination.Converged
     assert rel_res(O.Csr.from_arrays(rp, ci, v2, n), x, b) <= 1e-10
+
+
+@pytest.mark.parametrize("interval,delta", [(0, 0.0), (0, 1e-1), (0, 1e-5), (1, 0.0), (3, 0.0), (8, 0.0),
+                                            (32, 0.0), (-1, 0.0)])
+def test_refined_cg_reliable_updates_amg(interval, delta):
+    # FP32 PCG with FP64 residual replacement when the residual dropped by
+    # delta (interval 0) or every `interval` iterations (-1: restarted
+    # refinement).  All reach the FP64 threshold; the uninterrupted
+    # recurrence keeps the FP64 iteration count.  interval 32 replaces only at
+    # convergence, where the FP64 residual is far above the recursive one:
+    # the direction restart keeps it from diverging.
+    rp, ci, v, b, n = _config2_small(32)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    amg = ca.AlgebraicMultigrid(a, ca.AMGConfig.smoothed_aggregation())
+    x64 = np.zeros(n)
+    r64 = ca.cg_preconditioned(a, b, amg, x64, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500))
+    x = np.zeros(n)
+    cfg = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500)
+           .with_precision(ca.Precision.FP32_REFINED).with_reliable_interval(interval).with_reliable_delta(delta))
+    rep = ca.cg_preconditioned(a, b, amg, x, cfg)
+    assert rep.termination == ca.Termination.Converged, rep
+    res = rel_res(orc, x, b)
+    assert res <= 1e-10
+    # the reported norm is the explicit FP64 residual of the returned x
+    assert rep.final_residual_norm == pytest.approx(res * np.linalg.norm(b), rel=1e-6)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) <= 1e-8
+    if 0 <= interval <= 8 and delta != 1e-5:
+        assert rep.iterations <= r64.iterations + 2, (rep.iterations, r64.iterations)
+    if interval >= 0:
+        assert rep.iterations <= 2 * r64.iterations, (rep.iterations, r64.iterations)
+        assert rep.outer_iterations >= 1
+
+
+def test_refined_cg_reliable_updates_jacobi_config1():
+    # kappa ~ 1e4: the FP32 recurrence loses orthogonality faster than the
+    # FP64 one (measured 909 against 750 iterations; restarted refinement
+    # takes 1471)
+    rp, ci, v, b = P.config1(128)
+    n = 128 * 128
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    jac = ca.JacobiPreconditioner(a)
+    x64 = np.zeros(n)
+    r64 = ca.cg_preconditioned(a, b, jac, x64, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000))
+    x = np.zeros(n)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(ca.Precision.FP32_REFINED)
+    rep = ca.cg_preconditioned(a, b, jac, x, cfg)
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    assert rep.iterations <= 1.3 * r64.iterations, (rep.iterations, r64.iterations)
+
+
+def test_refined_cg_reliable_budget_and_initial_residual():
+    rp, ci, v, b = P.config1(32)
+    n = 1024
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    cfg = ca.IterativeSolverConfig.new(1e-12).with_max_iterations(5).with_precision(ca.Precision.FP32_REFINED)
+    x = np.zeros(n)
+    rep = ca.cg_preconditioned(a, b, ca.JacobiPreconditioner(a), x, cfg)
+    assert rep.termination == ca.Termination.MaxIterations and rep.iterations == 5
+    # the FP32 correction is folded into x and the reported norm is its FP64 residual
+    assert np.any(x != 0)
+    assert rep.final_residual_norm == pytest.approx(np.linalg.norm(b - orc.spmv(x)), rel=1e-9)
+    x = np.zeros(n)
+    rep = ca.cg_preconditioned(a, np.zeros(n), ca.JacobiPreconditioner(a), x, cfg)
+    assert rep.termination == ca.Termination.InitialResidual and rep.iterations == 0
+    d = ca.CsrMatrix.from_parts([1.0, -1.0, 2.0], [0, 1, 2], [0, 1, 2, 3], 3, 3)
+    x = np.zeros(3)
+    rep = ca.cg(d, np.ones(3), x, ca.IterativeSolverConfig.new(1e-12).with_max_iterations(10)
+                .with_precision(ca.Precision.FP32_REFINED))
+    assert rep.termination in (ca.Termination.NonPositiveCurvature, ca.Termination.Breakdown)
