@@ -38,8 +38,8 @@ struct BArgs {
   const int *items;
   unsigned *queue;
   int n_block_items, n_wave_items, n_wave_sys;
-  double tol_abs, tol_rel, inner_tol;
-  int max_it, max_outer, refined;
+  double tol_abs, tol_rel, inner_tol, delta;
+  int max_it, max_outer, refined, reliable;
 };
 
 struct WaveTeam {
@@ -147,6 +147,104 @@ __device__ int team_cg(const BArgs &A, int r0, int n, int tl, T *acc, T *ps, T *
   return CFDLA_TERM_MAX_ITERATIONS;
 }
 
+// FP32 Jacobi-PCG with reliable FP64 residual replacement, team-local (the
+// scheme of the single-system reliable-update PCG): the FP32 recurrence runs
+// on the residual scaled by 1 / ||r0|| and accumulates its increments in d;
+// when its equilibrated residual has dropped by delta since the latest
+// replacement, or passes the threshold, d is folded into the FP64 solution
+// in LDS and the residual is replaced by the FP64 one.  A replacement that
+// comes out above twice the recursive residual restarts the direction and
+// widens delta tenfold (up to 0.5): the cycle outran FP32 precision.
+template <class Team>
+__device__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *xs, float *ps, double rr_true,
+                                double thr2, int &iters, int &nrep, double &rr_out, double *sh) {
+  const double sc = 1.0 / sqrt(rr_true);
+  const double sc2 = sc * sc, thr2s = thr2 * sc2;
+  double d2 = A.delta * A.delta;  // widened tenfold (up to 0.5) after each direction restart
+  float *d = A.d32 + r0;
+  float *rv = A.r32, *zv = A.z32, *qv = A.q32;
+  const float *vals = A.v32, *jd = A.jd32;
+  auto load_residual = [&]() {  // r32 = sc * r64, z = D^-1 r32; returns r.z
+    double rz = 0.0;
+    for (int i = tl; i < n; i += Team::size) {
+      const int row = r0 + i;
+      const float r = (float)(A.r64[row] * sc);
+      rv[row] = r;
+      const float z = jd[row] * r;
+      zv[row] = z;
+      rz += (double)r * (double)z;
+    }
+    return Team::sum(rz, sh);
+  };
+  auto fold = [&]() {  // xs += d / sc, d = 0, then the FP64 residual
+    for (int i = tl; i < n; i += Team::size) {
+      xs[i] += (double)d[i] / sc;
+      d[i] = 0.0f;
+    }
+    Team::sync();
+    rr_out = team_resid<Team>(A, r0, n, tl, xs, sh);
+    ++nrep;
+  };
+  for (int i = tl; i < n; i += Team::size) d[i] = 0.0f;
+  double rz = load_residual();
+  for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i];
+  Team::sync();
+  double ref = rr_true * sc2;
+  bool restart = false;
+  while (iters < A.max_it) {
+    double pq = 0.0;
+    for (int i = tl; i < n; i += Team::size) {
+      const int row = r0 + i;
+      float sv = 0.0f;
+      for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) sv += vals[k] * ps[A.ci[k] - r0];
+      qv[row] = sv;
+      pq += (double)ps[i] * (double)sv;
+    }
+    pq = Team::sum(pq, sh);
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      fold();
+      return CFDLA_TERM_NONPOSITIVE_CURVATURE;
+    }
+    const double alpha = rz / pq;
+    double rr = 0.0, rzn = 0.0;
+    for (int i = tl; i < n; i += Team::size) {
+      const int row = r0 + i;
+      d[i] += (float)alpha * ps[i];
+      const float r = rv[row] - (float)alpha * qv[row];
+      rv[row] = r;
+      const float z = jd[row] * r;
+      zv[row] = z;
+      const double e = A.eq[row] * (double)r;
+      rr += e * e;
+      rzn += (double)r * (double)z;
+    }
+    Team::sum2(rr, rzn, sh);
+    ++iters;
+    if (rr <= thr2s || rr <= d2 * ref) {
+      Team::sync();
+      fold();
+      if (rr_out <= thr2) return CFDLA_TERM_CONVERGED;
+      if (!isfinite(rr_out)) return CFDLA_TERM_BREAKDOWN;
+      restart = rr_out * sc2 > 4.0 * rr;
+      if (restart) d2 = fmin(0.25, d2 * 100.0);
+      ref = rr_out * sc2;
+      rzn = load_residual();
+    }
+    if (rzn == 0.0 || !isfinite(rzn)) {
+      Team::sync();
+      fold();
+      return CFDLA_TERM_BREAKDOWN;
+    }
+    const double beta = restart ? 0.0 : rzn / rz;
+    restart = false;
+    rz = rzn;
+    for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i] + (float)beta * ps[i];
+    Team::sync();
+  }
+  fold();
+  return rr_out <= thr2 ? CFDLA_TERM_CONVERGED : CFDLA_TERM_MAX_ITERATIONS;
+}
+
 template <class Team>
 __device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double *psd, double *sh) {
   const int r0 = A.sys_row[sys], n = A.sys_row[sys + 1] - r0;
@@ -182,6 +280,8 @@ __device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double
       }
       // the explicit residual now in r64 restarts the recurrence
     }
+  } else if (A.reliable) {
+    term = team_cg_reliable<Team>(A, r0, n, tl, xs, reinterpret_cast<float *>(psd), rr, thr2, iters, outer, rr, sh);
   } else {
     float *ps = reinterpret_cast<float *>(psd);
     float *d = A.d32 + r0;
@@ -366,6 +466,8 @@ void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *
   a.max_it = (int)cfg.max_iter;
   a.max_outer = cfg.max_outer > 0 ? cfg.max_outer : 30;
   a.refined = cfg.precision == CFDLA_FP32_REFINED ? 1 : 0;
+  a.reliable = cfg.reliable_interval >= 0 ? 1 : 0;  // a replacement interval does not apply per team: delta only
+  a.delta = cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1;
   const int grid = std::max(1, sched.blocks_per_cu) * num_cus();
   hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream(), d_queue.p);
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBlock), 0, stream(), a);

@@ -184,6 +184,8 @@ struct Scal {
   int32_t kstep;  // GMRES: Arnoldi steps in the current cycle
   int32_t nupd;   // reliable-update CG: FP64 residual replacements performed
   int32_t restart_dir;  // reliable-update CG: the next OP_CG_BETA restarts the direction (beta = 0)
+  int32_t nrestart;     // reliable-update CG: replacements that restarted the direction
+  int32_t pad;
 };
 
 // Per-stream reduction workspace: block partials and an arrival counter that

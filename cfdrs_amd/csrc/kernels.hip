@@ -122,6 +122,7 @@ __device__ void finalize(Scal *s, int op) {
       } else {
         s->done = rr <= s->thr_final2 ? 4 : 0;
         s->restart_dir = rr > 4.0 * rec ? 1 : 0;
+        s->nrestart += s->restart_dir;
       }
     } break;
     case OP_BI_RR: {
@@ -579,6 +580,7 @@ __global__ void k_scal_reset(Scal *s, double thr2) {
     s->kstep = 0;
     s->nupd = 0;
     s->restart_dir = 0;
+    s->nrestart = 0;
     s->term = CFDLA_TERM_MAX_ITERATIONS;
   }
 }

@@ -620,7 +620,8 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
 //     with the iteration (skipped once the solve has stopped);
 //   * delta > 0: in the next enqueued iteration once the lagged host poll
 //     shows the recursive residual at or below delta times the latest FP64
-//     residual (the reduction criterion of Clark et al.);
+//     residual (the reduction criterion of Clark et al.), delta widened
+//     tenfold (up to 0.5) after each replacement that restarted the direction;
 //   * when the FP32 recursive residual passes the FP64 threshold (done = 1,
 //     seen by the lagged host poll), to confirm convergence.
 // A replacement leaves rz and p in place: z = M r_new, beta = r_new.z / rz,
@@ -705,8 +706,11 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
       // Reduction trigger (Clark et al. 2010): once every replacement enqueued
       // so far shows in the polled state, replace again when the recursive
       // residual has dropped by delta since the latest FP64 residual.
+      // Each replacement that had to restart the direction shows a cycle
+      // longer than FP32 precision carries: widen delta tenfold (at most 0.5).
       const double ref = h->nupd > 0 ? h->rr_true : o.rn0 * o.rn0;
-      if (delta > 0.0 && h->nupd >= replacements && h->rr <= delta * delta * ref) want = true;
+      const double de = std::min(0.5, delta * std::pow(10.0, (double)h->nrestart));
+      if (delta > 0.0 && h->nupd >= replacements && h->rr <= de * de * ref) want = true;
       continue;
     }
     if (h->done == 1 && enq < max_it) {
@@ -830,7 +834,7 @@ SolveOut run_refined(const DevCsr &A, Precond *M, const double *b, double *x, do
   CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
   if (cfg.kind == CFDLA_CG && cfg.reliable_interval >= 0)
     return pcg_reliable(A, M, b, x, thr, cfg.max_iter, cfg.reliable_interval,
-                        cfg.reliable_interval == 0 ? (cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-2) : 0.0,
+                        cfg.reliable_interval == 0 ? (cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1) : 0.0,
                         cfg.check_every);
   Context &c = ctx();
   const int n = A.n;

@@ -109,7 +109,7 @@ class IterativeSolverConfig:
 
     def with_reliable_delta(self, delta: float) -> "IterativeSolverConfig":
         """FP32_REFINED CG: residual reduction between FP64 residual
-        replacements (0 = library default, 1e-2)."""
+        replacements (0 = library default, 1e-1)."""
         self.reliable_delta = float(delta)
         return self
 

@@ -90,7 +90,7 @@ typedef struct {
                               * reliable_delta since the last replacement; k > 0 = every k iterations.
                               * -1 = restarted refinement (inner_tol, max_outer) as for BiCGSTAB/GMRES */
   int32_t reserved;
-  double reliable_delta;   /* residual reduction between replacements for reliable_interval 0 (<= 0: 1e-2) */
+  double reliable_delta;   /* residual reduction between replacements for reliable_interval 0 (<= 0: 1e-1) */
 } cfdla_solver_config;
 
 /* athena SolveReport {iterations, initial_residual_norm,

@@ -44,12 +44,15 @@ def small():
     return nets, NW.channel_batch(nets)
 
 
-@pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
+@pytest.mark.parametrize("precision,reliable", [(ca.Precision.FP64, 0), (ca.Precision.FP32_REFINED, 0),
+                                                (ca.Precision.FP32_REFINED, -1)],
+                         ids=["fp64", "fp32-reliable", "fp32-restarted"])
 @pytest.mark.parametrize("wave_max", [0, 256, 512])
-def test_batch_matches_dense(small, precision, wave_max):
+def test_batch_matches_dense(small, precision, reliable, wave_max):
     nets, batch = small
     batch.schedule = {"wave_max_rows": wave_max, "blocks_per_cu": 2}
-    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+    cfg = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+           .with_reliable_interval(reliable))
     x, reps = batch.solve(None, cfg)
     assert all(r.termination == ca.Termination.Converged for r in reps), [r for r in reps if
                                                                           r.termination != ca.Termination.Converged][:3]
@@ -65,6 +68,23 @@ def test_batch_matches_dense(small, precision, wave_max):
     # (row length) * eps * ||D(|b| + |A||x|)|| / ||D b||, i.e. ~1e-15 here
     rep_res = np.array([r.final_residual_norm / max(r.threshold / 1e-10, 1e-300) for r in reps])
     assert np.allclose(rep_res, res, rtol=1e-6, atol=1e-14)
+
+
+def test_batch_reliable_updates_keep_fp64_iteration_counts(small):
+    # FP32 recurrence with FP64 residual replacement and adaptive delta: the
+    # iteration counts stay near the FP64 ones (restarted refinement roughly
+    # doubles them on these ill-conditioned Laplacians)
+    _, batch = small
+    batch.schedule = {"wave_max_rows": 256, "blocks_per_cu": 2}
+    base = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+    _, r64 = batch.solve(None, base)
+    _, r32 = batch.solve(None, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+                         .with_precision(ca.Precision.FP32_REFINED))
+    i64 = np.array([r.iterations for r in r64])
+    i32 = np.array([r.iterations for r in r32])
+    assert np.all(i32 <= 2 * i64 + 20), np.max(i32 / i64)
+    assert i32.mean() <= 1.5 * i64.mean(), (i32.mean(), i64.mean())
+    assert all(r.outer_iterations >= 1 for r in r32)
 
 
 def test_batch_torch_device_and_warm_start(small):
