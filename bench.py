@@ -29,11 +29,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the roofline kernel (FP64 y = A x on the 128^3
-# operator), from two rocprofv3 counter passes over `bench.py --spmv-only`:
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors that make the counters
-# read 512 MiB each for a 512 MiB device copy
-# (profiles/k_sell_spmv_varcoef128.md).
-SPMV_FP64_TRAFFIC_BYTES = round((90927.0 * 2.0 + 16384.0) * 1024)
+# operator in diagonal storage), from two rocprofv3 counter passes over
+# `bench.py --spmv-only`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors
+# that make the counters read 512 MiB each for a 512 MiB device copy
+# (profiles/k_dia_spmv_varcoef128.md).  Keyed by the device format measured.
+SPMV_FP64_TRAFFIC_BYTES = {"dia": round((42327.151 * 2.0 + 16384.0) * 1024)}
 
 
 def log(msg: str) -> None:
@@ -202,7 +202,7 @@ def main() -> None:
         "copy_calibration": copy,
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": SPMV_FP64_TRAFFIC_BYTES if nn == 128 else None,
+                     "traffic": SPMV_FP64_TRAFFIC_BYTES.get(spmv["format"]) if nn == 128 else None,
                      "kernel": "FP64 y = A x, " + spmv["kernel"],
                      "avg_us": round(spmv[8]["us"], 3),
                      "timing": f"HIP events over {args.spmv_launches} back-to-back launches rotating over "
@@ -236,7 +236,7 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
               "sell64": f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)",
               "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)"}[storage["format"]]
     fn = N.lib().cfdla_spmv_device
-    out = {"kernel": kernel}
+    out = {"kernel": kernel, "format": storage["format"]}
     for dt, s, code in ((torch.float64, 8, N.DT_F64), (torch.float32, 4, N.DT_F32)):
         xs = [torch.from_numpy(b_host).to(dev).to(dt) for _ in mats]
         ys = [torch.empty_like(xs[0]) for _ in mats]
@@ -284,7 +284,19 @@ def copy_calibration(enabled: bool):
     nbytes = 2 * src.numel() * 8
     log(f"copy calibration: {src.numel() * 8} bytes read and written per copy, {us:.1f} us each, "
         f"{nbytes / (us * 1e-6) / 1e9:.1f} GB/s")
-    return {"bytes_per_launch": nbytes, "us": round(us, 2), "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)}
+    # read-only streaming: a sum over the two 512 MiB buffers alternately
+    tot = torch.zeros((), dtype=torch.float64, device="cuda:0")
+    tot += src.sum()
+    e0.record()
+    for k in range(4):
+        tot += (src if k % 2 == 0 else dst).sum()
+    e1.record()
+    e1.synchronize()
+    rus = e0.elapsed_time(e1) * 1e3 / 4
+    rbytes = src.numel() * 8
+    log(f"read calibration: sum over {rbytes} bytes, {rus:.1f} us each, {rbytes / (rus * 1e-6) / 1e9:.1f} GB/s")
+    return {"bytes_per_launch": nbytes, "us": round(us, 2), "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+            "read_gbs": round(rbytes / (rus * 1e-6) / 1e9, 1)}
 
 
 def _time_solves(fn, steps: int, warmup: int = 1):
