@@ -31,6 +31,7 @@ __device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, c
     e.d = dinv[row];
     e.x = omega * e.d * e.b;
   }
+  if constexpr (EPI == EPI_RJ0) e.d = dinv[row];
   if constexpr (EPI == EPI_ADD) e.y = y[row];
   if constexpr (DOT) {
     if (dotw != y) e.w = dotw[row];
@@ -38,16 +39,19 @@ __device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, c
   return e;
 }
 
+// EPI_RJ0 also writes y2 = omega * dinv .* y (the next level's first Jacobi
+// iterate from zero, evaluated as k_jacobi_first does).
 template <typename T, int EPI, bool DOT>
 __device__ __forceinline__ void epi_store(const EpiIn<T> &e, int row, T sum, T omega, T *y, const T *dotw,
-                                          double &dacc) {
+                                          double &dacc, T *y2 = nullptr) {
   T out;
-  if constexpr (EPI == EPI_Y) out = sum;
+  if constexpr (EPI == EPI_Y || EPI == EPI_RJ0) out = sum;
   else if constexpr (EPI == EPI_RESID) out = e.b - sum;
   else if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) out = e.x + omega * e.d * (e.b - sum);
   else out = e.y + sum;
   if constexpr (DOT) dacc += (double)(dotw == y ? out : e.w) * (double)out;
   y[row] = out;
+  if constexpr (EPI == EPI_RJ0) y2[row] = omega * e.d * out;
 }
 
 // The gathered operand of a product: x[c], or for EPI_JACOBI0 the first

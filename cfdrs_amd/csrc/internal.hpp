@@ -219,7 +219,7 @@ enum FinalizeOp : int {
                        // restart_dir if it exceeds 4x the recursive rr it replaces
 };
 
-enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4 };
+enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4, EPI_RJ0 = 5 };
 
 // ---- kernel launchers (kernels.hip) -------------------------------------
 // Diagonal-format product with epilogue (dia.hip); A.dia must hold and the
@@ -241,12 +241,15 @@ int xcd_grid(int units, int units_per_block, int cap = kGridCap);
 //   EPI_JACOBI0 two Jacobi sweeps from x = 0 in one pass: with
 //               x1 = omega * dinv .* b formed on the fly at every gathered
 //               column (x is not read), y = x1 + omega * dinv .* (b - A x1)
+//   EPI_RJ0     y = A x and y2 = omega * dinv .* y (row-block CSR only: a
+//               restriction that also forms the coarse level's first Jacobi
+//               iterate from zero; dinv and y2 are the coarse level's)
 // When dot_op != OP_NONE, also reduces sum_i dotw_i * y_i (dotw defaults to
 // x when null) and runs dot_op.  Skipped entirely when *s->done != 0 and
 // check_done is set.
 template <typename T>
 void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, double omega, T *y,
-               Scal *s, bool check_done, int dot_op, const T *dotw, RedWs &ws);
+               Scal *s, bool check_done, int dot_op, const T *dotw, RedWs &ws, T *y2 = nullptr);
 
 // Reductions: up to 2 dot products of (a0,b0), (a1,b1) then op.
 template <typename T>

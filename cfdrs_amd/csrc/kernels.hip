@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
                                                    const int *__restrict__ ci, const T *__restrict__ av,
                                                    const T *__restrict__ x, const T *__restrict__ b,
                                                    const T *__restrict__ dinv, T omega, T *y, const T *dotw,
-                                                   Scal *s, int check_done, double *partials) {
+                                                   Scal *s, int check_done, double *partials, T *y2) {
   if (check_done && s->done) return;
   constexpr int CAP = rb_cap<T>();
   constexpr int U = CAP / kBlock;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
         T sum = T(0);
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) sum += wsum[w];
-        epi_store<T, EPI, DOT>(e, r0, sum, omega, y, dotw, dacc);
+        epi_store<T, EPI, DOT>(e, r0, sum, omega, y, dotw, dacc, y2);
       }
       __syncthreads();
     } else {
@@ -291,14 +291,14 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
         T sum = T(0);
         for (int k = ks[0] + lane; k < ke; k += tpr) sum += prod[k];
         for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-        if (own && lane == 0) epi_store<T, EPI, DOT>(e[0], r0 + tid / tpr, sum, omega, y, dotw, dacc);
+        if (own && lane == 0) epi_store<T, EPI, DOT>(e[0], r0 + tid / tpr, sum, omega, y, dotw, dacc, y2);
       } else {
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
           if (tid + j * kBlock < nr) {
             T sum = T(0);
             for (int k = ks[j]; k < kr[j]; ++k) sum += prod[k];
-            epi_store<T, EPI, DOT>(e[j], r0 + tid + j * kBlock, sum, omega, y, dotw, dacc);
+            epi_store<T, EPI, DOT>(e[j], r0 + tid + j * kBlock, sum, omega, y, dotw, dacc, y2);
           }
         }
       }
@@ -316,12 +316,12 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
 
 template <typename T, int EPI, bool DOT>
 void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
-                   Scal *s, bool check_done, int op, RedWs &ws) {
+                   Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr) {
   const int nrb = A.nrb<T>();
-  const int blocks = xcd_grid(nrb, 1);
+  const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
   hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), nrb,
                      reinterpret_cast<const int2 *>(A.rb<T>()), A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega,
-                     y, dotw, s, check_done ? 1 : 0, ws.partials.p);
+                     y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_coef_grad_2d(int nx, int ny, double 
 // ---- launchers -----------------------------------------------------------------------
 template <typename T>
 void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, double omega, T *y, Scal *s,
-               bool check_done, int dot_op, const T *dotw, RedWs &ws) {
+               bool check_done, int dot_op, const T *dotw, RedWs &ws, T *y2) {
   const bool dot = dot_op != OP_NONE;
   if (!dotw) dotw = x;
 #define CFDLA_EPI_CASE(E)                                                                         \
@@ -649,6 +649,11 @@ void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, 
     CFDLA_EPI_CASE(EPI_JACOBI)
     CFDLA_EPI_CASE(EPI_ADD)
     CFDLA_EPI_CASE(EPI_JACOBI0)
+    case EPI_RJ0:
+      CFDLA_REQUIRE(!dot && y2 != nullptr && dinv != nullptr, CFDLA_ERR_INVALID_ARG,
+                    "restriction epilogue takes the coarse dinv and a second output, no dot");
+      launch_csr_rb<T, EPI_RJ0, false>(A, x, b, dinv, omega, y, dotw, s, check_done, dot_op, ws, y2);
+      break;
     default:
       throw Error(CFDLA_ERR_INVALID_ARG, "unknown CSR epilogue");
   }
@@ -802,7 +807,7 @@ void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, co
 
 #define CFDLA_INSTANTIATE(T)                                                                                   \
   template void csr_apply<T>(const DevCsr &, int, const T *, const T *, const T *, double, T *, Scal *, bool, \
-                             int, const T *, RedWs &);                                                        \
+                             int, const T *, RedWs &, T *);                                                   \
   template void dot2<T>(int, const T *, const T *, const T *, const T *, Scal *, bool, int, RedWs &);          \
   template void jacobi_first<T>(int, const T *, double, const T *, T *, Scal *, bool);                        \
   template void cg_update<T>(int, const T *, const T *, T *, T *, const T *, T *, Scal *, RedWs &);            \

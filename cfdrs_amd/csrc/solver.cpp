@@ -282,6 +282,9 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   for (int64_t i = 0; i <= h.n; ++i) rp32[i] = (int32_t)h.rp[i];
   rp.upload(rp32.data(), rp32.size());
   std::vector<int32_t> blocks;
+  // (Smaller caps for small matrices, to reach 8 blocks per CU, measured
+  // slower on every AMG level of the 128^3 problem: 7.6 -> 10.4 us per level-2
+  // sweep; the per-block descriptor loads and barriers dominate.)
   build_row_blocks(h, rb_cap<double>(), blocks);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
@@ -441,7 +444,19 @@ void AmgPrecond::update_values(const double *values) {
 }
 
 template <typename T>
-void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) {
+T *AmgPrecond::first_sweep_target(int l) {
+  DevLevel &L = lv[(size_t)l];
+  const bool coarsest = (size_t)l + 1 == lv.size();
+  if (coarsest && cfg.coarse_solver == 1) return nullptr;
+  const int down = coarsest ? cfg.pre_smooth_iterations + 10 : cfg.pre_smooth_iterations;
+  const int up = coarsest ? 0 : cfg.post_smooth_iterations;
+  if (down < 1 || (down >= 2 && L.A.nnz <= 8 * (int64_t)L.A.n)) return nullptr;
+  const int flips = down - 1 + up;
+  return flips % 2 == 0 ? L.x<T>() : L.t<T>();
+}
+
+template <typename T>
+void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op, bool first_ready) {
   DevLevel &L = lv[(size_t)l];
   RedWs &ws = ctx().ws;
   const int n = L.A.n;
@@ -472,7 +487,7 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) 
     cur ^= 1;
     k = 2;
   } else if (down >= 1) {
-    jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
+    if (!first_ready) jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
   } else {
     fill_vec<T>(n, T(0), buf[cur]);
   }
@@ -483,8 +498,15 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) 
   if (!coarsest) {
     DevLevel &C = lv[(size_t)l + 1];
     csr_apply<T>(L.A, EPI_RESID, buf[cur], b, nullptr, 0.0, L.r<T>(), s, chk, OP_NONE, nullptr, ws);
-    csr_apply<T>(L.R, EPI_Y, L.r<T>(), nullptr, nullptr, 0.0, C.b<T>(), s, chk, OP_NONE, nullptr, ws);
-    vcycle<T>(l + 1, C.b<T>(), C.x<T>(), s, chk, OP_NONE);
+    // b_c = R r; when the coarse level starts with a plain sweep from zero,
+    // the restriction also writes that first iterate, omega D_c^-1 b_c
+    T *first = first_sweep_target<T>(l + 1);
+    if (first)
+      csr_apply<T>(L.R, EPI_RJ0, L.r<T>(), nullptr, C.A.sdinv<T>(), w, C.b<T>(), s, chk, OP_NONE, nullptr, ws,
+                   first);
+    else
+      csr_apply<T>(L.R, EPI_Y, L.r<T>(), nullptr, nullptr, 0.0, C.b<T>(), s, chk, OP_NONE, nullptr, ws);
+    vcycle<T>(l + 1, C.b<T>(), C.x<T>(), s, chk, OP_NONE, first != nullptr);
     csr_apply<T>(L.P, EPI_ADD, C.x<T>(), nullptr, nullptr, 0.0, buf[cur], s, chk, OP_NONE, nullptr, ws);
     for (int j = 0; j < up; ++j) {
       // the last sweep writes x (the flip count fixes the parity) and, when
@@ -501,8 +523,8 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) 
   if (dot_op != OP_NONE) dot2<T>(n, b, x, nullptr, nullptr, s, chk, dot_op, ws);
 }
 
-template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool, int);
-template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool, int);
+template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool, int, bool);
+template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool, int, bool);
 
 // ---------------------------------------------------------------------------
 // Krylov drivers

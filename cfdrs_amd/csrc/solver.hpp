@@ -92,7 +92,13 @@ struct AmgPrecond : Precond {
     ensure_device();
     vcycle<float>(0, r, z, s, check_done, dot_op);
   }
-  template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done, int dot_op);
+  // first_ready: the caller's restriction already wrote the level's first
+  // Jacobi iterate from zero into first_sweep_target(l).
+  template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done, int dot_op,
+                                    bool first_ready = false);
+  // Buffer that level l's V-cycle writes its first Jacobi iterate to when it
+  // starts with a plain sweep from zero; nullptr when it starts otherwise.
+  template <typename T> T *first_sweep_target(int l);
 };
 
 struct SolveOut {
