@@ -187,7 +187,7 @@ struct Scal {
   int32_t term;
   int32_t kstep;  // GMRES: Arnoldi steps in the current cycle
   int32_t nupd;   // reliable-update CG: FP64 residual replacements performed
-  int32_t restart_dir;  // reliable-update CG: the next OP_CG_BETA restarts the direction (beta = 0)
+  int32_t restart_dir;  // reliable updates: the next OP_CG_BETA / OP_BI_RHO restarts the direction (beta = 0)
   int32_t nrestart;     // reliable-update CG: replacements that restarted the direction
   int32_t pad;
 };
@@ -214,7 +214,7 @@ enum FinalizeOp : int {
   OP_CG_RR = 5,        // rr = red0; iters++; done = 1 if rr <= thr2
   OP_CG_BETA = 6,      // rz_new = red0; beta = rz_new / rz
   OP_CG_RR_RZ = 7,     // rr = red0, rz_new = red1 (fused Jacobi update)
-  OP_BI_RHO = 8,       // BiCGSTAB: rho_new = red0 -> beta
+  OP_BI_RHO = 8,       // BiCGSTAB: rho_new = red0 -> beta (0 once after a restart_dir replacement)
   OP_BI_ALPHA = 9,     // BiCGSTAB: alpha = rho / (rhat . v)
   OP_BI_SNORM = 10,    // BiCGSTAB: ||s||^2 = red0
   OP_BI_OMEGA = 11,    // BiCGSTAB: omega = (t.s)/(t.t), red0 = t.s, red1 = t.t

@@ -79,7 +79,9 @@ __device__ void finalize(Scal *s, int op) {
         s->term = CFDLA_TERM_BREAKDOWN;
         s->done = 2;
       } else {
-        s->beta = (rho_new / s->rho) * (s->alpha / s->omega);
+        // a residual replacement far from the recurrence restarts p = r
+        s->beta = s->restart_dir ? 0.0 : (rho_new / s->rho) * (s->alpha / s->omega);
+        s->restart_dir = 0;
         s->rho = rho_new;
       }
     } break;

@@ -842,6 +842,128 @@ SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int
   return o;
 }
 
+// FP32-inner BiCGSTAB with reliable FP64 residual replacement (Sleijpen &
+// van der Vorst 1996, reliable updated residuals in hybrid Bi-CG methods):
+// the FP32 recurrence accumulates its solution increments in an FP32
+// correction xc and keeps running; its residual is replaced by the FP64
+// residual b - A (x + xc) when the lagged host poll shows it has dropped by
+// delta since the last replacement (OP_RELIABLE: done = 4 when that residual
+// is below the threshold), and when the recurrence reports convergence, to
+// confirm it.  A confirmation that fails restarts BiCGSTAB from the FP64
+// residual (rhat = r); a replacement far above the recursive residual
+// restarts the direction (p = r).
+SolveOut bicgstab_reliable(const DevCsr &A, Precond *M, const double *b, double *x, double thr, int64_t max_it,
+                           double delta, int chunk) {
+  CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
+  Context &c = ctx();
+  RedWs &ws = c.ws;
+  const int n = A.n;
+  double *r64 = c.dv_at(8, n);
+  float *r = c.fv_at(0, n), *rhat = c.fv_at(1, n), *p = c.fv_at(2, n), *v = c.fv_at(3, n);
+  float *ph = c.fv_at(4, n), *sv = c.fv_at(5, n), *sh = c.fv_at(6, n), *t = c.fv_at(7, n), *xc = c.fv_at(8, n);
+  Scal *s = c.s(0);
+  SolveOut o;
+  o.thr = thr;
+  scal_reset(s, thr * thr);
+  o.rn0 = o.rn = true_residual<double>(A, b, x, r64, s);
+  if (o.rn0 <= thr) {
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
+  f64_to_f32(n, r64, r, s, false);
+  fill_vec<float>(n, 0.0f, xc);
+  auto restart = [&]() {
+    copy_vec<float>(n, r, rhat, s, false);
+    fill_vec<float>(n, 0.0f, p);
+    fill_vec<float>(n, 0.0f, v);
+    scal_bicg_restart(s);
+  };
+  auto replace = [&](bool check_done) {
+    fold_correction(n, xc, x, s, check_done);
+    csr_apply<double>(A, EPI_RESID, x, b, nullptr, 0.0, r64, s, check_done, OP_RELIABLE, r64, ws);
+    f64_to_f32(n, r64, r, s, check_done);
+  };
+  restart();
+  if (chunk <= 0) chunk = auto_chunk(M, n);
+  int64_t enq = 0;
+  int32_t replacements = 0;
+  bool want = false;
+  Poller poll(s);
+  for (;;) {
+    const Scal *h = nullptr;
+    if (enq < max_it) {
+      const int64_t steps = std::min<int64_t>(chunk, max_it - enq);
+      for (int64_t k = 0; k < steps; ++k) {
+        if (want) {
+          ++replacements;
+          replace(true);
+          want = false;
+        }
+        dot2<float>(n, rhat, r, nullptr, nullptr, s, true, OP_BI_RHO, ws);
+        bicg_p_update<float>(n, r, v, p, s);
+        M->apply(p, ph, s, true);
+        csr_apply<float>(A, EPI_Y, ph, nullptr, nullptr, 0.0, v, s, true, OP_BI_ALPHA, rhat, ws);
+        bicg_s_update<float>(n, r, v, sv, s, ws);
+        M->apply(sv, sh, s, true);
+        csr_apply<float>(A, EPI_Y, sh, nullptr, nullptr, 0.0, t, s, true, OP_NONE, nullptr, ws);
+        dot2<float>(n, t, sv, t, t, s, true, OP_BI_OMEGA, ws);
+        bicg_x_r_update<float>(n, ph, sh, sv, t, xc, r, s, ws);
+      }
+      enq += steps;
+      poll.post();
+      h = poll.lagged();
+    } else {
+      h = poll.latest();
+    }
+    if (!h) continue;
+    if (h->done == 0 && enq < max_it) {
+      const double ref = h->nupd > 0 ? h->rr_true : o.rn0 * o.rn0;
+      const double de = std::min(0.5, delta * std::pow(10.0, (double)h->nrestart));
+      if (delta > 0.0 && h->nupd >= replacements && h->rr > 0.0 && h->rr <= de * de * ref) want = true;
+      continue;
+    }
+    sync_stream();
+    Scal now = c.read(s);
+    o.iters = now.iters;
+    if ((now.done == 1 || now.done == 3) && enq < max_it) {
+      // the recurrence reports convergence: confirm in FP64, restart if not
+      if (now.done == 3) axpy_scal<float>(n, ph, xc, s, 0);  // xc += alpha phat
+      ++replacements;
+      replace(false);
+      now = c.read(s);
+      if (now.done == 4) {
+        o.term = CFDLA_TERM_CONVERGED;
+        o.outer = now.nupd;
+        o.rn = std::sqrt(now.rr_true);
+        return o;
+      }
+      if (now.done == 2) {
+        o.term = now.term;
+        o.outer = now.nupd;
+        o.rn = std::sqrt(now.rr_true);
+        return o;
+      }
+      restart();
+      poll = Poller(s);
+      continue;
+    }
+    if (now.done == 4) {
+      o.term = CFDLA_TERM_CONVERGED;
+      o.outer = now.nupd;
+      o.rn = std::sqrt(now.rr_true);
+      return o;
+    }
+    // stopped (breakdown) or out of budget: fold and report the FP64 residual
+    if (now.done == 3) axpy_scal<float>(n, ph, xc, s, 0);
+    fold_correction(n, xc, x, s, false);
+    o.rn = true_residual<double>(A, b, x, r64, s);
+    o.outer = now.nupd + 1;
+    if (now.done == 2) o.term = now.term;
+    else o.term = o.rn <= thr ? CFDLA_TERM_CONVERGED : CFDLA_TERM_MAX_ITERATIONS;
+    return o;
+  }
+}
+
 SolveOut run_fp64(const DevCsr &A, Precond *M, const double *b, double *x, double thr, const cfdla_solver_config &cfg) {
   switch (cfg.kind) {
     case CFDLA_CG: return pcg<double>(A, M, b, x, thr, cfg.max_iter, cfg.check_every, true);
@@ -854,10 +976,12 @@ SolveOut run_fp64(const DevCsr &A, Precond *M, const double *b, double *x, doubl
 SolveOut run_refined(const DevCsr &A, Precond *M, const double *b, double *x, double thr,
                      const cfdla_solver_config &cfg) {
   CFDLA_REQUIRE(A.v32.p != nullptr, CFDLA_ERR_CONFIG, "matrix has no FP32 copy");
+  const double delta = cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1;
   if (cfg.kind == CFDLA_CG && cfg.reliable_interval >= 0)
     return pcg_reliable(A, M, b, x, thr, cfg.max_iter, cfg.reliable_interval,
-                        cfg.reliable_interval == 0 ? (cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1) : 0.0,
-                        cfg.check_every);
+                        cfg.reliable_interval == 0 ? delta : 0.0, cfg.check_every);
+  if (cfg.kind == CFDLA_BICGSTAB && cfg.reliable_interval >= 0)
+    return bicgstab_reliable(A, M, b, x, thr, cfg.max_iter, delta, cfg.check_every);
   Context &c = ctx();
   const int n = A.n;
   double *r64 = c.dv_at(8, n);

@@ -85,10 +85,11 @@ typedef struct {
   double inner_tol;        /* FP32_REFINED restarted refinement: relative tolerance of each inner solve */
   int32_t max_outer;       /* FP32_REFINED restarted refinement: refinement step budget */
   int32_t check_every;     /* host convergence poll interval in iterations (0 = auto) */
-  int32_t reliable_interval; /* FP32_REFINED CG runs one uninterrupted FP32 PCG whose residual is replaced
-                              * by the FP64 residual at convergence and: 0 = when it has dropped by
-                              * reliable_delta since the last replacement; k > 0 = every k iterations.
-                              * -1 = restarted refinement (inner_tol, max_outer) as for BiCGSTAB/GMRES */
+  int32_t reliable_interval; /* FP32_REFINED CG and BiCGSTAB run one uninterrupted FP32 recurrence whose
+                              * residual is replaced by the FP64 residual at convergence and: 0 = when it
+                              * has dropped by reliable_delta since the last replacement; k > 0 (CG only)
+                              * = every k iterations.  -1 = restarted refinement (inner_tol, max_outer),
+                              * which GMRES always uses */
   int32_t reserved;
   double reliable_delta;   /* residual reduction between replacements for reliable_interval 0 (<= 0: 1e-1) */
 } cfdla_solver_config;

@@ -256,15 +256,20 @@ def _momentum(nn=64, seed=3):
 
 
 @pytest.mark.parametrize("kind", ["bicgstab", "gmres"])
-@pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
-def test_nonsymmetric_solvers(kind, precision):
+@pytest.mark.parametrize("precision,reliable", [(ca.Precision.FP64, 0), (ca.Precision.FP32_REFINED, 0),
+                                                (ca.Precision.FP32_REFINED, -1)],
+                         ids=["fp64", "fp32-reliable", "fp32-restarted"])
+def test_nonsymmetric_solvers(kind, precision, reliable):
+    # FP32 BiCGSTAB runs with FP64 residual replacement (reliable 0) or
+    # restarted refinement (-1); GMRES always restarts
     orc = _momentum()
     rp, ci, v = orc.arrays()
     n = rp.size - 1
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
     b = P.uniform(31, n)
     x = np.zeros(n)
-    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(5000).with_precision(precision)
+    cfg = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(5000).with_precision(precision)
+           .with_reliable_interval(reliable))
     jac = ca.JacobiPreconditioner(a)
     if kind == "bicgstab":
         rep = ca.bicgstab_preconditioned(a, b, jac, x, cfg)
@@ -408,6 +413,26 @@ def test_refined_cg_reliable_updates_jacobi_config1():
     assert rep.termination == ca.Termination.Converged, rep
     assert rel_res(orc, x, b) <= 1e-10
     assert rep.iterations <= 1.3 * r64.iterations, (rep.iterations, r64.iterations)
+
+
+def test_refined_bicgstab_reliable_keeps_fp64_iteration_count():
+    orc = _momentum(nn=128, seed=5)
+    rp, ci, v = orc.arrays()
+    n = rp.size - 1
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    jac = ca.JacobiPreconditioner(a)
+    b = P.uniform(33, n)
+    reps = {}
+    for name, prec, rel in (("fp64", ca.Precision.FP64, 0), ("reliable", ca.Precision.FP32_REFINED, 0),
+                            ("restarted", ca.Precision.FP32_REFINED, -1)):
+        x = np.zeros(n)
+        cfg = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(prec)
+               .with_reliable_interval(rel))
+        reps[name] = ca.bicgstab_preconditioned(a, b, jac, x, cfg)
+        assert reps[name].termination == ca.Termination.Converged, (name, reps[name])
+        assert rel_res(orc, x, b) <= 1e-10, name
+    assert reps["reliable"].iterations <= 1.3 * reps["fp64"].iterations + 5, \
+        (reps["reliable"].iterations, reps["fp64"].iterations, reps["restarted"].iterations)
 
 
 def test_refined_cg_reliable_budget_and_initial_residual():
