@@ -380,6 +380,9 @@ def secondary_configs(args) -> dict:
     netres = {"systems": nets.count, "rows": int(batch.n_rows)}
     for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED):
         cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(50000).with_precision(prec)
+        # the row-bin schedule the autotuner picks for this size histogram
+        # (timed once per candidate, cached as JSON)
+        tuned = batch.autotune(cfg)
 
         def run():
             xb.zero_()
@@ -388,7 +391,8 @@ def secondary_configs(args) -> dict:
         ms, reps = _time_solves(run, steps)
         netres[prec.name] = {"ms": round(ms, 3), "systems_per_s": round(nets.count * 1e3 / ms, 1),
                              "max_iterations": int(max(r.iterations for r in reps)),
-                             "all_converged": all(r.termination == ca.Termination.Converged for r in reps)}
+                             "all_converged": all(r.termination == ca.Termination.Converged for r in reps),
+                             "schedule": tuned["best"]}
     out["networks_4096"] = netres
     log(f"networks: {netres}")
 

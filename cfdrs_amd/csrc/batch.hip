@@ -12,8 +12,10 @@
 // ticket per item, exits when the queue is drained).  A system with more
 // than wave_max_rows rows is one item for the whole 256-thread block; smaller
 // systems are grouped four to an item, one per wavefront, and reduce with
-// shuffles only.  Each team keeps the shared vectors (x, p) of its system in
-// LDS; every other vector is touched only by the lane that owns the row.
+// shuffles only.  Each team keeps the shared vectors of its system (x, p) in
+// LDS and each lane its rows' share of r, z, q and the FP32 correction in
+// registers, so during the iterations global memory serves only the matrix
+// and three per-row inputs.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -25,15 +27,20 @@ namespace cfdla {
 
 namespace {
 
-constexpr int kMaxBatchRows = 2048;  // rows of one block-team system (LDS: x and p, FP64)
+constexpr int kMaxBatchRows = 2048;  // rows of one block-team system
 constexpr int kWaveSlice = kMaxBatchRows / (kBlock / 64);
+// A lane owns rows tl, tl + size, ... of its team's system: at most 8 (2048
+// rows over 256 lanes, or 512 over 64).  Its share of r, z, q and the FP32
+// correction stays in registers; only x and p, which the product gathers
+// across lanes, live in LDS.
+constexpr int kRowsPerLane = kMaxBatchRows / kBlock;
+static_assert(kWaveSlice / 64 == kRowsPerLane, "wave and block teams own the same rows per lane");
 
 struct BArgs {
   const int *sys_row, *rp, *ci;
   const double *v64, *b, *jd64, *eq;
   const float *v32, *jd32;
-  double *x, *r64, *z64, *q64;
-  float *r32, *z32, *q32, *d32;
+  double *x;
   BatchReport *rep;
   const int *items;
   unsigned *queue;
@@ -80,60 +87,82 @@ struct BlockTeam {
   }
 };
 
-// r = b - A x over the team's rows (x in LDS); returns sum (eq_i r_i)^2.
+// (A v)_row over the team-local vector v in LDS.  (An ELL copy with a fixed,
+// unrolled slot loop took 256 VGPRs, one block per CU, and ran slower:
+// 39.7 against 28.5 ms for the 4,096 networks in FP64.)
+template <typename T, typename V>
+__device__ __forceinline__ T row_product(const BArgs &A, const T *vals, int row, int r0, const V *v) {
+  T s = T(0);
+  for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += vals[k] * (T)v[A.ci[k] - r0];
+  return s;
+}
+
+// r = b - A x over the lane's rows (x in LDS); returns sum (eq_i r_i)^2.
 template <class Team>
-__device__ double team_resid(const BArgs &A, int r0, int n, int tl, const double *xs, double *sh) {
+__device__ __forceinline__ double team_resid(const BArgs &A, int r0, int n, int tl, const double *xs,
+                             double (&r)[kRowsPerLane], double *sh) {
   double acc = 0.0;
-  for (int i = tl; i < n; i += Team::size) {
-    const int row = r0 + i;
-    double s = 0.0;
-    for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += A.v64[k] * xs[A.ci[k] - r0];
-    const double r = A.b[row] - s;
-    A.r64[row] = r;
-    const double e = A.eq[row] * r;
-    acc += e * e;
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) {
+    const int i = tl + j * Team::size;
+    r[j] = 0.0;
+    if (i < n) {
+      const int row = r0 + i;
+      r[j] = A.b[row] - row_product<double>(A, A.v64, row, r0, xs);
+      const double e = A.eq[row] * r[j];
+      acc += e * e;
+    }
   }
   return Team::sum(acc, sh);
 }
 
-// Jacobi-PCG on A d = r (r given in rv), accumulating d into acc (indexed by
-// local row).  Returns the termination; iters counts across calls.
-template <class Team, typename T>
-__device__ int team_cg(const BArgs &A, int r0, int n, int tl, T *acc, T *ps, T *rv, T *zv, T *qv, const T *vals,
-                       const T *jd, double thr2, int &iters, double *sh) {
+// Jacobi-PCG on A d = r, accumulating d into acc (x in LDS for FP64, the
+// lane's FP32 correction for FP32), p in LDS, r/z/q in registers.  Returns
+// the termination; iters counts across calls.
+template <class Team, typename T, typename Acc>
+__device__ __forceinline__ int team_cg(const BArgs &A, int r0, int n, int tl, Acc &acc, T *ps, T (&r)[kRowsPerLane],
+                       const T *vals, const T *jd, double thr2, int &iters, double *sh) {
+  T z[kRowsPerLane], q[kRowsPerLane];
   double rz = 0.0;
-  for (int i = tl; i < n; i += Team::size) {
-    const int row = r0 + i;
-    const T z = jd[row] * rv[row];
-    zv[row] = z;
-    ps[i] = z;
-    rz += (double)rv[row] * (double)z;
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) {
+    const int i = tl + j * Team::size;
+    z[j] = T(0);
+    if (i < n) {
+      z[j] = jd[r0 + i] * r[j];
+      ps[i] = z[j];
+      rz += (double)r[j] * (double)z[j];
+    }
   }
   rz = Team::sum(rz, sh);
   Team::sync();
   while (iters < A.max_it) {
     double pq = 0.0;
-    for (int i = tl; i < n; i += Team::size) {
-      const int row = r0 + i;
-      T s = T(0);
-      for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += vals[k] * ps[A.ci[k] - r0];
-      qv[row] = s;
-      pq += (double)ps[i] * (double)s;
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      q[j] = T(0);
+      if (i < n) {
+        q[j] = row_product<T>(A, vals, r0 + i, r0, ps);
+        pq += (double)ps[i] * (double)q[j];
+      }
     }
     pq = Team::sum(pq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) return CFDLA_TERM_NONPOSITIVE_CURVATURE;
     const double alpha = rz / pq;
     double rr = 0.0, rzn = 0.0;
-    for (int i = tl; i < n; i += Team::size) {
-      const int row = r0 + i;
-      acc[i] += (T)alpha * ps[i];
-      const T r = rv[row] - (T)alpha * qv[row];
-      rv[row] = r;
-      const T z = jd[row] * r;
-      zv[row] = z;
-      const double e = A.eq[row] * (double)r;
-      rr += e * e;
-      rzn += (double)r * (double)z;
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      if (i < n) {
+        const int row = r0 + i;
+        acc.add(j, i, (T)alpha * ps[i]);
+        r[j] = r[j] - (T)alpha * q[j];
+        z[j] = jd[row] * r[j];
+        const double e = A.eq[row] * (double)r[j];
+        rr += e * e;
+        rzn += (double)r[j] * (double)z[j];
+      }
     }
     Team::sum2(rr, rzn, sh);
     ++iters;
@@ -141,64 +170,87 @@ __device__ int team_cg(const BArgs &A, int r0, int n, int tl, T *acc, T *ps, T *
     if (rzn == 0.0 || !isfinite(rzn)) return CFDLA_TERM_BREAKDOWN;
     const double beta = rzn / rz;
     rz = rzn;
-    for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i] + (T)beta * ps[i];
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      if (i < n) ps[i] = z[j] + (T)beta * ps[i];
+    }
     Team::sync();
   }
   return CFDLA_TERM_MAX_ITERATIONS;
 }
+
+// Accumulators of team_cg: the FP64 solution in LDS, or the lane's FP32
+// correction in registers.
+struct LdsAcc {
+  double *x;
+  __device__ void add(int, int i, double v) { x[i] += v; }
+};
+struct RegAcc {
+  float d[kRowsPerLane];
+  __device__ void add(int j, int, float v) { d[j] += v; }
+};
 
 // FP32 Jacobi-PCG with reliable FP64 residual replacement, team-local (the
 // scheme of the single-system reliable-update PCG): the FP32 recurrence runs
 // on the residual scaled by 1 / ||r0|| and accumulates its increments in d;
 // when its equilibrated residual has dropped by delta since the latest
 // replacement, or passes the threshold, d is folded into the FP64 solution
-// in LDS and the residual is replaced by the FP64 one.  A replacement that
-// comes out above twice the recursive residual restarts the direction and
-// widens delta tenfold (up to 0.5): the cycle outran FP32 precision.
+// and the residual is replaced by the FP64 one.  A replacement that comes out
+// above twice the recursive residual restarts the direction and widens delta
+// tenfold (up to 0.5): the cycle outran FP32 precision.
 template <class Team>
-__device__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *xs, float *ps, double rr_true,
-                                double thr2, int &iters, int &nrep, double &rr_out, double *sh) {
+__device__ __forceinline__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *xs, float *ps,
+                                double (&r64)[kRowsPerLane], double rr_true, double thr2, int &iters, int &nrep,
+                                double &rr_out, double *sh) {
   const double sc = 1.0 / sqrt(rr_true);
   const double sc2 = sc * sc, thr2s = thr2 * sc2;
-  double d2 = A.delta * A.delta;  // widened tenfold (up to 0.5) after each direction restart
-  float *d = A.d32 + r0;
-  float *rv = A.r32, *zv = A.z32, *qv = A.q32;
+  double d2 = A.delta * A.delta;
   const float *vals = A.v32, *jd = A.jd32;
+  float r[kRowsPerLane], z[kRowsPerLane], q[kRowsPerLane], d[kRowsPerLane];
   auto load_residual = [&]() {  // r32 = sc * r64, z = D^-1 r32; returns r.z
     double rz = 0.0;
-    for (int i = tl; i < n; i += Team::size) {
-      const int row = r0 + i;
-      const float r = (float)(A.r64[row] * sc);
-      rv[row] = r;
-      const float z = jd[row] * r;
-      zv[row] = z;
-      rz += (double)r * (double)z;
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      r[j] = (float)(r64[j] * sc);
+      z[j] = i < n ? jd[r0 + i] * r[j] : 0.0f;
+      rz += (double)r[j] * (double)z[j];
     }
     return Team::sum(rz, sh);
   };
-  auto fold = [&]() {  // xs += d / sc, d = 0, then the FP64 residual
-    for (int i = tl; i < n; i += Team::size) {
-      xs[i] += (double)d[i] / sc;
-      d[i] = 0.0f;
+  auto fold = [&]() {  // x += d / sc, d = 0, then the FP64 residual
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      if (i < n) xs[i] += (double)d[j] / sc;
+      d[j] = 0.0f;
     }
     Team::sync();
-    rr_out = team_resid<Team>(A, r0, n, tl, xs, sh);
+    rr_out = team_resid<Team>(A, r0, n, tl, xs, r64, sh);
     ++nrep;
   };
-  for (int i = tl; i < n; i += Team::size) d[i] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) d[j] = 0.0f;
   double rz = load_residual();
-  for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i];
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) {
+    const int i = tl + j * Team::size;
+    if (i < n) ps[i] = z[j];
+  }
   Team::sync();
   double ref = rr_true * sc2;
   bool restart = false;
   while (iters < A.max_it) {
     double pq = 0.0;
-    for (int i = tl; i < n; i += Team::size) {
-      const int row = r0 + i;
-      float sv = 0.0f;
-      for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) sv += vals[k] * ps[A.ci[k] - r0];
-      qv[row] = sv;
-      pq += (double)ps[i] * (double)sv;
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      q[j] = 0.0f;
+      if (i < n) {
+        q[j] = row_product<float>(A, vals, r0 + i, r0, ps);
+        pq += (double)ps[i] * (double)q[j];
+      }
     }
     pq = Team::sum(pq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
@@ -207,16 +259,18 @@ __device__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *x
     }
     const double alpha = rz / pq;
     double rr = 0.0, rzn = 0.0;
-    for (int i = tl; i < n; i += Team::size) {
-      const int row = r0 + i;
-      d[i] += (float)alpha * ps[i];
-      const float r = rv[row] - (float)alpha * qv[row];
-      rv[row] = r;
-      const float z = jd[row] * r;
-      zv[row] = z;
-      const double e = A.eq[row] * (double)r;
-      rr += e * e;
-      rzn += (double)r * (double)z;
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      if (i < n) {
+        const int row = r0 + i;
+        d[j] += (float)alpha * ps[i];
+        r[j] = r[j] - (float)alpha * q[j];
+        z[j] = jd[row] * r[j];
+        const double e = A.eq[row] * (double)r[j];
+        rr += e * e;
+        rzn += (double)r[j] * (double)z[j];
+      }
     }
     Team::sum2(rr, rzn, sh);
     ++iters;
@@ -238,7 +292,11 @@ __device__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *x
     const double beta = restart ? 0.0 : rzn / rz;
     restart = false;
     rz = rzn;
-    for (int i = tl; i < n; i += Team::size) ps[i] = zv[r0 + i] + (float)beta * ps[i];
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      if (i < n) ps[i] = z[j] + (float)beta * ps[i];
+    }
     Team::sync();
   }
   fold();
@@ -246,7 +304,7 @@ __device__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *x
 }
 
 template <class Team>
-__device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double *psd, double *sh) {
+__device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double *psd, double *sh) {
   const int r0 = A.sys_row[sys], n = A.sys_row[sys + 1] - r0;
   double bb = 0.0;
   for (int i = tl; i < n; i += Team::size) {
@@ -259,16 +317,18 @@ __device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double
   const double thr = fmax(A.tol_abs, A.tol_rel * sqrt(bb));
   const double thr2 = thr * thr;
   Team::sync();
-  double rr = team_resid<Team>(A, r0, n, tl, xs, sh);
+  double r[kRowsPerLane];
+  double rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);
   const double rn0 = sqrt(rr);
   int iters = 0, outer = 0, term;
   if (rr <= thr2) {
     term = CFDLA_TERM_INITIAL_RESIDUAL;
   } else if (!A.refined) {
+    LdsAcc acc{xs};
     while (true) {
-      term = team_cg<Team, double>(A, r0, n, tl, xs, psd, A.r64, A.z64, A.q64, A.v64, A.jd64, thr2, iters, sh);
+      term = team_cg<Team, double>(A, r0, n, tl, acc, psd, r, A.v64, A.jd64, thr2, iters, sh);
       Team::sync();
-      rr = team_resid<Team>(A, r0, n, tl, xs, sh);  // explicit FP64 residual
+      rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);  // explicit FP64 residual
       if (rr <= thr2) {
         term = CFDLA_TERM_CONVERGED;
         break;
@@ -278,27 +338,34 @@ __device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double
         term = CFDLA_TERM_MAX_ITERATIONS;
         break;
       }
-      // the explicit residual now in r64 restarts the recurrence
+      // the explicit residual now in r restarts the recurrence
     }
   } else if (A.reliable) {
-    term = team_cg_reliable<Team>(A, r0, n, tl, xs, reinterpret_cast<float *>(psd), rr, thr2, iters, outer, rr, sh);
+    term = team_cg_reliable<Team>(A, r0, n, tl, xs, reinterpret_cast<float *>(psd), r, rr, thr2, iters, outer, rr,
+                                  sh);
   } else {
     float *ps = reinterpret_cast<float *>(psd);
-    float *d = A.d32 + r0;
     double rn = rn0;
     int stalls = 0;
     term = CFDLA_TERM_MAX_ITERATIONS;
     for (outer = 1; outer <= A.max_outer && iters < A.max_it; ++outer) {
       const double sc = 1.0 / rn;
-      for (int i = tl; i < n; i += Team::size) {
-        A.r32[r0 + i] = (float)(A.r64[r0 + i] * sc);
-        d[i] = 0.0f;
+      float r32[kRowsPerLane];
+      RegAcc d;
+#pragma unroll
+      for (int j = 0; j < kRowsPerLane; ++j) {
+        r32[j] = (float)(r[j] * sc);
+        d.d[j] = 0.0f;
       }
-      const int t = team_cg<Team, float>(A, r0, n, tl, d, ps, A.r32, A.z32, A.q32, A.v32, A.jd32,
+      const int t = team_cg<Team, float>(A, r0, n, tl, d, ps, r32, A.v32, A.jd32,
                                           A.inner_tol * A.inner_tol, iters, sh);
-      for (int i = tl; i < n; i += Team::size) xs[i] += rn * (double)d[i];
+#pragma unroll
+      for (int j = 0; j < kRowsPerLane; ++j) {
+        const int i = tl + j * Team::size;
+        if (i < n) xs[i] += rn * (double)d.d[j];
+      }
       Team::sync();
-      rr = team_resid<Team>(A, r0, n, tl, xs, sh);
+      rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);
       const double rn2 = sqrt(rr);
       if (rr <= thr2) {
         term = CFDLA_TERM_CONVERGED;
@@ -318,15 +385,15 @@ __device__ void solve_system(const BArgs &A, int sys, int tl, double *xs, double
   }
   for (int i = tl; i < n; i += Team::size) A.x[r0 + i] = xs[i];
   if (tl == 0) {
-    BatchReport r;
-    r.iterations = iters;
-    r.outer = outer;
-    r.rn0 = rn0;
-    r.rn = sqrt(rr);
-    r.thr = thr;
-    r.term = term;
-    r.pad = 0;
-    A.rep[sys] = r;
+    BatchReport rep;
+    rep.iterations = iters;
+    rep.outer = outer;
+    rep.rn0 = rn0;
+    rep.rn = sqrt(rr);
+    rep.thr = thr;
+    rep.term = term;
+    rep.pad = 0;
+    A.rep[sys] = rep;
   }
 }
 
@@ -416,12 +483,11 @@ void BatchSolver::upload() {
   d_ci.upload(host.ci.data(), std::max<size_t>(1, host.ci.size()));
   d_v64.upload(host.v.data(), std::max<size_t>(1, host.v.size()));
   d_v32.upload(v32.data(), std::max<size_t>(1, v32.size()));
+
   d_b.upload(rhs.data(), N);
   d_jd64.upload(jd.data(), N);
   d_eq.upload(eq.data(), N);
   d_jd32.upload(jd32.data(), N);
-  d_r64.ensure(N); d_z64.ensure(N); d_q64.ensure(N);
-  d_r32.ensure(N); d_z32.ensure(N); d_q32.ensure(N); d_d32.ensure(N);
   d_rep.ensure((size_t)std::max<int64_t>(1, nsys));
   d_queue.ensure(1);
   make_items();
@@ -447,13 +513,6 @@ void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *
   a.v32 = d_v32.p;
   a.jd32 = d_jd32.p;
   a.x = x;
-  a.r64 = d_r64.p;
-  a.z64 = d_z64.p;
-  a.q64 = d_q64.p;
-  a.r32 = d_r32.p;
-  a.z32 = d_z32.p;
-  a.q32 = d_q32.p;
-  a.d32 = d_d32.p;
   a.rep = d_rep.p;
   a.items = d_items.p;
   a.queue = d_queue.p;

@@ -58,8 +58,8 @@ struct BatchSolver {
   BatchSchedule sched;
   bool on_device = false;
   DevBuf<int32_t> d_sys_row, d_rp, d_ci, d_items;
-  DevBuf<double> d_v64, d_b, d_jd64, d_eq, d_r64, d_z64, d_q64;
-  DevBuf<float> d_v32, d_jd32, d_r32, d_z32, d_q32, d_d32;
+  DevBuf<double> d_v64, d_b, d_jd64, d_eq;
+  DevBuf<float> d_v32, d_jd32;
   DevBuf<BatchReport> d_rep;
   DevBuf<unsigned> d_queue;
   int n_block_items = 0, n_wave_items = 0, n_wave_sys = 0;
