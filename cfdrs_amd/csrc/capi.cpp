@@ -259,6 +259,7 @@ cfdla_status cfdla_spmv(cfdla_matrix m, const double *x, double *y) {
     require_ptr(m, "matrix");
     require_ptr(x, "x");
     require_ptr(y, "y");
+    if (m->host.n == 0) return;
     Context &c = ctx();
     double *dx = c.dv_at(9, std::max<int64_t>(1, m->host.m)), *dy = c.dv_at(10, std::max<int64_t>(1, m->host.n));
     CFDLA_HIP(hipMemcpyAsync(dx, x, m->host.m * 8, hipMemcpyHostToDevice, stream()));
@@ -271,6 +272,9 @@ cfdla_status cfdla_spmv(cfdla_matrix m, const double *x, double *y) {
 cfdla_status cfdla_spmv_device(cfdla_matrix m, int32_t dtype, const void *x, void *y) {
   return guarded([&] {
     require_ptr(m, "matrix");
+    CFDLA_REQUIRE(dtype == CFDLA_F64 || dtype == CFDLA_F32, CFDLA_ERR_INVALID_ARG,
+                  "dtype must be CFDLA_F64 or CFDLA_F32");
+    if (m->host.n == 0) return;
     Context &c = ctx();
     if (dtype == CFDLA_F64)
       csr_apply<double>(devmat(m), EPI_Y, static_cast<const double *>(x), nullptr, nullptr, 0.0,
@@ -286,6 +290,10 @@ cfdla_status cfdla_spmv_device(cfdla_matrix m, int32_t dtype, const void *x, voi
 cfdla_status cfdla_residual_device(cfdla_matrix m, const double *b, const double *x, double *r, double *rnorm) {
   return guarded([&] {
     require_ptr(m, "matrix");
+    if (m->host.n == 0) {
+      if (rnorm) *rnorm = 0.0;
+      return;
+    }
     Context &c = ctx();
     Scal *s = c.s(3);
     scal_reset(s, 0.0);
@@ -326,6 +334,7 @@ cfdla_status cfdla_precond_apply(cfdla_precond p, const double *r, double *z) {
     require_ptr(z, "z");
     Context &c = ctx();
     const int64_t n = p->m->host.n;
+    if (n == 0) return;
     double *dr = c.dv_at(9, std::max<int64_t>(1, n)), *dz = c.dv_at(10, std::max<int64_t>(1, n));
     CFDLA_HIP(hipMemcpyAsync(dr, r, n * 8, hipMemcpyHostToDevice, stream()));
     scal_reset(c.s(3), 0.0);
@@ -338,6 +347,9 @@ cfdla_status cfdla_precond_apply(cfdla_precond p, const double *r, double *z) {
 cfdla_status cfdla_precond_apply_device(cfdla_precond p, int32_t dtype, const void *r, void *z) {
   return guarded([&] {
     require_ptr(p, "preconditioner");
+    CFDLA_REQUIRE(dtype == CFDLA_F64 || dtype == CFDLA_F32, CFDLA_ERR_INVALID_ARG,
+                  "dtype must be CFDLA_F64 or CFDLA_F32");
+    if (p->m->host.n == 0) return;
     Context &c = ctx();
     scal_reset(c.s(3), 0.0);
     if (dtype == CFDLA_F64) p->p->apply(static_cast<const double *>(r), static_cast<double *>(z), c.s(3), false);

@@ -395,6 +395,7 @@ template <> float *DevLevel::r<float>() { return r32.p; }
 
 AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
   CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "AMG requires a square matrix");
+  CFDLA_REQUIRE(a.n >= 1, CFDLA_ERR_INVALID_ARG, "AMG requires at least one row");
   CFDLA_REQUIRE(c.max_levels >= 1, CFDLA_ERR_CONFIG, "AMG max_levels must be >= 1");
   CFDLA_REQUIRE(c.pre_smooth_iterations >= 0 && c.post_smooth_iterations >= 0, CFDLA_ERR_CONFIG,
                 "AMG smoothing counts must be >= 0");
@@ -1044,6 +1045,12 @@ SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, c
   CFDLA_REQUIRE(cfg.reliable_interval >= -1, CFDLA_ERR_CONFIG, "reliable_interval must be >= -1");
   CFDLA_REQUIRE(std::isfinite(cfg.tol_abs) && std::isfinite(cfg.tol_rel) && cfg.tol_abs >= 0.0 && cfg.tol_rel >= 0.0,
                 CFDLA_ERR_CONFIG, "tolerances must be finite and non-negative");
+  if (A.n == 0) {  // nothing to solve: ||b|| = 0 meets any threshold
+    SolveOut o;
+    o.thr = cfg.tol_abs;
+    o.term = CFDLA_TERM_INITIAL_RESIDUAL;
+    return o;
+  }
   std::unique_ptr<IdentityPrecond> ident;
   if (!M) {
     ident = std::make_unique<IdentityPrecond>(A.n);
