@@ -393,6 +393,24 @@ def secondary_configs(args) -> dict:
                              "max_iterations": int(max(r.iterations for r in reps)),
                              "all_converged": all(r.termination == ca.Termination.Converged for r in reps),
                              "schedule": tuned["best"]}
+    # design-loop refresh: widths -> conductances -> assembled values, right-
+    # hand side and preconditioner factors, on the device against the host
+    w_dev = torch.from_numpy(nets.width).to(dev)
+    l_dev = torch.from_numpy(nets.length).to(dev)
+
+    def refresh_device():
+        g, _ = NW.rect_conductance_device(w_dev, NW.CHANNEL_HEIGHT, l_dev)
+        batch.update_conductance_device(g)
+
+    def refresh_host():
+        g, _ = NW.rect_conductance(nets.width, NW.CHANNEL_HEIGHT, nets.length)
+        batch.update_conductance(g)
+
+    ms_dev, _ = _time_solves(refresh_device, steps)
+    ms_host, _ = _time_solves(refresh_host, 1, warmup=0)
+    netres["refresh_device_ms"] = round(ms_dev, 3)
+    netres["refresh_host_ms"] = round(ms_host, 1)
+    netres["edges"] = int(nets.edge_i.size)
     out["networks_4096"] = netres
     log(f"networks: {netres}")
 

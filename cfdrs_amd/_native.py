@@ -116,6 +116,7 @@ SIGNATURES = {
                                         _VP, _VP, _VP, _VP]),
     "cfdla_batch_create": (C.c_int, [C.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(_VP)]),
     "cfdla_batch_update_conductance": (C.c_int, [_VP, _VP]),
+    "cfdla_batch_update_conductance_device": (C.c_int, [_VP, _VP]),
     "cfdla_batch_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
     "cfdla_batch_export": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "cfdla_batch_set_schedule": (C.c_int, [_VP, C.POINTER(BatchSchedule)]),
@@ -127,6 +128,7 @@ SIGNATURES = {
     "cfdla_batch_destroy": (C.c_int, [_VP]),
     "cfdla_synth_networks": (C.c_int, [C.c_int64, C.c_uint64, C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP, _VP]),
     "cfdla_rect_conductance": (C.c_int, [C.c_int64, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "cfdla_rect_conductance_device": (C.c_int, [C.c_int64, _VP, _VP, _VP, C.c_double, _VP, _VP]),
 }
 
 _lib = None

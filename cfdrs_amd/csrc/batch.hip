@@ -430,8 +430,14 @@ __global__ void k_reset_queue(unsigned *q) {
 
 void BatchSolver::build(const NetworkInput &in) {
   nsys = in.nsys;
-  assemble_networks(in, host, rhs);
+  assemble_networks(in, host, rhs, &plan);
   nrows = host.n;
+  nedges = in.edge_off[nsys];
+  isdir_h.assign(in.is_dirichlet, in.is_dirichlet + nrows);
+  dirval_h.assign(in.dirichlet_value, in.dirichlet_value + nrows);
+  neumann_h.clear();
+  if (in.neumann) neumann_h.assign(in.neumann, in.neumann + nrows);
+  device_newer = false;
   sys_row.resize((size_t)nsys + 1);
   for (int64_t s = 0; s <= nsys; ++s) sys_row[s] = (int32_t)in.node_off[s];
   for (int64_t s = 0; s < nsys; ++s)
@@ -490,6 +496,15 @@ void BatchSolver::upload() {
   d_jd32.upload(jd32.data(), N);
   d_rep.ensure((size_t)std::max<int64_t>(1, nsys));
   d_queue.ensure(1);
+  d_slot_off.upload(plan.slot_off.data(), plan.slot_off.size());
+  d_slot_src.upload(plan.slot_src.data(), std::max<size_t>(1, plan.slot_src.size()));
+  d_rhs_off.upload(plan.rhs_off.data(), plan.rhs_off.size());
+  d_rhs_edge.upload(plan.rhs_edge.data(), plan.rhs_edge.size());
+  d_rhs_node.upload(plan.rhs_node.data(), plan.rhs_node.size());
+  d_isdir.upload(isdir_h.data(), isdir_h.size());
+  d_dirval.upload(dirval_h.data(), dirval_h.size());
+  if (!neumann_h.empty()) d_neumann.upload(neumann_h.data(), neumann_h.size());
+  d_bad.ensure(1);
   make_items();
   sync_stream();
   on_device = true;
@@ -540,8 +555,184 @@ void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *
 void BatchSolver::set_values(const double *g, const NetworkInput &in) {
   NetworkInput copy = in;
   copy.g = g;
-  assemble_networks(copy, host, rhs);
+  assemble_networks(copy, host, rhs, &plan);
+  device_newer = false;
   if (on_device) upload();
+}
+
+namespace {
+// Device re-assembly from conductances, mirroring assemble_networks (and the
+// Jacobi / equilibration factors of BatchSolver::upload) operation for
+// operation: sequential sums in the recorded order, no contraction into FMA
+// (the host side is built with -ffp-contract=off), so the values are the
+// host's bit for bit.
+__global__ __launch_bounds__(kBlock) void k_net_check(int64_t ne, const double *__restrict__ g, int *bad) {
+  int local = 0;
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock)
+    local += (!isfinite(g[e]) || !(g[e] > 0.0)) ? 1 : 0;
+  if (local) atomicAdd(bad, local);
+}
+
+__global__ void k_net_clear(int *bad) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *bad = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_net_values(int64_t nnz, const int *__restrict__ slot_off,
+                                                       const int *__restrict__ slot_src,
+                                                       const double *__restrict__ g, double *__restrict__ v64,
+                                                       float *__restrict__ v32) {
+#pragma clang fp contract(off)
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock) {
+    double v = 0.0;
+    for (int t = slot_off[k]; t < slot_off[k + 1]; ++t) {
+      const int src = slot_src[t];
+      const double c = src == 0 ? 1.0 : (src > 0 ? g[src - 1] : -g[-src - 1]);
+      v = t == slot_off[k] ? c : v + c;
+    }
+    v64[k] = v;
+    v32[k] = (float)v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_net_rows(int64_t n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const double *__restrict__ v64, const int *__restrict__ rhs_off,
+                                                     const int *__restrict__ rhs_edge,
+                                                     const int *__restrict__ rhs_node, const double *__restrict__ g,
+                                                     const unsigned char *__restrict__ isdir,
+                                                     const double *__restrict__ dirval,
+                                                     const double *__restrict__ neumann, double *__restrict__ b,
+                                                     double *__restrict__ jd64, float *__restrict__ jd32,
+                                                     double *__restrict__ eq) {
+#pragma clang fp contract(off)
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    if (isdir[i]) {
+      b[i] = dirval[i];
+    } else {
+      double acc = 0.0;
+      for (int t = rhs_off[i]; t < rhs_off[i + 1]; ++t) acc += g[rhs_edge[t]] * dirval[rhs_node[t]];
+      if (neumann) acc += neumann[i];
+      b[i] = acc;
+    }
+    double d = 0.0, mx = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      if (ci[k] == i) d = v64[k];
+      mx = fmax(mx, fabs(v64[k]));
+    }
+    const double j = fabs(d) <= 2.220446049250313e-16 ? 1.0 : 1.0 / d;
+    jd64[i] = j;
+    jd32[i] = (float)j;
+    eq[i] = (mx > 2.220446049250313e-16 && isfinite(mx)) ? 1.0 / mx : 1.0;
+  }
+}
+}  // namespace
+
+void BatchSolver::set_values_device(const double *g) {
+  if (!on_device) upload();
+  if (nedges > 0) {
+    hipLaunchKernelGGL(k_net_clear, dim3(1), dim3(64), 0, stream(), d_bad.p);
+    hipLaunchKernelGGL(k_net_check, dim3(blocks_for(nedges, kBlock * 4)), dim3(kBlock), 0, stream(), nedges, g,
+                       d_bad.p);
+    int bad = 0;
+    CFDLA_HIP(hipMemcpyAsync(&bad, d_bad.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    sync_stream();
+    CFDLA_REQUIRE(bad == 0, CFDLA_ERR_INVALID_ARG, "Non-positive or non-finite conductance encountered in network assembly");
+  }
+  const int64_t nnz = host.nnz();
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_net_values, dim3(blocks_for(nnz, kBlock * 4)), dim3(kBlock), 0, stream(), nnz,
+                       d_slot_off.p, d_slot_src.p, g, d_v64.p, d_v32.p);
+  if (nrows > 0)
+    hipLaunchKernelGGL(k_net_rows, dim3(blocks_for(nrows, kBlock * 4)), dim3(kBlock), 0, stream(), nrows, d_rp.p,
+                       d_ci.p, d_v64.p, d_rhs_off.p, d_rhs_edge.p, d_rhs_node.p, g, d_isdir.p, d_dirval.p,
+                       neumann_h.empty() ? nullptr : d_neumann.p, d_b.p, d_jd64.p, d_jd32.p, d_eq.p);
+  CFDLA_HIP(hipGetLastError());
+  sync_stream();
+  device_newer = true;
+}
+
+void BatchSolver::download(std::vector<double> &values, std::vector<double> &rhs_out) const {
+  if (!device_newer) {
+    values = host.v;
+    rhs_out = rhs;
+    return;
+  }
+  values.resize(host.v.size());
+  rhs_out.resize((size_t)nrows);
+  if (!values.empty()) d_v64.download(values.data(), values.size());
+  if (nrows > 0) d_b.download(rhs_out.data(), (size_t)nrows);
+  sync_stream();
+}
+
+namespace {
+// Rectangular-duct conductance on the device, the host formulas of
+// network.cpp operation for operation (pi^5 is passed in from the host);
+// tanh comes from the device math library, so G may differ from the host's
+// in the last bits.
+__device__ double po_of_dev(double eps, double pi5) {
+#pragma clang fp contract(off)
+  if (eps == 0.0) return 96.0;
+  const double kPi = 3.14159265358979323846;
+  const double t1 = (1.0 + eps) * (1.0 + eps);
+  const double t2 = 1.0 - (192.0 * eps / pi5) * tanh(kPi / (2.0 * eps));
+  return 96.0 / (t1 * t2);
+}
+
+__device__ double dpo_deps_dev(double eps, double pi5) {
+#pragma clang fp contract(off)
+  const double kPi = 3.14159265358979323846;
+  const double c = 192.0 / pi5;
+  const double u = kPi / (2.0 * eps);
+  const double th = tanh(u);
+  const double sech2 = 1.0 - th * th;
+  const double T = 1.0 - c * eps * th;
+  const double dT = -c * th + c * u * sech2;
+  const double q = (1.0 + eps) * (1.0 + eps) * T;
+  const double dq = 2.0 * (1.0 + eps) * T + (1.0 + eps) * (1.0 + eps) * dT;
+  return -96.0 * dq / (q * q);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rect_conductance(int64_t n, const double *__restrict__ w,
+                                                             const double *__restrict__ h,
+                                                             const double *__restrict__ L, double mu, double pi5,
+                                                             double *__restrict__ g, double *__restrict__ dgdw) {
+#pragma clang fp contract(off)
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const double we = w[e], he = h[e], Le = L[e];
+    // G = 1 / R, R = Po(eps) mu L / (2 A Dh^2)
+    const double area = we * he;
+    const double dh = 4.0 * area / (2.0 * (we + he));
+    const double lo = we <= he ? we : he, hi = we >= he ? we : he;
+    const double R = (po_of_dev(lo / hi, pi5) * mu * Le) / (2.0 * area * dh * dh);
+    g[e] = 1.0 / R;
+    if (dgdw) {
+      const double s = we + he;
+      const double f = 4.0 * we * we * we * he * he * he / (s * s);
+      const double df = 4.0 * he * he * he * (3.0 * we * we / (s * s) - 2.0 * we * we * we / (s * s * s));
+      double eps, deps;
+      if (we >= he) {
+        eps = he / we;
+        deps = -he / (we * we);
+      } else {
+        eps = we / he;
+        deps = 1.0 / he;
+      }
+      const double po = po_of_dev(eps, pi5);
+      const double dpo = dpo_deps_dev(eps, pi5) * deps;
+      const double Rf = po * mu * Le / (2.0 * f);
+      const double dR = mu * Le / 2.0 * (dpo * f - po * df) / (f * f);
+      dgdw[e] = -dR / (Rf * Rf);
+    }
+  }
+}
+}  // namespace
+
+void rect_conductance_device(int64_t n, const double *w, const double *h, const double *L, double mu, double *g,
+                             double *dgdw) {
+  if (n == 0) return;
+  const double pi5 = std::pow(3.14159265358979323846, 5);
+  hipLaunchKernelGGL(k_rect_conductance, dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, w, h, L, mu,
+                     pi5, g, dgdw);
+  CFDLA_HIP(hipGetLastError());
 }
 
 namespace {

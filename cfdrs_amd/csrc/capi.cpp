@@ -734,8 +734,20 @@ cfdla_status cfdla_batch_export(cfdla_batch b, int64_t *row_ptr, int64_t *col_in
     const HostCsr &a = b->solver.host;
     if (row_ptr) std::copy(a.rp.begin(), a.rp.end(), row_ptr);
     if (col_indices) for (size_t k = 0; k < a.ci.size(); ++k) col_indices[k] = a.ci[k];
-    if (values) std::copy(a.v.begin(), a.v.end(), values);
-    if (rhs) std::copy(b->solver.rhs.begin(), b->solver.rhs.end(), rhs);
+    if (values || rhs) {
+      std::vector<double> v, r;
+      b->solver.download(v, r);  // device-formed values when newer than the host assembly
+      if (values) std::copy(v.begin(), v.end(), values);
+      if (rhs) std::copy(r.begin(), r.end(), rhs);
+    }
+  });
+}
+
+cfdla_status cfdla_batch_update_conductance_device(cfdla_batch b, const double *conductance_dev) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    if (!b->g.empty()) require_ptr(conductance_dev, "conductance");
+    b->solver.set_values_device(conductance_dev);
   });
 }
 
@@ -889,6 +901,20 @@ cfdla_status cfdla_synth_networks(int64_t count, uint64_t seed_base, int64_t nmi
     }
     if (width) std::copy(s.width.begin(), s.width.end(), width);
     if (length) std::copy(s.length.begin(), s.length.end(), length);
+  });
+}
+
+cfdla_status cfdla_rect_conductance_device(int64_t n, const double *width, const double *height, const double *length,
+                                           double viscosity, double *g, double *dg_dw) {
+  return guarded([&] {
+    CFDLA_REQUIRE(n >= 0, CFDLA_ERR_INVALID_ARG, "n must be non-negative");
+    if (n) {
+      require_ptr(width, "width");
+      require_ptr(height, "height");
+      require_ptr(length, "length");
+      require_ptr(g, "g");
+    }
+    rect_conductance_device(n, width, height, length, viscosity, g, dg_dw);
   });
 }
 

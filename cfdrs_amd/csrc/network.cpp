@@ -150,7 +150,7 @@ void synth_networks(int64_t count, uint64_t seed_base, int64_t nmin, int64_t nma
   }
 }
 
-void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &rhs) {
+void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &rhs, AssemblyPlan *plan) {
   const int64_t ns = in.nsys;
   const int64_t N = in.node_off[ns];
   CFDLA_REQUIRE(N < INT32_MAX, CFDLA_ERR_INVALID_ARG, "too many network nodes");
@@ -159,11 +159,26 @@ void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &
   A.ci.clear();
   A.v.clear();
   rhs.assign((size_t)N, 0.0);
-  std::vector<std::vector<std::pair<int32_t, double>>> rows;
+  // entries of a row in push order: (column, value, source code of the plan)
+  struct Entry {
+    int32_t col;
+    double v;
+    int32_t src;
+  };
+  std::vector<std::vector<Entry>> rows;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> rhs_src;  // (edge, dirichlet node) per free row
+  if (plan) {
+    plan->slot_off.assign(1, 0);
+    plan->slot_src.clear();
+    plan->rhs_off.assign((size_t)N + 1, 0);
+    plan->rhs_edge.clear();
+    plan->rhs_node.clear();
+  }
   for (int64_t s = 0; s < ns; ++s) {
     const int64_t r0 = in.node_off[s], n = in.node_off[s + 1] - r0;
     CFDLA_REQUIRE(n >= 1, CFDLA_ERR_INVALID_ARG, "network with no nodes");
     rows.assign((size_t)n, {});
+    rhs_src.assign((size_t)n, {});
     std::vector<char> touched((size_t)n, 0);
     bool any_dirichlet = false;
     for (int64_t i = 0; i < n; ++i) any_dirichlet |= in.is_dirichlet[r0 + i] != 0;
@@ -177,17 +192,20 @@ void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &
       CFDLA_REQUIRE(c > 0.0, CFDLA_ERR_INVALID_ARG, "Non-positive conductance encountered in network assembly");
       touched[i] = touched[j] = 1;
       const bool di = in.is_dirichlet[r0 + i] != 0, dj = in.is_dirichlet[r0 + j] != 0;
+      const int32_t plus = (int32_t)(e + 1), minus = -(int32_t)(e + 1);
       if (!di && !dj) {
-        rows[i].emplace_back((int32_t)i, c);
-        rows[j].emplace_back((int32_t)j, c);
-        rows[i].emplace_back((int32_t)j, -c);
-        rows[j].emplace_back((int32_t)i, -c);
+        rows[i].push_back({(int32_t)i, c, plus});
+        rows[j].push_back({(int32_t)j, c, plus});
+        rows[i].push_back({(int32_t)j, -c, minus});
+        rows[j].push_back({(int32_t)i, -c, minus});
       } else if (di && !dj) {
-        rows[j].emplace_back((int32_t)j, c);
+        rows[j].push_back({(int32_t)j, c, plus});
         rhs[r0 + j] += c * in.dirichlet_value[r0 + i];
+        rhs_src[j].emplace_back((int32_t)e, (int32_t)(r0 + i));
       } else if (!di && dj) {
-        rows[i].emplace_back((int32_t)i, c);
+        rows[i].push_back({(int32_t)i, c, plus});
         rhs[r0 + i] += c * in.dirichlet_value[r0 + j];
+        rhs_src[i].emplace_back((int32_t)e, (int32_t)(r0 + j));
       }
     }
     for (int64_t i = 0; i < n; ++i) {
@@ -197,21 +215,32 @@ void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &
     }
     for (int64_t i = 0; i < n; ++i)
       if (in.is_dirichlet[r0 + i]) {
-        rows[i].emplace_back((int32_t)i, 1.0);
+        rows[i].push_back({(int32_t)i, 1.0, 0});
         rhs[r0 + i] = in.dirichlet_value[r0 + i];
+        rhs_src[i].clear();
       }
     for (int64_t i = 0; i < n; ++i) {
       auto &row = rows[i];
-      std::stable_sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+      std::stable_sort(row.begin(), row.end(), [](const Entry &a, const Entry &b) { return a.col < b.col; });
       for (size_t k = 0; k < row.size(); ++k) {
-        if (k > 0 && row[k].first == row[k - 1].first) {
-          A.v.back() += row[k].second;
+        if (k > 0 && row[k].col == row[k - 1].col) {
+          A.v.back() += row[k].v;
         } else {
-          A.ci.push_back((int32_t)(row[k].first + r0));
-          A.v.push_back(row[k].second);
+          A.ci.push_back((int32_t)(row[k].col + r0));
+          A.v.push_back(row[k].v);
+          if (plan && k > 0) plan->slot_off.push_back((int32_t)plan->slot_src.size());
         }
+        if (plan) plan->slot_src.push_back(row[k].src);
       }
+      if (plan && !row.empty()) plan->slot_off.push_back((int32_t)plan->slot_src.size());
       A.rp[r0 + i + 1] = (int64_t)A.ci.size();
+      if (plan) {
+        for (auto &t : rhs_src[i]) {
+          plan->rhs_edge.push_back(t.first);  // e runs over the batch's edges already
+          plan->rhs_node.push_back(t.second);
+        }
+        plan->rhs_off[(size_t)(r0 + i + 1)] = (int32_t)plan->rhs_edge.size();
+      }
     }
   }
   CFDLA_REQUIRE(A.nnz() < INT32_MAX, CFDLA_ERR_INVALID_ARG, "too many network matrix entries");

@@ -32,8 +32,25 @@ struct NetworkInput {
   const double *neumann;           // per node (global), nullable
 };
 
-// Block-diagonal matrix (global column indices) and right-hand side.
-void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &rhs);
+// The order in which assemble_networks forms every value, so a device kernel
+// can re-form them from new conductances bit for bit: slot k of the CSR is
+// the sequential sum of its sources slot_src[slot_off[k] .. slot_off[k+1])
+// (e + 1: +g_e, -(e + 1): -g_e, 0: the 1 of a Dirichlet row; e indexes the
+// batch's edges); free row i's right-hand side is the sequential sum of
+// g_{rhs_edge[t]} * p_D[rhs_node[t]] over t in rhs_off[i] .. rhs_off[i+1],
+// plus its Neumann source.
+struct AssemblyPlan {
+  std::vector<int32_t> slot_off, slot_src;
+  std::vector<int32_t> rhs_off, rhs_edge, rhs_node;
+};
+
+// Block-diagonal matrix (global column indices) and right-hand side; fills
+// plan when given.
+void assemble_networks(const NetworkInput &in, HostCsr &A, std::vector<double> &rhs, AssemblyPlan *plan = nullptr);
+
+// G = 1 / R and dG/dw of rectangular ducts on the device (arrays of n).
+void rect_conductance_device(int64_t n, const double *w, const double *h, const double *L, double mu, double *g,
+                             double *dgdw);
 
 // Edge-wise adjoint gradient (batch.hip); ei/ej global node ids, all device arrays.
 void network_gradient(int64_t ne, const int32_t *ei, const int32_t *ej, const uint8_t *isdir, const double *p,
@@ -61,12 +78,27 @@ struct BatchSolver {
   DevBuf<double> d_v64, d_b, d_jd64, d_eq;
   DevBuf<float> d_v32, d_jd32;
   DevBuf<BatchReport> d_rep;
+  // device re-assembly from new conductances (set_values_device)
+  AssemblyPlan plan;
+  int64_t nedges = 0;
+  std::vector<uint8_t> isdir_h;
+  std::vector<double> dirval_h, neumann_h;
+  DevBuf<int32_t> d_slot_off, d_slot_src, d_rhs_off, d_rhs_edge, d_rhs_node, d_bad;
+  DevBuf<uint8_t> d_isdir;
+  DevBuf<double> d_dirval, d_neumann;
+  bool device_newer = false;  // values / rhs were re-formed on the device since the last host assembly
   DevBuf<unsigned> d_queue;
   int n_block_items = 0, n_wave_items = 0, n_wave_sys = 0;
   void build(const NetworkInput &in);
   void upload();
   void make_items();
   void set_values(const double *g, const NetworkInput &in);
+  // Re-forms values, right-hand side, Jacobi and equilibration factors on the
+  // device from device conductances g (n_edges), as assemble_networks does;
+  // rejects non-positive or non-finite conductances before changing anything.
+  void set_values_device(const double *g);
+  // Host copies of the (possibly device-formed) values and right-hand side.
+  void download(std::vector<double> &values, std::vector<double> &rhs_out) const;
   // x: device FP64 packed solutions (initial guess in, solution out); reports: host array of nsys or null
   void solve(double *x, const cfdla_solver_config &cfg, BatchReport *reports, const double *rhs_override = nullptr);
 };

@@ -71,6 +71,22 @@ def rect_conductance(width, height, length, viscosity=WATER_VISCOSITY):
     return g, dg
 
 
+def rect_conductance_device(width, height, length, viscosity=WATER_VISCOSITY):
+    """(G, dG/dw) of rectangular ducts on float64 CUDA tensors (same shape)."""
+    import torch
+
+    w = width.contiguous()
+    h = torch.broadcast_to(torch.as_tensor(height, dtype=torch.float64, device=w.device), w.shape).contiguous()
+    L = torch.broadcast_to(torch.as_tensor(length, dtype=torch.float64, device=w.device), w.shape).contiguous()
+    if w.dtype != torch.float64 or not w.is_cuda:
+        raise N.CfdlaError(N.ERR_INVALID_ARG, "width must be a float64 CUDA tensor")
+    g, dg = torch.empty_like(w), torch.empty_like(w)
+    bind_torch_stream()
+    N.call("cfdla_rect_conductance_device", w.numel(), w.data_ptr(), h.data_ptr(), L.data_ptr(), viscosity,
+           g.data_ptr(), dg.data_ptr())
+    return g, dg
+
+
 def _report(r: N.Report) -> SolveReport:
     return SolveReport(r.iterations, r.initial_residual_norm, r.final_residual_norm, r.threshold,
                        Termination(r.termination), r.outer_iterations)
@@ -85,13 +101,14 @@ class NetworkBatch:
         self.edge_offset = np.ascontiguousarray(edge_offset, np.int64)
         self.edge_i = np.ascontiguousarray(edge_i, np.int64)
         self.edge_j = np.ascontiguousarray(edge_j, np.int64)
-        self.conductance = np.ascontiguousarray(conductance, np.float64)
+        self._g_host = np.ascontiguousarray(conductance, np.float64)
+        self._g_dev = None  # device tensor of the latest device update, copied to the host on demand
         self.is_dirichlet = np.ascontiguousarray(is_dirichlet, np.uint8)
         self.dirichlet_value = np.ascontiguousarray(dirichlet_value, np.float64)
         self.neumann = None if neumann is None else np.ascontiguousarray(neumann, np.float64)
         h = C.c_void_p()
         N.call("cfdla_batch_create", self.node_offset.size - 1, N.ptr(self.node_offset), N.ptr(self.edge_offset),
-               N.ptr(self.edge_i), N.ptr(self.edge_j), N.ptr(self.conductance), N.ptr(self.is_dirichlet),
+               N.ptr(self.edge_i), N.ptr(self.edge_j), N.ptr(self._g_host), N.ptr(self.is_dirichlet),
                N.ptr(self.dirichlet_value), N.ptr(self.neumann), C.byref(h))
         self._h = h
         ns, nr, nz = C.c_int64(), C.c_int64(), C.c_int64()
@@ -104,10 +121,30 @@ class NetworkBatch:
             raise ValueError("batch has been closed")
         return self._h
 
+    @property
+    def conductance(self) -> np.ndarray:
+        """Edge conductances of the assembled systems (host copy)."""
+        if self._g_dev is not None:
+            self._g_host = self._g_dev.detach().cpu().numpy().copy()
+            self._g_dev = None
+        return self._g_host
+
     def update_conductance(self, conductance) -> None:
         g = np.ascontiguousarray(conductance, np.float64)
         N.call("cfdla_batch_update_conductance", self.handle, N.ptr(g))
-        self.conductance = g
+        self._g_host, self._g_dev = g, None
+
+    def update_conductance_device(self, conductance) -> None:
+        """Re-assemble every network on the GPU from a float64 CUDA tensor of
+        edge conductances (bit for bit the host assembly); no host copy."""
+        import torch
+
+        if not (_is_torch(conductance) and conductance.is_cuda and conductance.dtype == torch.float64
+                and conductance.is_contiguous() and conductance.numel() == self.edge_i.size):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "conductance must be a contiguous float64 CUDA tensor of n_edges")
+        bind_torch_stream()
+        N.call("cfdla_batch_update_conductance_device", self.handle, conductance.data_ptr())
+        self._g_dev = conductance
 
     def export(self):
         """(row_ptr, col_indices, values, rhs) of the assembled batch."""

@@ -290,6 +290,15 @@ cfdla_status cfdla_batch_create(int64_t n_sys, const int64_t *node_offset, const
                                 const double *neumann, cfdla_batch *out);
 /* Re-assemble with new conductances (same graphs), e.g. per Picard step. */
 cfdla_status cfdla_batch_update_conductance(cfdla_batch b, const double *conductance);
+/* Re-forms every network's assembled values, right-hand side and Jacobi /
+ * equilibration factors on the device from device conductances (n_edges
+ * doubles), bit for bit as cfdla_batch_update_conductance assembles them on
+ * the host (matrix_assembly.rs:163-247: the recorded order of every sum is
+ * replayed with no FMA contraction).  Rejects non-positive or non-finite
+ * conductances before changing anything.  No counterpart in the reference,
+ * which re-assembles each network on the host: it keeps design loops (new
+ * widths -> conductances -> assembly -> solve) on the GPU. */
+cfdla_status cfdla_batch_update_conductance_device(cfdla_batch b, const double *conductance_dev);
 cfdla_status cfdla_batch_info(cfdla_batch b, int64_t *n_sys, int64_t *n_rows, int64_t *nnz);
 /* Assembled block-diagonal system (global column ids) and right-hand side. */
 cfdla_status cfdla_batch_export(cfdla_batch b, int64_t *row_ptr, int64_t *col_indices, double *values,
@@ -343,6 +352,13 @@ cfdla_status cfdla_problem_pressure_rhs(int64_t nx, int64_t ny, double dx, doubl
  * if dg_dw is non-NULL, dG/dw at fixed h, L, mu. */
 cfdla_status cfdla_rect_conductance(int64_t n, const double *width, const double *height,
                                     const double *length, double viscosity, double *g, double *dg_dw);
+/* Same on device arrays (no geometry validation: a zero width gives a
+ * non-finite G, which cfdla_batch_update_conductance_device rejects).  tanh
+ * comes from the device math library, so G can differ from the host's in the
+ * last bits. */
+cfdla_status cfdla_rect_conductance_device(int64_t n, const double *width_dev, const double *height_dev,
+                                           const double *length_dev, double viscosity, double *g_dev,
+                                           double *dg_dw_dev);
 
 /* SplitMix64 uniform[lo, hi) stream: out[i] for i < n from seed. */
 cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n, double *out);

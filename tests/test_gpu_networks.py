@@ -178,3 +178,79 @@ def test_width_gradient_matches_finite_differences():
         s = sys_of_edge[e]
         fd = (J(wp)[0][s] - J(wm)[0][s]) / (2 * h)
         assert abs(grad[e] - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.max(np.abs(grad))), (e, grad[e], fd)
+
+
+def test_device_reassembly_matches_host_bit_for_bit(small):
+    # new conductances formed on the GPU: values, right-hand side and the
+    # solve equal those of the host assembly from the same conductances
+    import torch
+
+    nets, _ = small
+    rng = np.random.default_rng(17)
+    g0 = NW.rect_conductance(nets.width, NW.CHANNEL_HEIGHT, nets.length)[0]
+    g1 = g0 * rng.uniform(0.5, 2.0, g0.size)
+    host = NW.channel_batch(nets)
+    host.update_conductance(g1)
+    dev = NW.channel_batch(nets)
+    dev.update_conductance_device(torch.from_numpy(g1).cuda())
+    rp_h, ci_h, v_h, b_h = host.export()
+    rp_d, ci_d, v_d, b_d = dev.export()
+    assert np.array_equal(rp_h, rp_d) and np.array_equal(ci_h, ci_d)
+    assert np.array_equal(v_h, v_d)
+    assert np.array_equal(b_h, b_d)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+    xh, _ = host.solve(None, cfg)
+    xd, reps = dev.solve(None, cfg)
+    assert all(r.termination == ca.Termination.Converged for r in reps)
+    assert np.array_equal(xh, xd)  # same values, same kernel, same schedule
+    assert np.array_equal(dev.conductance, g1)
+
+
+def test_device_reassembly_rejects_bad_conductance(small):
+    import torch
+
+    nets, batch = small
+    before = batch.export()
+    g = torch.from_numpy(batch.conductance.copy()).cuda()
+    g[3] = -1.0
+    with pytest.raises(ca.CfdlaError):
+        batch.update_conductance_device(g)
+    g[3] = float("nan")
+    with pytest.raises(ca.CfdlaError):
+        batch.update_conductance_device(g)
+    after = batch.export()
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_device_conductance_matches_host():
+    # Po(eps) fit and dG/dw on the device: same formulas, device tanh, so
+    # agreement to a few ulp (relative 1e-15); widths 50-500 um at h = 50 um
+    import torch
+
+    nets = NW.synth_networks(count=16, seed_base=4100, nmin=50, nmax=400)
+    gh, dh = NW.rect_conductance(nets.width, NW.CHANNEL_HEIGHT, nets.length)
+    gd, dd = NW.rect_conductance_device(torch.from_numpy(nets.width).cuda(), NW.CHANNEL_HEIGHT,
+                                        torch.from_numpy(nets.length).cuda())
+    assert np.allclose(gd.cpu().numpy(), gh, rtol=1e-15, atol=0)
+    assert np.allclose(dd.cpu().numpy(), dh, rtol=1e-14, atol=0)
+
+
+def test_design_loop_on_device(small):
+    # widths -> conductances -> assembly -> solve, all on the GPU, equals the
+    # host pipeline from the same widths
+    import torch
+
+    nets, _ = small
+    w = nets.width * 1.3
+    dev = NW.channel_batch(nets)
+    gd, _ = NW.rect_conductance_device(torch.from_numpy(w).cuda(), NW.CHANNEL_HEIGHT,
+                                       torch.from_numpy(nets.length).cuda())
+    dev.update_conductance_device(gd)
+    x = torch.zeros(dev.n_rows, dtype=torch.float64, device="cuda")
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000)
+    _, reps = dev.solve(x, cfg)
+    assert all(r.termination == ca.Termination.Converged for r in reps)
+    host = NW.channel_batch(nets)
+    host.update_conductance(NW.rect_conductance(w, NW.CHANNEL_HEIGHT, nets.length)[0])
+    xh, _ = host.solve(None, cfg)
+    assert np.max(np.abs(x.cpu().numpy() - xh)) <= 1e-9 * NW.INLET_PRESSURE
