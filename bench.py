@@ -365,10 +365,12 @@ def secondary_configs(args) -> dict:
     npress = bp.size
     ap_ = ca.CsrMatrix.from_parts(pv, pci, prp, npress, npress)
     amg_p = ca.AlgebraicMultigrid(ap_, ca.AMGConfig.smoothed_aggregation())
+    sm = ca.SuccessiveOverRelaxation(am, 1.0)  # the reference's momentum preconditioner, red-black order
     cav = {}
     for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED):
         cav[f"momentum_bicgstab_jacobi_{prec.name}"] = krylov(am, bm, jm, "bicgstab", prec)
         cav[f"momentum_gmres30_jacobi_{prec.name}"] = krylov(am, bm, jm, "gmres", prec)
+        cav[f"momentum_gmres30_sor_{prec.name}"] = krylov(am, bm, sm, "gmres", prec)
         cav[f"pressure_amg_pcg_{prec.name}"] = krylov(ap_, bp, amg_p, "cg", prec)
     out["lid_driven_cavity_512x512_re1000"] = cav
     log(f"cavity: {cav}")

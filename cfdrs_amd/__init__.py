@@ -11,14 +11,15 @@ from .linear_solver import (IterativeSolverConfig, KrylovWorkspace, Precision, S
                             bicgstab_preconditioned, cg, cg_preconditioned, converged_or_none, gmres,
                             gmres_preconditioned, gmres_preconditioned_with_workspace, interpret, restart_width)
 from .multigrid import (AlgebraicMultigrid, AMGConfig, AMGStatistics, CoarseningStrategy, CoarseSolver,
-                        InterpolationStrategy, JacobiPreconditioner, SmootherType)
+                        InterpolationStrategy, JacobiPreconditioner, SmootherType, SuccessiveOverRelaxation)
 from .sparse import CsrMatrix, SparseMatrixBuilder
 
 __all__ = [
     "AlgebraicMultigrid", "AMGConfig", "AMGStatistics", "CfdlaError", "CoarseningStrategy", "CoarseSolver",
     "CsrMatrix", "InterpolationStrategy", "IterativeSolverConfig", "JacobiPreconditioner", "KrylovWorkspace",
     "NativeLibraryMissing", "Precision", "SmootherType", "SolveError", "SolveOutcome", "SolveReport",
-    "SolverError", "SolverKind", "SparseMatrixBuilder", "Termination", "attempt", "bicgstab",
+    "SolverError", "SolverKind", "SparseMatrixBuilder", "SuccessiveOverRelaxation", "Termination", "attempt",
+    "bicgstab",
     "bicgstab_preconditioned", "cg", "cg_preconditioned", "converged_or_none", "gmres", "gmres_preconditioned",
     "gmres_preconditioned_with_workspace", "interpret", "restart_width",
 ]

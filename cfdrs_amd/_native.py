@@ -92,6 +92,7 @@ SIGNATURES = {
     "cfdla_residual_device": (C.c_int, [_VP, _VP, _VP, _VP, _F64P]),
     "cfdla_precond_create_jacobi": (C.c_int, [_VP, C.POINTER(_VP)]),
     "cfdla_precond_create_amg": (C.c_int, [_VP, C.POINTER(AmgConfig), C.POINTER(_VP)]),
+    "cfdla_precond_create_sor": (C.c_int, [_VP, C.c_double, _I32P, C.POINTER(_VP)]),
     "cfdla_precond_apply": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_precond_apply_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
     "cfdla_amg_update_values": (C.c_int, [_VP, _VP]),

@@ -315,6 +315,19 @@ cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out) {
   });
 }
 
+cfdla_status cfdla_precond_create_sor(cfdla_matrix m, double omega, int32_t *ncolours, cfdla_precond *out) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(out, "out");
+    auto p = std::make_unique<cfdla_precond_s>();
+    auto sor = std::make_unique<ColoredSorPrecond>(m->host, omega);
+    if (ncolours) *ncolours = sor->ncolours;
+    p->p = std::move(sor);
+    p->m = m;
+    *out = p.release();
+  });
+}
+
 cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *config, cfdla_precond *out) {
   return guarded([&] {
     require_ptr(m, "matrix");

@@ -299,6 +299,13 @@ void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T
 template <typename T>
 void axpy_scal(int n, const T *x, T *y, Scal *s, int which);  // y += s->{alpha} x
 
+// One colour of a multicolour SOR sweep: for the rows perm[0 .. count),
+// z_i = piv_i * (r_i - sum_{j != i} a_ij z_j) (rows of later colours still
+// hold z = 0).  Skipped when check_done and s->done != 0.
+template <typename T>
+void sor_colour(int count, const int *perm, const int *rp, const int *ci, const T *v, const T *piv, const T *r, T *z,
+                Scal *s, bool check_done);
+
 // Scalars helpers
 void scal_reset(Scal *s, double thr2);
 void scal_set_done(Scal *s, int value);

@@ -577,6 +577,23 @@ __global__ __launch_bounds__(kBlock) void k_axpy_alpha(int n, const T *__restric
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += a * x[i];
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sor_colour(int count, const int *__restrict__ perm,
+                                                       const int *__restrict__ rp, const int *__restrict__ ci,
+                                                       const T *__restrict__ v, const T *__restrict__ piv,
+                                                       const T *__restrict__ r, T *z, Scal *s, int check_done) {
+  if (check_done && s->done) return;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < count; t += gridDim.x * kBlock) {
+    const int i = perm[t];
+    T acc = r[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      if (j != i) acc -= v[k] * z[j];
+    }
+    z[i] = piv[i] * acc;
+  }
+}
+
 __global__ void k_scal_reset(Scal *s, double thr2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     for (int i = 0; i < 8; ++i) s->red[i] = 0.0;
@@ -783,6 +800,19 @@ void axpy_scal(int n, const T *x, T *y, Scal *s, int /*which*/) {
   hipLaunchKernelGGL((k_axpy_alpha<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, x, y, s);
   CFDLA_HIP(hipGetLastError());
 }
+
+template <typename T>
+void sor_colour(int count, const int *perm, const int *rp, const int *ci, const T *v, const T *piv, const T *r, T *z,
+                Scal *s, bool check_done) {
+  if (count == 0) return;
+  hipLaunchKernelGGL((k_sor_colour<T>), dim3(blocks_for(count, kBlock * 4)), dim3(kBlock), 0, stream(), count, perm,
+                     rp, ci, v, piv, r, z, s, check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+template void sor_colour<double>(int, const int *, const int *, const int *, const double *, const double *,
+                                 const double *, double *, Scal *, bool);
+template void sor_colour<float>(int, const int *, const int *, const int *, const float *, const float *,
+                                const float *, float *, Scal *, bool);
 
 void scal_reset(Scal *s, double thr2) {
   hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, stream(), s, thr2);

@@ -376,6 +376,108 @@ JacobiPrecond::JacobiPrecond(const HostCsr &a) {
 void JacobiPrecond::apply(const double *r, double *z, Scal *s, bool chk) { jacobi_first<double>(n, d64.p, 1.0, r, z, s, chk); }
 void JacobiPrecond::apply(const float *r, float *z, Scal *s, bool chk) { jacobi_first<float>(n, d32.p, 1.0, r, z, s, chk); }
 
+ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
+  CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "SOR requires a square matrix");
+  CFDLA_REQUIRE(std::isfinite(omega) && omega > 0.0 && omega < 2.0, CFDLA_ERR_CONFIG,
+                "SOR relaxation factor must lie in (0, 2)");
+  n = (int)a.n;
+  // symmetrized adjacency (couplings in either direction)
+  std::vector<int64_t> deg((size_t)n + 1, 0);
+  for (int64_t i = 0; i < a.n; ++i)
+    for (int64_t k = a.rp[i]; k < a.rp[i + 1]; ++k)
+      if (a.ci[k] != i) {
+        deg[(size_t)i + 1]++;
+        deg[(size_t)a.ci[k] + 1]++;
+      }
+  for (int i = 0; i < n; ++i) deg[(size_t)i + 1] += deg[(size_t)i];
+  std::vector<int32_t> adj((size_t)deg[(size_t)n]);
+  std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+  for (int64_t i = 0; i < a.n; ++i)
+    for (int64_t k = a.rp[i]; k < a.rp[i + 1]; ++k)
+      if (a.ci[k] != i) {
+        adj[(size_t)fill[(size_t)i]++] = a.ci[k];
+        adj[(size_t)fill[(size_t)a.ci[k]]++] = (int32_t)i;
+      }
+  // Greedy colouring in breadth-first order (from the lowest unvisited row of
+  // each component): on a bipartite pattern -- any 5-point stencil, identity
+  // boundary rows included -- every row's coloured neighbours sit one level
+  // up, so it yields the two colours of red-black ordering.
+  std::vector<int32_t> bfs;  // visit order; doubles as the queue
+  bfs.reserve((size_t)n);
+  {
+    std::vector<char> queued((size_t)n, 0);
+    for (int root = 0; root < n; ++root) {
+      if (queued[(size_t)root]) continue;
+      queued[(size_t)root] = 1;
+      bfs.push_back(root);
+      for (size_t head = bfs.size() - 1; head < bfs.size(); ++head) {
+        const int32_t i = bfs[head];
+        for (int64_t k = deg[(size_t)i]; k < deg[(size_t)i + 1]; ++k) {
+          const int32_t j = adj[(size_t)k];
+          if (queued[(size_t)j]) continue;
+          queued[(size_t)j] = 1;
+          bfs.push_back(j);
+        }
+      }
+    }
+  }
+  std::vector<int32_t> colour((size_t)n, -1);
+  std::vector<int32_t> seen;  // seen[c] == i: colour c is taken by a neighbour of i
+  for (const int i : bfs) {
+    for (int64_t k = deg[(size_t)i]; k < deg[(size_t)i + 1]; ++k) {
+      const int32_t c = colour[(size_t)adj[(size_t)k]];
+      if (c >= 0) {
+        if ((size_t)c >= seen.size()) seen.resize((size_t)c + 1, -1);
+        seen[(size_t)c] = i;
+      }
+    }
+    int32_t c = 0;
+    while ((size_t)c < seen.size() && seen[(size_t)c] == i) ++c;
+    colour[(size_t)i] = c;
+    ncolours = std::max(ncolours, c + 1);
+  }
+  colour_off.assign((size_t)ncolours + 1, 0);
+  for (int i = 0; i < n; ++i) colour_off[(size_t)colour[(size_t)i] + 1]++;
+  for (int c = 0; c < ncolours; ++c) colour_off[(size_t)c + 1] += colour_off[(size_t)c];
+  std::vector<int32_t> order((size_t)n);
+  std::vector<int32_t> pos(colour_off.begin(), colour_off.end() - 1);
+  for (int i = 0; i < n; ++i) order[(size_t)pos[(size_t)colour[(size_t)i]]++] = i;
+  // pivots omega / a_ii; a row without a (nonzero) diagonal takes a unit
+  // pivot, as SuccessiveOverRelaxation::from_csr_with_identity_rows does
+  std::vector<double> piv((size_t)n, omega);
+  for (int64_t i = 0; i < a.n; ++i)
+    for (int64_t k = a.rp[i]; k < a.rp[i + 1]; ++k)
+      if (a.ci[k] == i && a.v[k] != 0.0) piv[(size_t)i] = omega / a.v[k];
+  for (double p : piv) CFDLA_REQUIRE(std::isfinite(p), CFDLA_ERR_NUMERICAL, "SOR: non-finite pivot");
+  std::vector<int32_t> rp32((size_t)n + 1);
+  for (int i = 0; i <= n; ++i) rp32[(size_t)i] = (int32_t)a.rp[(size_t)i];
+  std::vector<float> v32h(a.v.size()), piv32h(piv.size());
+  for (size_t k = 0; k < a.v.size(); ++k) v32h[k] = (float)a.v[k];
+  for (size_t k = 0; k < piv.size(); ++k) piv32h[k] = (float)piv[k];
+  perm.upload(order.data(), order.size());
+  rp.upload(rp32.data(), rp32.size());
+  ci.upload(a.ci.data(), a.ci.size());
+  v64.upload(a.v.data(), a.v.size());
+  v32.upload(v32h.data(), v32h.size());
+  piv64.upload(piv.data(), piv.size());
+  piv32.upload(piv32h.data(), piv32h.size());
+  sync_stream();
+}
+
+void ColoredSorPrecond::apply(const double *r, double *z, Scal *s, bool chk) {
+  fill_vec<double>(n, 0.0, z);
+  for (int c = 0; c < ncolours; ++c)
+    sor_colour<double>(colour_off[(size_t)c + 1] - colour_off[(size_t)c], perm.p + colour_off[(size_t)c], rp.p, ci.p,
+                       v64.p, piv64.p, r, z, s, chk);
+}
+
+void ColoredSorPrecond::apply(const float *r, float *z, Scal *s, bool chk) {
+  fill_vec<float>(n, 0.0f, z);
+  for (int c = 0; c < ncolours; ++c)
+    sor_colour<float>(colour_off[(size_t)c + 1] - colour_off[(size_t)c], perm.p + colour_off[(size_t)c], rp.p, ci.p,
+                      v32.p, piv32.p, r, z, s, chk);
+}
+
 struct IdentityPrecond : Precond {
   int n;
   explicit IdentityPrecond(int n_) : n(n_) {}

@@ -26,7 +26,7 @@ Context &ctx();
 
 struct Precond {
   virtual ~Precond() = default;
-  virtual int kind() const = 0;  // 0 identity, 1 Jacobi, 2 AMG
+  virtual int kind() const = 0;  // 0 identity, 1 Jacobi, 2 AMG, 3 multicolour SOR
   virtual void apply(const double *r, double *z, Scal *s, bool check_done) = 0;
   virtual void apply(const float *r, float *z, Scal *s, bool check_done) = 0;
   // z = M r, then r . z reduced and dot_op run on the device scalars.
@@ -47,6 +47,25 @@ struct JacobiPrecond : Precond {
   void apply(const float *r, float *z, Scal *s, bool check_done) override;
   const double *jacobi_dinv64() const override { return d64.p; }
   const float *jacobi_dinv32() const override { return d32.p; }
+};
+
+// Multicolour SOR as a preconditioner: z = (D / omega + L_c)^-1 r, one
+// forward sweep from z = 0 in colour order, L_c the couplings to rows of
+// earlier colours.  Rows are coloured greedily in breadth-first order on the
+// symmetrized pattern, so no two coupled rows share a colour and each colour
+// is one parallel kernel (a 5-point stencil gets the red-black checkerboard).
+// Replaces the lexicographic SuccessiveOverRelaxation of the momentum solve
+// (momentum/solve.rs:137-147): a sequential sweep has no parallel form.
+struct ColoredSorPrecond : Precond {
+  int n = 0, ncolours = 0;
+  std::vector<int32_t> colour_off;  // ncolours + 1 offsets into perm
+  DevBuf<int32_t> perm, rp, ci;      // rows grouped by colour; the matrix pattern
+  DevBuf<double> v64, piv64;         // values; omega / pivot (unit pivot where the diagonal is absent or zero)
+  DevBuf<float> v32, piv32;
+  ColoredSorPrecond(const HostCsr &a, double omega);
+  int kind() const override { return 3; }
+  void apply(const double *r, double *z, Scal *s, bool check_done) override;
+  void apply(const float *r, float *z, Scal *s, bool check_done) override;
 };
 
 struct DevLevel {

@@ -168,6 +168,25 @@ class JacobiPreconditioner(_Preconditioner):
         self._h, self.n, self.matrix = h, matrix.nrows, matrix
 
 
+class SuccessiveOverRelaxation(_Preconditioner):
+    """SOR preconditioner z = (D / omega + L)^-1 r of the momentum solve
+    (momentum/solve.rs:137-147, ext athena SuccessiveOverRelaxation), swept in
+    multicolour order on the device: rows coloured greedily in breadth-first
+    order (red-black for 5-point stencils), one parallel forward sweep per
+    colour; rows without a diagonal take a unit pivot, as
+    from_csr_with_identity_rows does."""
+
+    def __init__(self, matrix: CsrMatrix, omega: float = 1.0):
+        h = C.c_void_p()
+        nc = C.c_int32()
+        N.call("cfdla_precond_create_sor", matrix.handle, float(omega), C.byref(nc), C.byref(h))
+        self._h, self.n, self.matrix, self.omega, self.colours = h, matrix.nrows, matrix, float(omega), nc.value
+
+    @classmethod
+    def from_csr_with_identity_rows(cls, matrix: CsrMatrix, omega: float) -> "SuccessiveOverRelaxation":
+        return cls(matrix, omega)
+
+
 class AlgebraicMultigrid(_Preconditioner):
     """AMG preconditioner (amg.rs:100-558) with a device V-cycle."""
 

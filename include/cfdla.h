@@ -209,6 +209,16 @@ cfdla_status cfdla_residual_device(cfdla_matrix m, const double *b, const double
 cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out);
 /* AlgebraicMultigrid::new (amg.rs:133-152).  Host setup (coarsening,
  * interpolation, Galerkin products); device V-cycle. */
+/* Multicolour SOR preconditioner, z = (D / omega + L_c)^-1 r: one forward
+ * sweep from z = 0 over the rows in colour order (greedy colouring of the
+ * symmetrized pattern in breadth-first order; a 5-point stencil gets
+ * red-black),
+ * L_c the couplings to earlier colours.  Rows without a nonzero diagonal get
+ * a unit pivot.  Replaces SuccessiveOverRelaxation::from_csr_with_identity_rows
+ * (momentum/solve.rs:137-147), whose lexicographic sweep is sequential; the
+ * sweep order, hence the iterates, differ.  *ncolours (nullable) receives the
+ * colour count. */
+cfdla_status cfdla_precond_create_sor(cfdla_matrix m, double omega, int32_t *ncolours, cfdla_precond *out);
 cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *config,
                                       cfdla_precond *out);
 /* z = M^-1 r (z fully overwritten, amg.rs:553-554) on host vectors.

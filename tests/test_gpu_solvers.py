@@ -455,3 +455,95 @@ def test_refined_cg_reliable_budget_and_initial_residual():
     rep = ca.cg(d, np.ones(3), x, ca.IterativeSolverConfig.new(1e-12).with_max_iterations(10)
                 .with_precision(ca.Precision.FP32_REFINED))
     assert rep.termination in (ca.Termination.NonPositiveCurvature, ca.Termination.Breakdown)
+
+
+def test_sor_diagonal_closed_form():
+    # one colour: z = omega r / d
+    d = np.array([2.0, 4.0, 8.0, 5.0])
+    a = ca.CsrMatrix.from_parts(d, [0, 1, 2, 3], [0, 1, 2, 3, 4], 4, 4)
+    sor = ca.SuccessiveOverRelaxation(a, 1.5)
+    assert sor.colours == 1
+    r = np.array([1.0, -2.0, 3.0, 0.5])
+    z = np.empty(4)
+    sor.apply_to(r, z)
+    assert np.array_equal(z, (1.5 / d) * r)
+
+
+def test_sor_red_black_tridiagonal_by_hand():
+    # 1D operator: rows coloured even / odd; the even sweep sees z = 0, the
+    # odd sweep the even z -- written out with the two recurrences
+    n, w = 9, 1.2
+    main, lo, up = np.full(n, 4.0), -np.arange(1, n) * 0.5, -np.ones(n - 1)
+    dense = np.diag(main) + np.diag(lo, -1) + np.diag(up, 1)
+    rp, ci, v = dense_to_csr(dense)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    sor = ca.SuccessiveOverRelaxation(a, w)
+    assert sor.colours == 2
+    r = np.linspace(-1.0, 2.0, n)
+    z = np.empty(n)
+    sor.apply_to(r, z)
+    expect = np.zeros(n)
+    for i in range(0, n, 2):
+        expect[i] = (w / main[i]) * r[i]
+    for i in range(1, n, 2):
+        acc = r[i] - lo[i - 1] * expect[i - 1]
+        if i + 1 < n:
+            acc -= up[i] * expect[i + 1]
+        expect[i] = (w / main[i]) * acc
+    assert np.allclose(z, expect, rtol=1e-15, atol=0)
+
+
+def test_sor_dense_is_lexicographic_sweep():
+    # every pair of rows is coupled, so each row takes its own colour in
+    # visit order 0, 1, ..., n-1: the sweep is the sequential forward SOR
+    # sweep from zero, z = (D / omega + L)^-1 r with L the strict lower part
+    n, w = 7, 1.3
+    rng = np.random.default_rng(11)
+    dense = rng.uniform(-1.0, 1.0, (n, n)) + np.diag(np.full(n, 6.0))
+    rp, ci, v = dense_to_csr(dense)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    sor = ca.SuccessiveOverRelaxation(a, w)
+    assert sor.colours == n
+    r = rng.uniform(-1.0, 1.0, n)
+    z = np.empty(n)
+    sor.apply_to(r, z)
+    expect = np.linalg.solve(np.diag(np.diag(dense)) / w + np.tril(dense, -1), r)
+    assert np.allclose(z, expect, rtol=1e-13, atol=1e-15)
+
+
+def test_sor_unit_pivot_without_diagonal():
+    # row 1 carries no self-coupling: unit pivot, as from_csr_with_identity_rows
+    a = ca.CsrMatrix.from_parts([2.0, 1.0, 3.0], [0, 0, 2], [0, 1, 2, 3], 3, 3)
+    sor = ca.SuccessiveOverRelaxation.from_csr_with_identity_rows(a, 1.0)
+    r = np.array([4.0, 1.0, 6.0])
+    z = np.empty(3)
+    sor.apply_to(r, z)
+    # colours: row 0 -> 0, row 1 (coupled to 0) -> 1, row 2 -> 0
+    assert np.allclose(z, [2.0, 1.0 - 1.0 * 2.0, 2.0], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("kind", ["gmres", "bicgstab"])
+@pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
+def test_sor_preconditioned_momentum(kind, precision):
+    # the reference's momentum preconditioner (SOR, omega = 1) in red-black
+    # order: converges to the dense solution in fewer steps than Jacobi
+    orc = _momentum()
+    rp, ci, v = orc.arrays()
+    n = rp.size - 1
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    b = P.uniform(31, n)
+    sor = ca.SuccessiveOverRelaxation(a, 1.0)
+    assert sor.colours == 2
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(5000).with_precision(precision)
+    its = {}
+    for name, m in (("sor", sor), ("jacobi", ca.JacobiPreconditioner(a))):
+        x = np.zeros(n)
+        rep = (ca.gmres_preconditioned(a, b, m, x, cfg, 30) if kind == "gmres"
+               else ca.bicgstab_preconditioned(a, b, m, x, cfg))
+        assert rep.termination == ca.Termination.Converged, (name, rep)
+        assert rel_res(orc, x, b) <= 1e-10
+        its[name] = rep.iterations
+        if name == "sor":
+            xo = np.linalg.solve(orc.dense(), b)
+            assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
+    assert its["sor"] < its["jacobi"], its
