@@ -84,6 +84,8 @@ SIGNATURES = {
     "cfdla_matrix_create": (C.c_int, [C.c_int64, C.c_int64, _VP, _VP, _VP, C.POINTER(_VP)]),
     "cfdla_matrix_create_device": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _VP, _VP, _VP, C.POINTER(_VP)]),
     "cfdla_matrix_update_values": (C.c_int, [_VP, _VP]),
+    "cfdla_matrix_update_values_device": (C.c_int, [_VP, _VP]),
+    "cfdla_matrix_get_values": (C.c_int, [_VP, _VP]),
     "cfdla_matrix_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
     "cfdla_matrix_storage": (C.c_int, [_VP, _I32P, _I64P, _I64P]),
     "cfdla_matrix_destroy": (C.c_int, [_VP]),
@@ -96,6 +98,7 @@ SIGNATURES = {
     "cfdla_precond_apply": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_precond_apply_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
     "cfdla_amg_update_values": (C.c_int, [_VP, _VP]),
+    "cfdla_amg_update_values_device": (C.c_int, [_VP, _VP]),
     "cfdla_amg_stats": (C.c_int, [_VP, C.POINTER(AmgStatistics)]),
     "cfdla_amg_export_level": (C.c_int, [_VP, C.c_int32, C.c_int32, _I64P, _I64P, _I64P, _VP, _VP, _VP]),
     "cfdla_amg_export_map": (C.c_int, [_VP, C.c_int32, _VP]),
@@ -111,6 +114,12 @@ SIGNATURES = {
                                            C.c_double, _VP, _VP, _I64P, _VP, _VP, _VP]),
     "cfdla_problem_pressure_rhs": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                              _VP, _VP, C.c_int32, _VP]),
+    "cfdla_problem_simple_pressure": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _VP, _VP, _VP,
+                                                _VP, _VP, _VP, _I64P, _VP, _VP, _VP, _VP, _F64P]),
+    "cfdla_simple_pressure_refresh_device": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _VP,
+                                                       _VP, _VP, _VP, _VP, _VP, _F64P]),
+    "cfdla_simple_corrections_device": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, _VP,
+                                                  _VP, _VP, _VP, _VP, _VP]),
     "cfdla_problem_pressure_correction": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _I64P,
                                                     _VP, _VP, _VP]),
     "cfdla_problem_varcoef": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_double, _VP, C.c_int32, _VP, _I64P,

@@ -20,6 +20,9 @@ struct cfdla_matrix_s {
   int symmetric = -1;         // -1 unknown
   std::unique_ptr<cfdla_matrix_s> transpose;
   std::unique_ptr<Precond> transpose_prec;
+  bool host_stale = false;    // device values newer than host.v (device updates)
+  DevBuf<int32_t> dperm;      // perm on the device, for device updates
+  int64_t simple_nx = 0, simple_ny = 0;  // SIMPLE pressure pattern verified for this grid
 };
 
 struct cfdla_precond_s {
@@ -106,6 +109,31 @@ DevCsr &devmat(cfdla_matrix_s *m) {
   return m->dev;
 }
 
+// host.v after device updates (copied back on demand).
+const HostCsr &host_values(cfdla_matrix_s *m) {
+  if (m->host_stale) {
+    m->dev.v64.download(m->host.v.data(), m->host.v.size());
+    sync_stream();
+    m->host_stale = false;
+  }
+  return m->host;
+}
+
+// New values landed in dev.v64: derived device copies follow, host copies
+// and host-derived caches are dropped.
+void device_values_changed(cfdla_matrix_s *m) {
+  m->host_stale = true;
+  m->symmetric = -1;
+  m->transpose.reset();
+  m->transpose_prec.reset();
+  DevCsr &d = m->dev;
+  if (!d.maps_ready) d.build_maps(m->host);
+  if (d.refresh_from_v64()) return;
+  host_values(m);
+  d.general_dia = true;
+  d.upload(m->host, true, false);
+}
+
 void permuted_values(const cfdla_matrix_s *m, const double *values, std::vector<double> &out) {
   const size_t nnz = m->host.v.size();
   out.resize(nnz);
@@ -128,7 +156,7 @@ cfdla_report to_report(const SolveOut &o) {
 
 bool is_symmetric(cfdla_matrix_s *m) {
   if (m->symmetric >= 0) return m->symmetric == 1;
-  const HostCsr &a = m->host;
+  const HostCsr &a = host_values(m);
   bool sym = a.n == a.m;
   for (int64_t i = 0; i < a.n && sym; ++i)
     for (int64_t k = a.rp[i]; k < a.rp[i + 1] && sym; ++k) {
@@ -218,10 +246,80 @@ cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values) {
     std::vector<double> v;
     permuted_values(m, values, v);
     m->host.v = v;
+    m->host_stale = false;
     m->symmetric = -1;
     m->transpose.reset();
     m->transpose_prec.reset();
     if (m->on_device) m->dev.upload(m->host, true, false);
+  });
+}
+
+cfdla_status cfdla_matrix_update_values_device(cfdla_matrix m, const double *values_dev) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(values_dev, "values");
+    DevCsr &d = devmat(m);
+    if (d.nnz == 0) return;
+    if (!m->perm.empty() && !m->dperm.p) {
+      std::vector<int32_t> p32(m->perm.begin(), m->perm.end());
+      m->dperm.upload(p32.data(), p32.size());
+    }
+    gather_values(d.nnz, values_dev, m->perm.empty() ? nullptr : m->dperm.p, d.v64.p);
+    device_values_changed(m);
+  });
+}
+
+cfdla_status cfdla_matrix_get_values(cfdla_matrix m, double *values) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(values, "values");
+    const HostCsr &h = host_values(m);
+    if (m->perm.empty()) std::copy(h.v.begin(), h.v.end(), values);
+    else
+      for (size_t k = 0; k < h.v.size(); ++k) values[m->perm[k]] = h.v[k];
+  });
+}
+
+cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, double dx, double dy,
+                                                  const double *u, const double *v, const double *p,
+                                                  const double *d_u, const double *d_v, const double *rho,
+                                                  double *rhs_dev, double *max_residual) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    CFDLA_REQUIRE(u && v && p && d_u && d_v && rho && rhs_dev, CFDLA_ERR_INVALID_ARG,
+                  "fields and rhs must not be NULL");
+    CFDLA_REQUIRE(nx >= 3 && ny >= 3 && m->host.n == nx * ny && m->host.m == nx * ny, CFDLA_ERR_DIMENSION,
+                  "matrix is not the SIMPLE pressure system of this grid");
+    if (m->simple_nx != nx || m->simple_ny != ny) {
+      // every interior row must be the 5-point row the kernel rewrites
+      const HostCsr &h = m->host;
+      for (int64_t j = 1; j < ny - 1; ++j)
+        for (int64_t i = 1; i < nx - 1; ++i) {
+          const int64_t idx = j * nx + i, k0 = h.rp[idx];
+          const int64_t want[5] = {idx - nx, idx - 1, idx, idx + 1, idx + nx};
+          bool ok = h.rp[idx + 1] - k0 == 5;
+          for (int q = 0; q < 5 && ok; ++q) ok = h.ci[k0 + q] == want[q];
+          CFDLA_REQUIRE(ok, CFDLA_ERR_INVALID_ARG,
+                        "matrix pattern is not cfdla_problem_simple_pressure's for this grid");
+        }
+      m->simple_nx = nx;
+      m->simple_ny = ny;
+    }
+    DevCsr &d = devmat(m);
+    simple_refresh(nx, ny, dx, dy, u, v, p, d_u, d_v, rho, d.rp.p, d.v64.p, rhs_dev, max_residual != nullptr);
+    device_values_changed(m);
+    if (max_residual) *max_residual = simple_max_read();
+  });
+}
+
+cfdla_status cfdla_simple_corrections_device(int64_t nx, int64_t ny, double dx, double dy, double relax,
+                                             const double *p_prime, const double *d_u, const double *d_v, double *p,
+                                             double *u, double *v) {
+  return guarded([&] {
+    CFDLA_REQUIRE(p_prime && d_u && d_v && p && u && v, CFDLA_ERR_INVALID_ARG, "arrays must not be NULL");
+    CFDLA_REQUIRE(nx >= 3 && ny >= 3, CFDLA_ERR_DIMENSION, "SIMPLE grid needs at least 3x3 nodes");
+    ensure_init();
+    simple_corrections(nx, ny, dx, dy, relax, p_prime, d_u, d_v, p, u, v);
   });
 }
 
@@ -309,7 +407,7 @@ cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out) {
     require_ptr(out, "out");
     CFDLA_REQUIRE(m->host.n == m->host.m, CFDLA_ERR_DIMENSION, "Jacobi requires a square matrix");
     auto p = std::make_unique<cfdla_precond_s>();
-    p->p = std::make_unique<JacobiPrecond>(m->host);
+    p->p = std::make_unique<JacobiPrecond>(host_values(m));
     p->m = m;
     *out = p.release();
   });
@@ -320,7 +418,7 @@ cfdla_status cfdla_precond_create_sor(cfdla_matrix m, double omega, int32_t *nco
     require_ptr(m, "matrix");
     require_ptr(out, "out");
     auto p = std::make_unique<cfdla_precond_s>();
-    auto sor = std::make_unique<ColoredSorPrecond>(m->host, omega);
+    auto sor = std::make_unique<ColoredSorPrecond>(host_values(m), omega);
     if (ncolours) *ncolours = sor->ncolours;
     p->p = std::move(sor);
     p->m = m;
@@ -334,7 +432,7 @@ cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *co
     require_ptr(config, "config");
     require_ptr(out, "out");
     auto p = std::make_unique<cfdla_precond_s>();
-    p->p = std::make_unique<AmgPrecond>(m->host, *config);
+    p->p = std::make_unique<AmgPrecond>(host_values(m), *config);
     p->m = m;
     *out = p.release();
   });
@@ -383,6 +481,17 @@ cfdla_status cfdla_amg_update_values(cfdla_precond p, const double *values) {
   });
 }
 
+cfdla_status cfdla_amg_update_values_device(cfdla_precond p, cfdla_matrix m) {
+  return guarded([&] {
+    require_ptr(p, "preconditioner");
+    require_ptr(m, "matrix");
+    auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
+    CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
+    CFDLA_REQUIRE(m == p->m, CFDLA_ERR_INVALID_ARG, "matrix is not the one the AMG hierarchy was built from");
+    amg->update_values_device(devmat(m).v64.p);
+  });
+}
+
 cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out) {
   return guarded([&] {
     require_ptr(p, "preconditioner");
@@ -390,6 +499,7 @@ cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out) {
     auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
     CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
     std::memset(out, 0, sizeof(*out));
+    amg->sync_host();
     const auto &lv = amg->host.levels;
     out->num_levels = (int32_t)lv.size();
     auto count = [](const HostCsr &a) {
@@ -418,6 +528,7 @@ cfdla_status cfdla_amg_export_level(cfdla_precond p, int32_t level, int32_t whic
     CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
     CFDLA_REQUIRE(level >= 0 && level < (int32_t)amg->host.levels.size(), CFDLA_ERR_INVALID_ARG,
                   "level out of range");
+    if (values) amg->sync_host();
     const HostLevel &L = amg->host.levels[(size_t)level];
     CFDLA_REQUIRE(which >= 0 && which <= 2, CFDLA_ERR_INVALID_ARG, "which must be 0 (A), 1 (R) or 2 (P)");
     const HostCsr &a = which == 0 ? L.A : which == 1 ? L.R : L.P;
@@ -490,6 +601,7 @@ cfdla_status cfdla_solve_transpose_device(cfdla_matrix m, cfdla_precond p, const
       if (!m->transpose) {
         auto t = std::make_unique<cfdla_matrix_s>();
         HostCsr tr;
+        host_values(m);
         tr.n = m->host.m;
         tr.m = m->host.n;
         tr.rp.assign((size_t)tr.n + 1, 0);

@@ -147,10 +147,25 @@ struct DevCsr {
   DevBuf<double> sell_v64;
   DevBuf<float> sell_v32;
   bool dia = false;
+  bool general_dia = false;  // a device update broke symmetry once: keep every diagonal from then on
   DiaDesc dia_desc;
   DevBuf<double> dia_v64;
   DevBuf<float> dia_v32;
   void upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell = true);
+  // Device value refresh.  v64 (CSR order) is the truth; build_maps (once,
+  // from the host pattern) records for every CSR slot its position in the
+  // diagonal or sliced-ELL value arrays (fmt_pos, -1: none), for symmetric
+  // diagonal storage the slot of the transposed entry (fmt_mirror, -2 when
+  // the pattern lacks it, -1 on the main diagonal), and each row's diagonal
+  // slot (diag_slot, the last one, -1: none).  refresh_from_v64 rewrites the
+  // FP32 copy, the format arrays and the smoother's inverse diagonal from
+  // v64; it returns false, having changed nothing, when v64 is no longer
+  // symmetric under a symmetric diagonal layout (the caller then rebuilds
+  // the layout through the host with general_dia set).
+  bool maps_ready = false;
+  DevBuf<int32_t> fmt_pos, fmt_mirror, diag_slot;
+  void build_maps(const HostCsr &h);
+  bool refresh_from_v64();
   template <typename T> const T *vals() const;
   template <typename T> const T *sdinv() const;
   template <typename T> const T *sell_vals() const;
@@ -236,6 +251,30 @@ void launch_dia(const DevCsr &A, const T *x, const T *b, const T *dinv, double o
 // runs the finalize op (kernels.hip).
 void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done);
 int xcd_grid(int units, int units_per_block, int cap = kGridCap);
+
+// Device value refresh (refresh.hip).
+// v64[k] = src[perm ? perm[k] : k] for k < nnz.
+void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *v64);
+// Symmetric diagonal layout: true when v64 is still symmetric (synchronizes).
+bool symmetric_on_device(const DevCsr &d);
+// FP32 copy, diagonal / sliced-ELL arrays and smoother diagonal from v64.
+void scatter_formats(DevCsr &d);
+// SIMPLE pressure-correction interior rows into v64 (5 sorted slots from
+// rp[idx]) and rhs; with want_max, max |imbalance| is reduced on the device
+// for simple_max_read (which synchronizes).  The row kernel's block maxima
+// sit in the reduction partials until the next reducing launch.
+void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, const double *u, const double *v, const double *p,
+                    const double *du, const double *dv, const double *rho, const int32_t *rp, double *v64,
+                    double *rhs, bool want_max);
+double simple_max_read();
+void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
+                        const double *du, const double *dv, double *p, double *u, double *v);
+// cv = A B on C's fixed sorted pattern, the host product's summation order.
+void spgemm_numeric(int nrows, const int32_t *arp, const int32_t *aci, const double *av, const int32_t *brp,
+                    const int32_t *bci, const double *bv, const int32_t *crp, const int32_t *cci, double *cv);
+// n x n inverse of a CSR matrix by Gauss-Jordan with partial pivoting, the
+// host's operation order (throws CFDLA_ERR_NUMERICAL when singular).
+void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const double *v, double *inv);
 
 // Sparse product (diagonal, sliced-ELL or row-block CSR) with epilogue:
 //   EPI_Y       y = A x

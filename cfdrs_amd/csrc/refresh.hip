@@ -1,0 +1,400 @@
+// Device value refresh of libcfdla: new matrix values on a fixed pattern
+// without a host round trip.
+//
+// * Format scatter: the CSR FP64 values are the truth; the FP32 copy, the
+//   diagonal / sliced-ELL arrays and the smoother's inverse diagonal are
+//   rewritten from them through per-slot maps built once on the host.
+// * The SIMPLE pressure-correction rows and right-hand side
+//   (crates/cfd-2d/src/solvers/simple/pressure.rs:244-450) and the velocity /
+//   pressure corrections (:487-520).
+// * The AMG Galerkin refresh (amg.rs:179-214): numeric sparse products on
+//   fixed patterns and the dense coarsest inverse.
+//
+// Everything that replays a host computation does the host's floating-point
+// operations in the host's order with contraction off, so results are equal
+// bit for bit: a product C = A B accumulates each row's products in the
+// traversal order of the host's sparse accumulator, starting from -0.0 (the
+// additive identity of IEEE round-to-nearest: -0.0 + x == x for every x,
+// signed zeros included), which equals the host's "first product assigned,
+// later ones added".
+#include "internal.hpp"
+#include "simple_stencil.hpp"
+#include "solver.hpp"
+
+namespace cfdla {
+namespace {
+
+DevBuf<int32_t> g_flag;  // one device int: symmetry / singularity flags
+DevBuf<double> g_max;    // max |mass imbalance| of the latest SIMPLE refresh
+
+__global__ void k_zero_flag(int *f) { *f = 0; }
+
+int *flag_reset() {
+  g_flag.ensure(1);
+  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, stream(), g_flag.p);
+  return g_flag.p;
+}
+
+int flag_read() {
+  int32_t h = 0;
+  CFDLA_HIP(hipMemcpyAsync(&h, g_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+  sync_stream();
+  return h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(int64_t nnz, const double *__restrict__ src,
+                                                   const int32_t *__restrict__ perm, double *__restrict__ v64) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock)
+    v64[k] = src[perm ? perm[k] : k];
+}
+
+// Symmetric diagonal storage: every off-diagonal entry must equal its
+// transposed entry, or be zero where the pattern lacks the transpose.
+__global__ __launch_bounds__(kBlock) void k_mirror_check(int64_t nnz, const double *__restrict__ v64,
+                                                         const int32_t *__restrict__ mirror, int *bad) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock) {
+    const int32_t m = mirror[k];
+    if (m == -1) continue;
+    const double other = m >= 0 ? v64[m] : 0.0;
+    if (!(v64[k] == other)) *bad = 1;  // benign race: every writer stores 1
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(int64_t nnz, const double *__restrict__ v64, float *v32,
+                                                    const int32_t *__restrict__ pos, double *fv64, float *fv32) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock) {
+    const double v = v64[k];
+    if (v32) v32[k] = (float)v;
+    if (fv64) {
+      const int32_t q = pos[k];
+      if (q >= 0) {
+        fv64[q] = v;
+        if (fv32) fv32[q] = (float)v;
+      }
+    }
+  }
+}
+
+// Smoother inverse diagonal: 1 / a_ii, 0 where |a_ii| <= 1e-15 or absent.
+__global__ __launch_bounds__(kBlock) void k_sdinv(int n, const double *__restrict__ v64,
+                                                  const int32_t *__restrict__ dslot, double *d64, float *d32) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int32_t k = dslot[i];
+    const double a = k >= 0 ? v64[k] : 0.0;
+    const double d = fabs(a) > 1e-15 ? 1.0 / a : 0.0;
+    d64[i] = d;
+    if (d32) d32[i] = (float)d;
+  }
+}
+
+int grid_for(int64_t units) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kGridCap, (units + kBlock - 1) / kBlock));
+}
+
+// ---- SIMPLE pressure correction ------------------------------------------
+
+// One lane per node: interior rows (5 slots from rp[idx], columns sorted)
+// and the right-hand side; boundary nodes get rhs 0.  Block maxima of
+// |imbalance| (a NaN never wins the strict comparison) go to partials.
+__global__ __launch_bounds__(kBlock) void k_simple_refresh(int64_t nx, int64_t ny, double dx, double dy,
+                                                           const double *u, const double *v, const double *p,
+                                                           const double *du, const double *dv, const double *rho,
+                                                           const int32_t *__restrict__ rp, double *v64, double *rhs,
+                                                           double *partials) {
+  const int64_t n = nx * ny;
+  double mx = 0.0;
+  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    const int64_t j = idx / nx, i = idx - j * nx;
+    if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1) {
+      rhs[idx] = 0.0;
+      continue;
+    }
+    const SimpleNode s = simple_node(i, j, nx, ny, dx, dy, u, v, p, du, dv, rho);
+    double *row = v64 + rp[idx];
+    row[0] = s.as;
+    row[1] = s.aw;
+    row[2] = s.ap;
+    row[3] = s.ae;
+    row[4] = s.an;
+    rhs[idx] = s.imbalance;
+    const double a = fabs(s.imbalance);
+    if (a > mx) mx = a;
+  }
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int t = 0; t < kBlock; ++t)
+      if (sh[t] > m) m = sh[t];
+    partials[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_final(int nparts, const double *partials, double *out) {
+  __shared__ double sh[kBlock];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
+    if (partials[b] > m) m = partials[b];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 0; t < kBlock; ++t)
+      if (sh[t] > m) m = sh[t];
+    *out = m;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_simple_corrections(int64_t nx, int64_t ny, double dx, double dy,
+                                                               double relax, const double *__restrict__ pp,
+                                                               const double *__restrict__ du,
+                                                               const double *__restrict__ dv, double *p, double *u,
+                                                               double *v) {
+#pragma clang fp contract(off)
+  const int64_t n = nx * ny;
+  const double two = 2.0;
+  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * kBlock) {
+    const int64_t j = idx / nx, i = idx - j * nx;
+    if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1) continue;
+    p[idx] = p[idx] + relax * pp[idx];
+    const double dpx = (pp[idx + 1] - pp[idx - 1]) / (two * dx);
+    const double dpy = (pp[idx + nx] - pp[idx - nx]) / (two * dy);
+    u[idx] = u[idx] - du[idx] * dpx * dx;
+    v[idx] = v[idx] - dv[idx] * dpy * dy;
+  }
+}
+
+// ---- Galerkin refresh ------------------------------------------------------
+
+// C = A B on C's fixed pattern (sorted rows), one lane per row: products in
+// the order (entry of A's row) x (entry of B's row), each added to C's slot
+// of its column (binary search in the row).
+__global__ __launch_bounds__(kBlock) void k_spgemm_numeric(int nrows, const int32_t *__restrict__ arp,
+                                                           const int32_t *__restrict__ aci,
+                                                           const double *__restrict__ av,
+                                                           const int32_t *__restrict__ brp,
+                                                           const int32_t *__restrict__ bci,
+                                                           const double *__restrict__ bv,
+                                                           const int32_t *__restrict__ crp,
+                                                           const int32_t *__restrict__ cci, double *cv) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrows; i += gridDim.x * kBlock) {
+    const int c0 = crp[i], c1 = crp[i + 1];
+    for (int k = c0; k < c1; ++k) cv[k] = -0.0;
+    for (int ka = arp[i]; ka < arp[i + 1]; ++ka) {
+      const int r = aci[ka];
+      const double a = av[ka];
+      for (int kb = brp[r]; kb < brp[r + 1]; ++kb) {
+        const int col = bci[kb];
+        int lo = c0, hi = c1;  // first slot with cci >= col; present by construction
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (cci[mid] < col) lo = mid + 1;
+          else hi = mid;
+        }
+        const double prod = a * bv[kb];
+        cv[lo] = cv[lo] + prod;
+      }
+    }
+  }
+}
+
+// Dense [A | I] (n x 2n, row-major): identity on the right, then the CSR
+// entries added to the zeroed left half (0.0 + a_ij, the host's
+// accumulation).
+__global__ __launch_bounds__(kBlock) void k_dense_fill(int n, double *m) {
+  const int64_t w = 2 * (int64_t)n;
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < (int64_t)n * w; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t r = e / w, c = e - r * w;
+    m[e] = c == n + r ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_scatter(int n, const int32_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci,
+                                                          const double *__restrict__ v, double *m) {
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) m[(int64_t)r * 2 * n + ci[k]] = m[(int64_t)r * 2 * n + ci[k]] + v[k];
+}
+
+// Partial-pivot search, restated for a block of rows: the host scans rows
+// c, c+1, ... replacing the best only when |m_rc| is strictly larger, so
+// row c is the initial candidate (even when it is NaN) and later NaNs never
+// win.  A block not holding row c starts empty (-1); merging partials in
+// block order with the same strict rule gives the host's row.
+struct Piv {
+  double val;
+  int row;
+};
+
+__device__ inline Piv piv_merge(Piv a, Piv b) { return b.val > a.val ? b : a; }
+
+constexpr int kGjRows = 4;  // rows per block of the elimination step
+
+// Column 0's partials of the filled matrix.
+__global__ __launch_bounds__(kBlock) void k_gj_init(int n, const double *m, double *pval, int *prow) {
+  const int r0 = blockIdx.x * kGjRows;
+  if (threadIdx.x == 0) {
+    Piv acc{-1.0, -1};
+    for (int r = r0; r < min(n, r0 + kGjRows); ++r) {
+      const double a = fabs(m[(int64_t)r * 2 * n]);
+      if (r == 0) acc = Piv{a, 0};
+      else acc = piv_merge(acc, Piv{a, r});
+    }
+    pval[blockIdx.x] = acc.val;
+    prow[blockIdx.x] = acc.row;
+  }
+}
+
+// One Gauss-Jordan column step on [A | I], ping-pong src -> dst: pivot row p
+// from the partials, rows c and p swapped, row c scaled by d = 1 / m_pc,
+// every other row r with f = m_rc != 0 updated m_rj - f (m_pj d).  Writes
+// the next column's pivot partials.  A singular column (best pivot 0 or not
+// finite) sets *bad; every later step then returns at once.
+__global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *src, double *dst,
+                                                    const double *pin, const int *rin, double *pout, int *rout,
+                                                    int nparts, int *bad) {
+#pragma clang fp contract(off)
+  __shared__ int s_p;
+  __shared__ double s_next[kGjRows];
+  if (threadIdx.x == 0) {
+    Piv acc{-1.0, -1};
+    for (int b = 0; b < nparts; ++b) acc = piv_merge(acc, Piv{pin[b], rin[b]});
+    s_p = (*bad || !(acc.val > 0.0 && acc.val <= 1.7976931348623157e308)) ? -1 : acc.row;
+  }
+  __syncthreads();
+  const int p = s_p;
+  if (p < 0) {
+    if (threadIdx.x == 0) *bad = 1;
+    return;
+  }
+  const int64_t w = 2 * (int64_t)n;
+  const double d = 1.0 / src[(int64_t)p * w + c];
+  const int r0 = blockIdx.x * kGjRows;
+  for (int q = 0; q < kGjRows; ++q) {
+    const int r = r0 + q;
+    if (r >= n) break;
+    const int sr = r == c ? p : (r == p ? c : r);
+    const double *srow = src + (int64_t)sr * w;
+    const double *prow = src + (int64_t)p * w;
+    double *drow = dst + (int64_t)r * w;
+    const double f = srow[c];
+    for (int64_t j = threadIdx.x; j < w; j += kBlock) {
+      double out;
+      if (r == c) out = prow[j] * d;
+      else if (f == 0.0) out = srow[j];
+      else {
+        const double piv_j = prow[j] * d;
+        const double t = f * piv_j;
+        out = srow[j] - t;
+      }
+      drow[j] = out;
+      if (j == c + 1) s_next[q] = fabs(out);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && c + 1 < n) {
+    Piv acc{-1.0, -1};
+    for (int q = 0; q < kGjRows; ++q) {
+      const int r = r0 + q;
+      if (r >= n) break;
+      if (r < c + 1) continue;
+      if (r == c + 1) acc = Piv{s_next[q], r};
+      else acc = piv_merge(acc, Piv{s_next[q], r});
+    }
+    pout[blockIdx.x] = acc.val;
+    rout[blockIdx.x] = acc.row;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gj_extract(int n, const double *m, double *inv) {
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t r = e / n, c = e - r * n;
+    inv[e] = m[r * 2 * n + n + c];
+  }
+}
+
+}  // namespace
+
+void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *v64) {
+  if (nnz == 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(nnz)), dim3(kBlock), 0, stream(), nnz, src, perm, v64);
+}
+
+bool symmetric_on_device(const DevCsr &d) {
+  if (d.nnz == 0) return true;
+  int *bad = flag_reset();
+  hipLaunchKernelGGL(k_mirror_check, dim3(grid_for(d.nnz)), dim3(kBlock), 0, stream(), d.nnz, d.v64.p,
+                     d.fmt_mirror.p, bad);
+  return flag_read() == 0;
+}
+
+void scatter_formats(DevCsr &d) {
+  if (d.nnz > 0) {
+    double *f64 = d.dia ? d.dia_v64.p : (d.sell ? d.sell_v64.p : nullptr);
+    float *f32 = d.dia ? d.dia_v32.p : (d.sell ? d.sell_v32.p : nullptr);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(d.nnz)), dim3(kBlock), 0, stream(), d.nnz, d.v64.p, d.v32.p,
+                       d.fmt_pos.p, f64, f32);
+  }
+  if (d.sdinv64.p && d.n > 0)
+    hipLaunchKernelGGL(k_sdinv, dim3(grid_for(d.n)), dim3(kBlock), 0, stream(), d.n, d.v64.p, d.diag_slot.p,
+                       d.sdinv64.p, d.sdinv32.p);
+}
+
+void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, const double *u, const double *v, const double *p,
+                    const double *du, const double *dv, const double *rho, const int32_t *rp, double *v64,
+                    double *rhs, bool want_max) {
+  const int grid = grid_for(nx * ny);
+  RedWs &ws = ctx().ws;
+  hipLaunchKernelGGL(k_simple_refresh, dim3(grid), dim3(kBlock), 0, stream(), nx, ny, dx, dy, u, v, p, du, dv, rho,
+                     rp, v64, rhs, ws.partials.p);
+  if (!want_max) return;
+  g_max.ensure(1);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, stream(), grid, ws.partials.p, g_max.p);
+}
+
+double simple_max_read() {
+  double h = 0.0;
+  CFDLA_HIP(hipMemcpyAsync(&h, g_max.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  sync_stream();
+  return h;
+}
+
+void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
+                        const double *du, const double *dv, double *p, double *u, double *v) {
+  hipLaunchKernelGGL(k_simple_corrections, dim3(grid_for(nx * ny)), dim3(kBlock), 0, stream(), nx, ny, dx, dy,
+                     relax, pp, du, dv, p, u, v);
+}
+
+void spgemm_numeric(int nrows, const int32_t *arp, const int32_t *aci, const double *av, const int32_t *brp,
+                    const int32_t *bci, const double *bv, const int32_t *crp, const int32_t *cci, double *cv) {
+  if (nrows == 0) return;
+  hipLaunchKernelGGL(k_spgemm_numeric, dim3(std::max(1, std::min(kMaxBlocks, (nrows + kBlock - 1) / kBlock))),
+                     dim3(kBlock), 0, stream(), nrows, arp, aci, av, brp, bci, bv, crp, cci, cv);
+}
+
+void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const double *v, double *inv) {
+  DevBuf<double> buf[2];
+  buf[0].alloc((size_t)n * 2 * n);
+  buf[1].alloc((size_t)n * 2 * n);
+  const int nb = (n + kGjRows - 1) / kGjRows;
+  DevBuf<double> pv[2];
+  DevBuf<int> pr[2];
+  for (int k = 0; k < 2; ++k) {
+    pv[k].alloc((size_t)nb);
+    pr[k].alloc((size_t)nb);
+  }
+  int *bad = flag_reset();
+  hipLaunchKernelGGL(k_dense_fill, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
+  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(n)), dim3(kBlock), 0, stream(), n, rp, ci, v, buf[0].p);
+  hipLaunchKernelGGL(k_gj_init, dim3(nb), dim3(kBlock), 0, stream(), n, buf[0].p, pv[0].p, pr[0].p);
+  for (int c = 0; c < n; ++c)
+    hipLaunchKernelGGL(k_gj_step, dim3(nb), dim3(kBlock), 0, stream(), n, c, buf[c & 1].p, buf[(c + 1) & 1].p,
+                       pv[c & 1].p, pr[c & 1].p, pv[(c + 1) & 1].p, pr[(c + 1) & 1].p, nb, bad);
+  hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n, buf[n & 1].p,
+                     inv);
+  CFDLA_REQUIRE(flag_read() == 0, CFDLA_ERR_NUMERICAL,
+                "AMG coarsest operator is singular; use coarse_solver = reference sweeps");
+}
+
+}  // namespace cfdla

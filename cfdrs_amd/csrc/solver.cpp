@@ -209,7 +209,7 @@ namespace {
 // Diagonal layout (see DiaDesc) for square matrices of at least 4096 rows
 // whose nonzeros lie on few diagonals filled to at least 90 %.  Returns the
 // stored diagonals diagonal-major in val (nd x n).
-bool build_dia(const HostCsr &h, DiaDesc &d, std::vector<double> &val) {
+bool build_dia(const HostCsr &h, DiaDesc &d, std::vector<double> &val, bool allow_sym) {
   if (h.n != h.m || h.n < 4096 || h.nnz() == 0) return false;
   const int64_t n = h.n;
   std::vector<int64_t> offs;
@@ -235,7 +235,7 @@ bool build_dia(const HostCsr &h, DiaDesc &d, std::vector<double> &val) {
     auto it = std::lower_bound(offs.begin(), offs.end(), o);
     return it != offs.end() && *it == o ? it - offs.begin() : -1;
   };
-  bool sym = true;
+  bool sym = allow_sym;
   for (int64_t kd = 0; kd < nall && sym; ++kd) {
     const int64_t o = offs[(size_t)kd];
     if (o <= 0) continue;
@@ -311,7 +311,8 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     sdinv32.upload(df.data(), df.size());
   }
   std::vector<double> dvals;
-  dia = allow_sell && build_dia(h, dia_desc, dvals);
+  dia = allow_sell && build_dia(h, dia_desc, dvals, !general_dia);
+  maps_ready = false;
   if (dia) {
     dia_v64.upload(dvals.data(), dvals.size());
     if (want_fp32) {
@@ -340,6 +341,54 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     }
   }
   sync_stream();
+}
+
+void DevCsr::build_maps(const HostCsr &h) {
+  std::vector<int32_t> pos((size_t)h.nnz(), -1), mirror, dslot((size_t)h.n, -1);
+  if (dia && dia_desc.sym) mirror.assign((size_t)h.nnz(), -1);
+  for (int64_t i = 0; i < h.n; ++i)
+    for (int64_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
+      const int64_t c = h.ci[k];
+      if (c == i) dslot[(size_t)i] = (int32_t)k;
+      if (dia) {
+        const int64_t o = c - i;
+        if (dia_desc.sym && o != 0) {
+          // the transposed entry (c, i); sorted rows
+          auto b = h.ci.begin() + h.rp[c], e = h.ci.begin() + h.rp[c + 1];
+          auto it = std::lower_bound(b, e, (int32_t)i);
+          mirror[(size_t)k] = it != e && *it == i ? (int32_t)(it - h.ci.begin()) : -2;
+          if (o < 0) continue;  // lower entries are read through their upper mirror
+        }
+        for (int d = 0; d < dia_desc.nd; ++d)
+          if (dia_desc.off[d] == o) pos[(size_t)k] = (int32_t)(d * h.n + i);
+      }
+    }
+  if (sell) {
+    // the layout of build_sell: slice s starts after 64 * (longest row) slots
+    // of every slice before it; entry q of row r sits at start + q * 64 + lane
+    int64_t start = 0;
+    for (int64_t s0 = 0; s0 * 64 < h.n; ++s0) {
+      const int64_t r1 = std::min<int64_t>(h.n, s0 * 64 + 64);
+      int64_t w = 0;
+      for (int64_t r = s0 * 64; r < r1; ++r) {
+        w = std::max<int64_t>(w, h.rp[r + 1] - h.rp[r]);
+        for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) pos[(size_t)k] = (int32_t)(start + (k - h.rp[r]) * 64 + (r - s0 * 64));
+      }
+      start += w * 64;
+    }
+  }
+  fmt_pos.upload(pos.data(), pos.size());
+  if (!mirror.empty()) fmt_mirror.upload(mirror.data(), mirror.size());
+  diag_slot.upload(dslot.data(), dslot.size());
+  sync_stream();
+  maps_ready = true;
+}
+
+bool DevCsr::refresh_from_v64() {
+  CFDLA_REQUIRE(maps_ready, CFDLA_ERR_INVALID_ARG, "device refresh before its maps were built");
+  if (dia && dia_desc.sym && !symmetric_on_device(*this)) return false;
+  scatter_formats(*this);
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -513,6 +562,7 @@ AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
 }
 
 void AmgPrecond::upload() {
+  sync_host();
   const size_t L = host.levels.size();
   lv.clear();
   lv.resize(L);
@@ -543,7 +593,66 @@ void AmgPrecond::upload() {
 
 void AmgPrecond::update_values(const double *values) {
   recompute_hierarchy(host, values);
+  host_stale = false;
   if (on_device) upload();
+}
+
+namespace {
+// Rewrites the device formats of a level from its v64; a symmetric diagonal
+// layout whose values stopped being symmetric is rebuilt (once) as general.
+void refresh_level(DevCsr &d, HostCsr &h, bool want_dinv) {
+  if (!d.maps_ready) d.build_maps(h);
+  if (d.refresh_from_v64()) return;
+  d.v64.download(h.v.data(), h.v.size());
+  sync_stream();
+  d.general_dia = true;
+  d.upload(h, true, want_dinv);
+}
+}  // namespace
+
+void AmgPrecond::update_values_device(const double *fine_values) {
+  ensure_device();
+  const size_t L = lv.size();
+  DevCsr &a0 = lv[0].A;
+  if (a0.nnz > 0 && a0.v64.p != fine_values)
+    CFDLA_HIP(hipMemcpyAsync(a0.v64.p, fine_values, (size_t)a0.nnz * sizeof(double), hipMemcpyDeviceToDevice,
+                             stream()));
+  refresh_level(a0, host.levels[0].A, true);
+  if (ra.size() != L - 1) {
+    // the patterns of the host's intermediate products R_l A_l
+    ra.clear();
+    ra.resize(L - 1);
+    for (size_t l = 0; l + 1 < L; ++l) {
+      const HostCsr t = csr_multiply(host.levels[l].R, host.levels[l].A);
+      std::vector<int32_t> rp32(t.rp.size());
+      for (size_t k = 0; k < t.rp.size(); ++k) rp32[k] = (int32_t)t.rp[k];
+      ra[l].rp.upload(rp32.data(), rp32.size());
+      ra[l].ci.upload(t.ci.data(), t.ci.size());
+      ra[l].v.alloc(t.ci.size());
+    }
+    sync_stream();
+  }
+  for (size_t l = 0; l + 1 < L; ++l) {
+    DevLevel &f = lv[l], &c = lv[l + 1];
+    const int nc = (int)host.levels[l].R.n;
+    spgemm_numeric(nc, f.R.rp.p, f.R.ci.p, f.R.v64.p, f.A.rp.p, f.A.ci.p, f.A.v64.p, ra[l].rp.p, ra[l].ci.p,
+                   ra[l].v.p);
+    spgemm_numeric(nc, ra[l].rp.p, ra[l].ci.p, ra[l].v.p, f.P.rp.p, f.P.ci.p, f.P.v64.p, c.A.rp.p, c.A.ci.p,
+                   c.A.v64.p);
+    refresh_level(c.A, host.levels[l + 1].A, true);
+  }
+  if (cfg.coarse_solver == 1) {
+    const DevCsr &cA = lv.back().A;
+    dense_inverse_device(coarse_n, cA.rp.p, cA.ci.p, cA.v64.p, coarse_inv.p);
+  }
+  host_stale = true;
+}
+
+void AmgPrecond::sync_host() {
+  if (!host_stale) return;
+  for (size_t l = 0; l < lv.size(); ++l) lv[l].A.v64.download(host.levels[l].A.v.data(), host.levels[l].A.v.size());
+  sync_stream();
+  host_stale = false;
 }
 
 template <typename T>

@@ -78,6 +78,12 @@ struct DevLevel {
   template <typename T> T *r();
 };
 
+// Device pattern and values of an intermediate product (R_l A_l).
+struct DevPattern {
+  DevBuf<int32_t> rp, ci;
+  DevBuf<double> v;
+};
+
 struct AmgPrecond : Precond {
   cfdla_amg_config cfg{};
   HostHierarchy host;
@@ -86,6 +92,13 @@ struct AmgPrecond : Precond {
   int coarse_n = 0;
   double setup_time = 0.0;
   bool on_device = false;
+  bool host_stale = false;  // a device refresh left newer level values than host
+  std::vector<DevPattern> ra;  // pattern of R_l A_l per level, made on the first device refresh
+  // AlgebraicMultigrid::recompute (amg.rs:179-214) on the device from
+  // level-0 values already on the device (CSR order of the fine matrix).
+  void update_values_device(const double *fine_values);
+  // Host level values from the device after update_values_device.
+  void sync_host();
   // Builds the host hierarchy only; device copies are made on first use.
   AmgPrecond(const HostCsr &a, const cfdla_amg_config &c);
   void upload();

@@ -207,7 +207,15 @@ class AlgebraicMultigrid(_Preconditioner):
         """AlgebraicMultigrid::recompute (amg.rs:179-214)."""
         if fine_matrix.nnz != self.matrix.nnz:
             raise N.CfdlaError(N.ERR_DIMENSION, "recompute requires the same sparsity pattern")
-        N.call("cfdla_amg_update_values", self.handle, N.ptr(np.ascontiguousarray(fine_matrix.values)))
+        N.call("cfdla_amg_update_values", self.handle, N.ptr(np.ascontiguousarray(fine_matrix.creation_order_values())))
+
+    def recompute_device(self, fine_matrix: CsrMatrix) -> None:
+        """AlgebraicMultigrid::recompute (amg.rs:179-214) from the current
+        device values of the matrix the hierarchy was built from, on the GPU
+        (bit-identical to ``recompute`` with the same values)."""
+        if fine_matrix is not self.matrix:
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "recompute_device needs the matrix the hierarchy was built from")
+        N.call("cfdla_amg_update_values_device", self.handle, fine_matrix.handle)
 
     @property
     def statistics(self) -> AMGStatistics:

@@ -46,6 +46,29 @@ def pressure_correction_2d(nxc: int, nyc: int, dx: float, dy: float, dirichlet=(
     return rp, ci, v
 
 
+def simple_pressure(nx: int, ny: int, dx: float, dy: float, side, u, v, p, d_u, d_v, rho):
+    """(row_ptr, col, values, rhs, max_residual) of the SIMPLE
+    pressure-correction system on an nx x ny node grid
+    (crates/cfd-2d/src/solvers/simple/pressure.rs:18-240); side = (west,
+    east, south, north) of 0 plain / 1 pressure / 2 periodic."""
+    sd = np.asarray(side, np.int32)
+    f = [np.ascontiguousarray(a, np.float64) for a in (u, v, p, d_u, d_v, rho)]
+    for a in f:
+        if a.size != nx * ny:
+            raise N.CfdlaError(N.ERR_DIMENSION, "fields must have nx * ny entries")
+    nnz = C.c_int64()
+    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), *(N.ptr(a) for a in f), C.byref(nnz), None,
+           None, None, None, None)
+    rp = np.empty(nx * ny + 1, np.int64)
+    ci = np.empty(nnz.value, np.int64)
+    val = np.empty(nnz.value)
+    rhs = np.empty(nx * ny)
+    mr = C.c_double()
+    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), *(N.ptr(a) for a in f), C.byref(nnz),
+           N.ptr(rp), N.ptr(ci), N.ptr(val), N.ptr(rhs), C.byref(mr))
+    return rp, ci, val, rhs, mr.value
+
+
 def varcoef(nx: int, ny: int, nz: int, inv_h2: float, k, dir_mask: int = 0, dir_val=None):
     """(row_ptr, col, values, rhs_bc) of the variable-coefficient operator.
     dir_mask bits: 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z."""

@@ -33,10 +33,31 @@ class CsrMatrix:
     return host copies in sorted-column order.
     """
 
-    def __init__(self, handle, nrows: int, ncols: int, nnz: int, row_ptr, col_indices, values):
+    def __init__(self, handle, nrows: int, ncols: int, nnz: int, row_ptr, col_indices, values, order=None):
         self._h = handle
         self.nrows, self.ncols, self.nnz = nrows, ncols, nnz
-        self.row_ptr, self.col_indices, self.values = row_ptr, col_indices, values
+        self.row_ptr, self.col_indices, self._values = row_ptr, col_indices, values
+        self._order = order  # sorted position -> creation position (None: rows were sorted)
+
+    @property
+    def values(self):
+        """Values in sorted-column order (fetched from the device after a
+        device update)."""
+        if self._values is None:
+            v = np.empty(self.nnz)
+            N.call("cfdla_matrix_get_values", self.handle, N.ptr(v))
+            self._values = v if self._order is None else v[self._order]
+        return self._values
+
+    def creation_order_values(self):
+        """Values in the order the matrix was created with (the order every
+        values argument of the library takes)."""
+        v = self.values
+        if self._order is None:
+            return v
+        out = np.empty_like(v)
+        out[self._order] = v
+        return out
 
     @classmethod
     def from_parts(cls, values, col_indices, row_ptr, nrows: int, ncols: int) -> "CsrMatrix":
@@ -54,7 +75,7 @@ class CsrMatrix:
             if nrows and not _rows_sorted(rp, ci) else None
         if order is not None:
             ci, v = ci[order], v[order]
-        return cls(h, nrows, ncols, int(rp[-1]), rp, ci, v)
+        return cls(h, nrows, ncols, int(rp[-1]), rp, ci, v, order)
 
     @classmethod
     def from_scipy(cls, m) -> "CsrMatrix":
@@ -77,6 +98,21 @@ class CsrMatrix:
         if v.size != self.nnz:
             raise N.CfdlaError(N.ERR_DIMENSION, "values length differs from nnz")
         N.call("cfdla_matrix_update_values", self.handle, N.ptr(v))
+        self._values = v.copy() if self._order is None else v[self._order]
+
+    def update_values_device(self, values) -> None:
+        """Replace the values from a CUDA float64 tensor of nnz entries in the
+        caller's original order; the device copies are rewritten on the GPU
+        and no host copy is made."""
+        import torch
+
+        if not (_is_torch(values) and values.is_cuda and values.dtype == torch.float64 and values.is_contiguous()):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "values must be a contiguous CUDA float64 tensor")
+        if values.numel() != self.nnz:
+            raise N.CfdlaError(N.ERR_DIMENSION, "values length differs from nnz")
+        bind_torch_stream()
+        N.call("cfdla_matrix_update_values_device", self.handle, values.data_ptr())
+        self._values = None
 
     def spmv(self, x):
         """y = A x (numpy host vector or torch device tensor, FP64 or FP32)."""

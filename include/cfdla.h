@@ -175,6 +175,16 @@ cfdla_status cfdla_matrix_create_device(int64_t nrows, int64_t ncols, int64_t nn
                                         const double *values_dev, cfdla_matrix *out);
 /* Replace the values of the same pattern (host array of nnz doubles). */
 cfdla_status cfdla_matrix_update_values(cfdla_matrix m, const double *values);
+/* Same from a device array of nnz doubles in creation order: the FP64 and
+ * FP32 device copies and the diagonal / sliced-ELL arrays are rewritten on
+ * the device, without a host copy (the host copy is refreshed only when a
+ * host-side consumer needs it).  A symmetric diagonal layout whose new
+ * values are not symmetric is rebuilt once as a general one.  Device side of
+ * refilling CsrMatrix::values_mut on a fixed pattern
+ * (crates/cfd-2d/src/solvers/simple/pressure.rs:244-260). */
+cfdla_status cfdla_matrix_update_values_device(cfdla_matrix m, const double *values_dev);
+/* Current values into a host array of nnz doubles, in creation order. */
+cfdla_status cfdla_matrix_get_values(cfdla_matrix m, double *values);
 cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, int64_t *nnz);
 /* Device storage chosen for the matrix (uploads it if needed): *format is
  * CFDLA_STORAGE_DIA (stencil matrices: diagonal-major values, no column
@@ -207,8 +217,6 @@ cfdla_status cfdla_residual_device(cfdla_matrix m, const double *b, const double
  * DiagJacobi, crates/cfd-1d/src/solver/core/linear_system.rs:270-299).
  * Non-finite diagonal -> CFDLA_ERR_NUMERICAL. */
 cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out);
-/* AlgebraicMultigrid::new (amg.rs:133-152).  Host setup (coarsening,
- * interpolation, Galerkin products); device V-cycle. */
 /* Multicolour SOR preconditioner, z = (D / omega + L_c)^-1 r: one forward
  * sweep from z = 0 over the rows in colour order (greedy colouring of the
  * symmetrized pattern in breadth-first order; a 5-point stencil gets
@@ -219,6 +227,8 @@ cfdla_status cfdla_precond_create_jacobi(cfdla_matrix m, cfdla_precond *out);
  * sweep order, hence the iterates, differ.  *ncolours (nullable) receives the
  * colour count. */
 cfdla_status cfdla_precond_create_sor(cfdla_matrix m, double omega, int32_t *ncolours, cfdla_precond *out);
+/* AlgebraicMultigrid::new (amg.rs:133-152).  Host setup (coarsening,
+ * interpolation, Galerkin products); device V-cycle. */
 cfdla_status cfdla_precond_create_amg(cfdla_matrix m, const cfdla_amg_config *config,
                                       cfdla_precond *out);
 /* z = M^-1 r (z fully overwritten, amg.rs:553-554) on host vectors.
@@ -229,6 +239,16 @@ cfdla_status cfdla_precond_apply_device(cfdla_precond p, int32_t dtype, const vo
 /* AlgebraicMultigrid::recompute (amg.rs:179-214): new fine values, same
  * pattern; coarse operators recomputed with the existing P and R. */
 cfdla_status cfdla_amg_update_values(cfdla_precond p, const double *values);
+/* The same recompute from the current device values of m (the matrix the
+ * preconditioner was built from), on the device: the level-0 copy is
+ * refreshed, A_{l+1} = (R_l A_l) P_l is formed by numeric products on the
+ * fixed patterns in the host's summation order, then the smoother diagonals
+ * and the dense coarsest inverse (Gauss-Jordan with partial pivoting, the
+ * host's operation order) are recomputed.  The result is bit-identical to
+ * cfdla_amg_update_values with the same values; nothing crosses PCIe except
+ * a singularity flag.  Device side of the pressure path's cached-hierarchy
+ * refresh (crates/cfd-2d/src/pressure_velocity/correction.rs:56-89). */
+cfdla_status cfdla_amg_update_values_device(cfdla_precond p, cfdla_matrix m);
 cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out);
 /* Hierarchy export for structure checks.  which: 0 = A_l, 1 = R_l, 2 = P_l.
  * First call with arrays NULL to get sizes. */
@@ -376,6 +396,42 @@ cfdla_status cfdla_synth_uniform(uint64_t seed, double lo, double hi, int64_t n,
  * exp(-r^2 / (2 corr_len^2)) in cells, normalised to sample mean 0 and
  * sample standard deviation 1. */
 cfdla_status cfdla_synth_grf(int64_t nx, int64_t ny, double corr_len, uint64_t seed, double *out);
+/* SIMPLE pressure-correction system on an nx*ny node grid, node (i, j) =
+ * row j*nx + i (crates/cfd-2d/src/solvers/simple/pressure.rs:18-240).
+ * Interior rows: a_nb = rho d_face (face length / spacing) with d_face the
+ * mean of the nodes' momentum d_u / d_v, a_P = -sum a_nb; rhs = Rhie-Chow
+ * mass imbalance.  Boundary rows: identity across a pressure side, else 1 on
+ * the diagonal and -1/m on the m nodes across the touching sides (periodic
+ * sides: the partner node); node 0 is an identity row when no side is a
+ * pressure side.  side[4] = {west, east, south, north} of CFDLA_SIDE_*.
+ * Fields are nx*ny arrays.  rhs is 0 at boundary nodes; *max_residual
+ * (nullable) = max |mass imbalance|.  Two-phase: with row_ptr NULL only *nnz
+ * is written and the fields may be NULL. */
+#define CFDLA_SIDE_PLAIN 0    /* wall, outflow, velocity inlet */
+#define CFDLA_SIDE_PRESSURE 1 /* Dirichlet / pressure inlet / pressure outlet: anchors p' */
+#define CFDLA_SIDE_PERIODIC 2 /* periodic with the opposite side */
+cfdla_status cfdla_problem_simple_pressure(int64_t nx, int64_t ny, double dx, double dy, const int32_t *side,
+                                           const double *u, const double *v, const double *p, const double *d_u,
+                                           const double *d_v, const double *rho, int64_t *nnz, int64_t *row_ptr,
+                                           int64_t *col_indices, double *values, double *rhs,
+                                           double *max_residual);
+/* Per-iteration refresh of that system on the device (the refresh branch,
+ * pressure.rs:244-450): m must hold cfdla_problem_simple_pressure's pattern
+ * for the same nx, ny; its interior rows are recomputed from device fields
+ * (the same operations as the host assembly, bit for bit), boundary rows
+ * keep their values, rhs_dev (nx*ny) receives the imbalance and 0 at
+ * boundary nodes.  *max_residual (host, nullable; non-NULL synchronizes)
+ * receives max |mass imbalance|. */
+cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, double dx, double dy,
+                                                  const double *u, const double *v, const double *p,
+                                                  const double *d_u, const double *d_v, const double *rho,
+                                                  double *rhs_dev, double *max_residual);
+/* SimpleAlgorithm::apply_corrections (pressure.rs:487-520) on device arrays,
+ * interior nodes: p += relax p', u -= d_u (dp'/dx) dx, v -= d_v (dp'/dy) dy
+ * with central differences of p'. */
+cfdla_status cfdla_simple_corrections_device(int64_t nx, int64_t ny, double dx, double dy, double relax,
+                                             const double *p_prime, const double *d_u, const double *d_v,
+                                             double *p, double *u, double *v);
 /* Cell-centred pressure-correction Laplacian on nxc*nyc interior cells,
  * all fluid (crates/cfd-2d/src/pressure_velocity/correction.rs:273-343);
  * dirichlet[4] = {west, east, south, north}.  Without a Dirichlet side the

@@ -982,6 +982,109 @@ Csr assemble_momentum_u(i64 nx, i64 ny, double dx, double dy, double dt, double 
   return bd.build();
 }
 
+// SIMPLE pressure-correction system on an nx*ny node grid (node fields
+// indexed j*nx + i): crates/cfd-2d/src/solvers/simple/pressure.rs:18-240 (the
+// first assembly; the refresh branch :244-450 computes the same values into
+// the fixed pattern).  Interior rows: d-coefficient Laplacian with a_p the
+// negated sum, right-hand side the Rhie-Chow mass imbalance (:57-147).
+// Boundary rows (:150-240): a pressure-anchor side gives an identity row,
+// otherwise the row averages its neighbours across each touching side
+// (pressure_velocity/boundary.rs:29-66); without any anchor side node 0 is an
+// identity row.  side = {west, east, south, north}: 0 plain (wall, outflow,
+// velocity inlet), 1 pressure anchor, 2 periodic with the opposite side.
+// max_residual = max |mass imbalance| over interior nodes.
+Csr assemble_simple_pressure(i64 nx, i64 ny, double dx, double dy, const int side[4], const double *u,
+                             const double *v, const double *p, const double *du, const double *dv,
+                             const double *rho, double *rhs, double *max_residual) {
+  const i64 n = nx * ny;
+  const double half = 0.5, two = 2.0;
+  Builder bd(n, n);
+  std::fill(rhs, rhs + n, 0.0);
+  double maxr = 0.0;
+  for (i64 j = 1; j < ny - 1; ++j)
+    for (i64 i = 1; i < nx - 1; ++i) {
+      const i64 idx = j * nx + i;
+      const double u_p = u[idx], u_e = u[idx + 1], u_w = u[idx - 1];
+      const double v_p = v[idx], v_n = v[idx + nx], v_s = v[idx - nx];
+      const double p_p = p[idx], p_e = p[idx + 1], p_w = p[idx - 1], p_n = p[idx + nx], p_s = p[idx - nx];
+      // one-sided extrapolation next to the boundary (:77-96)
+      const double p_ee = i + 2 < nx ? p[idx + 2] : two * p_e - p_p;
+      const double p_ww = i >= 2 ? p[idx - 2] : two * p_w - p_p;
+      const double p_nn = j + 2 < ny ? p[idx + 2 * nx] : two * p_n - p_p;
+      const double p_ss = j >= 2 ? p[idx - 2 * nx] : two * p_s - p_p;
+      const double d_e = (du[idx] + du[idx + 1]) * half;
+      const double d_w = (du[idx - 1] + du[idx]) * half;
+      const double d_n = (dv[idx] + dv[idx + nx]) * half;
+      const double d_s = (dv[idx - nx] + dv[idx]) * half;
+      // face velocities: linear interpolation plus the Rhie-Chow difference
+      // between the averaged cell gradients and the face gradient (:103-117)
+      const double gx = (p_e - p_w) / (two * dx);
+      const double uf_e = (u_p + u_e) * half + d_e * ((gx + (p_ee - p_p) / (two * dx)) * half - (p_e - p_p) / dx);
+      const double uf_w = (u_w + u_p) * half + d_w * (((p_p - p_ww) / (two * dx) + gx) * half - (p_p - p_w) / dx);
+      const double gy = (p_n - p_s) / (two * dy);
+      const double vf_n = (v_p + v_n) * half + d_n * ((gy + (p_nn - p_p) / (two * dy)) * half - (p_n - p_p) / dy);
+      const double vf_s = (v_s + v_p) * half + d_s * (((p_p - p_ss) / (two * dy) + gy) * half - (p_p - p_s) / dy);
+      const double r = rho[idx];
+      const double imbalance = r * uf_e * dy - r * uf_w * dy + r * vf_n * dx - r * vf_s * dx;
+      const double a_e = r * d_e * dy / dx, a_w = r * d_w * dy / dx;
+      const double a_n = r * d_n * dx / dy, a_s = r * d_s * dx / dy;
+      bd.add(idx, idx, -(a_e + a_w + a_n + a_s));
+      bd.add(idx, idx + 1, a_e);
+      bd.add(idx, idx - 1, a_w);
+      bd.add(idx, idx + nx, a_n);
+      bd.add(idx, idx - nx, a_s);
+      rhs[idx] = imbalance;
+      if (std::fabs(imbalance) > maxr) maxr = std::fabs(imbalance);
+    }
+  const bool anchored = side[0] == 1 || side[1] == 1 || side[2] == 1 || side[3] == 1;
+  for (i64 j = 0; j < ny; ++j)
+    for (i64 i = 0; i < nx; ++i) {
+      if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) continue;
+      const i64 idx = j * nx + i;
+      // sides in the order west, east, south, north; the inner neighbour
+      // across a side, or the periodic partner node
+      const bool on[4] = {i == 0, i == nx - 1, j == 0, j == ny - 1};
+      const i64 inner[4] = {j * nx + 1, j * nx + nx - 2, nx + i, (ny - 2) * nx + i};
+      const i64 partner[4] = {j * nx + nx - 1, j * nx, (ny - 1) * nx + i, i};
+      std::vector<i64> nb;
+      bool identity = false;
+      for (int s = 0; s < 4 && !identity; ++s) {
+        if (!on[s]) continue;
+        if (side[s] == 1) identity = true;
+        else nb.push_back(side[s] == 2 ? partner[s] : inner[s]);
+      }
+      if (identity || (!anchored && idx == 0) || nb.empty()) {
+        bd.add(idx, idx, 1.0);
+        continue;
+      }
+      std::sort(nb.begin(), nb.end());
+      nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+      const double w = 1.0 / (double)nb.size();
+      bd.add(idx, idx, 1.0);
+      for (i64 c : nb) bd.add(idx, c, -w);
+    }
+  *max_residual = maxr;
+  return bd.build();
+}
+
+// Pressure and velocity corrections after the SIMPLE pressure solve
+// (crates/cfd-2d/src/solvers/simple/pressure.rs:487-520), interior nodes:
+// p += relax p', u -= d_u (dp'/dx) dx, v -= d_v (dp'/dy) dy with central
+// differences of p'.
+void simple_corrections(i64 nx, i64 ny, double dx, double dy, double relax, const double *pp, const double *du,
+                        const double *dv, double *p, double *u, double *v) {
+  const double two = 2.0;
+  for (i64 j = 1; j < ny - 1; ++j)
+    for (i64 i = 1; i < nx - 1; ++i) {
+      const i64 idx = j * nx + i;
+      p[idx] += relax * pp[idx];
+      const double dpx = (pp[idx + 1] - pp[idx - 1]) / (two * dx);
+      const double dpy = (pp[idx + nx] - pp[idx - nx]) / (two * dy);
+      u[idx] -= du[idx] * dpx * dx;
+      v[idx] -= dv[idx] * dpy * dy;
+    }
+}
+
 // Kirchhoff network Laplacian with Dirichlet row replacement and column
 // elimination: crates/cfd-1d/src/solver/core/matrix_assembly.rs:163-247.
 // Returns false on a self-loop or a non-positive / non-finite conductance
@@ -1277,6 +1380,16 @@ void *orc_assemble_varcoef(int64_t nx, int64_t ny, int64_t nz, double inv_h2, co
 void *orc_assemble_momentum_u(int64_t nx, int64_t ny, double dx, double dy, double dt, double mu, double rho,
                               const double *u, const double *v) {
   return new Csr(assemble_momentum_u(nx, ny, dx, dy, dt, mu, rho, u, v));
+}
+void *orc_assemble_simple_pressure(int64_t nx, int64_t ny, double dx, double dy, const int32_t *side4,
+                                   const double *u, const double *v, const double *p, const double *du,
+                                   const double *dv, const double *rho, double *rhs, double *max_residual) {
+  const int sd[4] = {side4[0], side4[1], side4[2], side4[3]};
+  return new Csr(assemble_simple_pressure(nx, ny, dx, dy, sd, u, v, p, du, dv, rho, rhs, max_residual));
+}
+void orc_simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
+                            const double *du, const double *dv, double *p, double *u, double *v) {
+  simple_corrections(nx, ny, dx, dy, relax, pp, du, dv, p, u, v);
 }
 void *orc_assemble_network(int64_t n, int64_t ne, const int64_t *ei, const int64_t *ej, const double *g,
                            const unsigned char *is_dir, const double *dir_val, const double *neumann, double *rhs) {

@@ -91,6 +91,10 @@ def lib():
                                           F64P, F64P]),
             "orc_assemble_momentum_u": (vp, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                              C.c_double, C.c_double, F64P, F64P]),
+            "orc_assemble_simple_pressure": (vp, [C.c_int64, C.c_int64, C.c_double, C.c_double, I32P, F64P, F64P,
+                                                  F64P, F64P, F64P, F64P, F64P, F64P]),
+            "orc_simple_corrections": (None, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, F64P, F64P,
+                                              F64P, F64P, F64P, F64P]),
             "orc_assemble_network": (vp, [C.c_int64, C.c_int64, I64P, I64P, F64P, U8P, F64P, F64P, F64P]),
             "orc_detect_spd": (C.c_int, [vp]),
             "orc_poiseuille_number": (C.c_double, [C.c_double]),
@@ -292,6 +296,25 @@ def assemble_varcoef(nx, ny, nz, inv_h2, k, dir_mask, dir_val):
 def assemble_momentum_u(nx, ny, dx, dy, dt, mu, rho, u, v) -> Csr:
     return Csr(lib().orc_assemble_momentum_u(nx, ny, dx, dy, dt, mu, rho, np.ascontiguousarray(u, np.float64),
                                              np.ascontiguousarray(v, np.float64)))
+
+
+def assemble_simple_pressure(nx, ny, dx, dy, side, u, v, p, du, dv, rho):
+    """SIMPLE pressure-correction system: (matrix, rhs, max |mass imbalance|)."""
+    f = [np.ascontiguousarray(a, np.float64) for a in (u, v, p, du, dv, rho)]
+    for a in f:
+        assert a.size == nx * ny
+    rhs = np.empty(nx * ny)
+    mr = np.zeros(1)
+    a = Csr(lib().orc_assemble_simple_pressure(nx, ny, dx, dy, np.array(side, np.int32), *f, rhs, mr))
+    return a, rhs, float(mr[0])
+
+
+def simple_corrections(nx, ny, dx, dy, relax, pp, du, dv, p, u, v):
+    """In-place p, u, v corrections after the SIMPLE pressure solve."""
+    for a in (p, u, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous and a.size == nx * ny
+    lib().orc_simple_corrections(nx, ny, dx, dy, relax, np.ascontiguousarray(pp, np.float64),
+                                 np.ascontiguousarray(du, np.float64), np.ascontiguousarray(dv, np.float64), p, u, v)
 
 
 def assemble_network(n, ei, ej, g, is_dir, dir_val, neumann=None):

@@ -1,0 +1,238 @@
+"""GPU parity of the device value refresh.
+
+* SIMPLE pressure-correction refresh (cfdla_simple_pressure_refresh_device)
+  against the host assembly, which equals the oracle bit for bit
+  (tests/test_simple_host.py): values, right-hand side and maximum residual
+  bit for bit; products through the refreshed device format against the
+  oracle's product to 1e-13.
+* Device matrix updates: creation-order permutation, symmetric diagonal
+  storage kept under symmetric updates and rebuilt as general under
+  asymmetric ones.
+* AMG recompute on the device against the host recompute: every level
+  operator and the preconditioner's output bit for bit (dense coarsest
+  inverse included).
+* Corrections against the oracle bit for bit, and three SIMPLE pressure
+  iterations run on the device against the same iterations through host
+  assembly and host AMG recompute: identical fields.
+"""
+import numpy as np
+import pytest
+
+import cfdrs_amd as ca
+from cfdrs_amd import problems as P
+from cfdrs_amd import simple as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+WEST_OUTLET = (S.SIDE_PRESSURE, S.SIDE_PLAIN, S.SIDE_PLAIN, S.SIDE_PLAIN)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _fields(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    n = nx * ny
+    return ([rng.uniform(-1, 1, n) for _ in range(3)] + [rng.uniform(0.5, 1.5, n) * 1e-2 for _ in range(2)]
+            + [rng.uniform(0.8, 1.2, n)])
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("nx,ny,side,fmt", [(67, 45, WEST_OUTLET, "csr_rowblock"), (96, 80, WEST_OUTLET, "dia"),
+                                            (96, 80, (0, 0, 0, 0), "dia"), (90, 70, (2, 2, 0, 1), None)],
+                         ids=["csr", "dia", "dia-no-anchor", "periodic"])
+def test_refresh_matches_host_assembly(nx, ny, side, fmt):
+    torch = _torch()
+    dx, dy = 1.0 / (nx - 1), 1.0 / (ny - 1)
+    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, side, fields=_fields(nx, ny, 1))
+    if fmt:
+        assert sysm.matrix.storage()["format"] == fmt
+    for seed in (2, 3):  # two refreshes in a row
+        f = _fields(nx, ny, seed)
+        dev = [torch.tensor(a, device="cuda") for a in f]
+        rhs = torch.full((nx * ny,), 7.0, dtype=torch.float64, device="cuda")
+        mr = sysm.assemble_device(*dev, rhs)
+        rp, ci, v, hrhs, hmr = P.simple_pressure(nx, ny, dx, dy, side, *f)
+        assert np.array_equal(_bits(sysm.matrix.values), _bits(v))
+        assert np.array_equal(_bits(rhs.cpu().numpy()), _bits(hrhs))
+        assert mr == hmr
+        x = np.random.default_rng(seed).uniform(-1, 1, nx * ny)
+        y = sysm.matrix.spmv(torch.tensor(x, device="cuda")).cpu().numpy()
+        yo = O.Csr.from_arrays(rp, ci, v, nx * ny).spmv(x)
+        assert np.allclose(y, yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+
+
+def test_refresh_rejects_foreign_pattern():
+    torch = _torch()
+    nx = ny = 9
+    n = nx * ny
+    m = ca.CsrMatrix.from_parts(np.ones(n), np.arange(n), np.arange(n + 1), n, n)  # interior rows not 5-point
+    z = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for gx, gy in ((nx, ny), (nx + 1, ny)):  # wrong pattern, wrong size
+        with pytest.raises(ca.CfdlaError):
+            ca._native.call("cfdla_simple_pressure_refresh_device", m.handle, gx, gy, 0.1, 0.1,
+                            *(z.data_ptr(),) * 7, None)
+
+
+def test_update_values_device_unsorted_creation():
+    torch = _torch()
+    rng = np.random.default_rng(7)
+    n = 300
+    dense = np.where(rng.random((n, n)) < 0.03, rng.uniform(-1, 1, (n, n)), 0.0) + np.eye(n) * 4
+    rows, cols = np.nonzero(dense)
+    perm = rng.permutation(rows.size)
+    order = np.lexsort((perm, rows))  # row-major, columns shuffled inside each row
+    r, c = rows[order], cols[order]
+    rp = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))])
+    m = ca.CsrMatrix.from_parts(dense[r, c], c, rp, n, n)
+    v2 = rng.uniform(-2, 2, r.size)
+    m.update_values_device(torch.tensor(v2, device="cuda"))
+    d2 = np.zeros((n, n))
+    d2[r, c] = v2
+    x = rng.uniform(-1, 1, n)
+    assert np.allclose(m.spmv(x), d2 @ x, rtol=1e-13, atol=1e-13)
+    assert np.array_equal(m.creation_order_values(), v2)
+    assert np.allclose(m.to_dense(), d2, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_update_values_device_diagonal_storage(symmetric):
+    torch = _torch()
+    nn = 80
+    rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(41, nn * nn)))
+    m = ca.CsrMatrix.from_parts(v, ci, rp, nn * nn, nn * nn)
+    st = m.storage()
+    assert st["format"] == "dia" and st["units"] == 3  # main + two upper diagonals
+    if symmetric:
+        _, _, v2, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(42, nn * nn)))
+    else:
+        v2 = v * (1.0 + 0.1 * P.uniform(43, v.size))
+    m.update_values_device(torch.tensor(v2, device="cuda"))
+    st = m.storage()
+    assert st["format"] == "dia" and st["units"] == (3 if symmetric else 5)
+    x = P.uniform(44, nn * nn)
+    yo = O.Csr.from_arrays(rp, ci, v2, nn * nn).spmv(x)
+    assert np.allclose(m.spmv(x), yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+    y32 = m.spmv(torch.tensor(x, dtype=torch.float32, device="cuda")).cpu().numpy()
+    assert np.allclose(y32, yo, rtol=1e-4, atol=1e-5 * np.abs(yo).max())
+
+
+def _amg_pair(rp, ci, v, n, cfg):
+    a_d = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    a_h = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    m_d, m_h = ca.AlgebraicMultigrid(a_d, cfg), ca.AlgebraicMultigrid(a_h, cfg)
+    r = P.uniform(51, n)
+    z = np.empty(n)
+    m_d.apply_to(r, z)  # both hierarchies on the device before the update
+    m_h.apply_to(r, z)
+    return a_d, a_h, m_d, m_h
+
+
+@pytest.mark.parametrize("case", ["2d-dense-coarse", "2d-sweeps", "3d-dense-coarse", "simple-nonsymmetric"])
+def test_amg_recompute_device_matches_host(case):
+    torch = _torch()
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    if case.startswith("2d"):
+        nn = 96
+        n = nn * nn
+        rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(61, n)), 0b10)
+        _, _, v2, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(2 * P.uniform(62, n)), 0b10)
+        cfg.min_coarse_size = 100
+        if case == "2d-sweeps":
+            cfg.coarse_solver = ca.CoarseSolver.ReferenceSweeps
+    elif case == "3d-dense-coarse":
+        nn = 24
+        n = nn ** 3
+        rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
+        _, _, v2, _ = P.varcoef(nn, nn, nn, float(nn * nn), np.exp(P.uniform(63, n)), 1 << 4)
+    else:
+        nx, ny = 72, 64
+        n = nx * ny
+        rp, ci, v, _, _ = P.simple_pressure(nx, ny, 1 / 71, 1 / 63, WEST_OUTLET, *_fields(nx, ny, 64))
+        _, _, v2, _, _ = P.simple_pressure(nx, ny, 1 / 71, 1 / 63, WEST_OUTLET, *_fields(nx, ny, 65))
+        cfg.min_coarse_size = 200
+    a_d, a_h, m_d, m_h = _amg_pair(rp, ci, v, n, cfg)
+    L = m_d.statistics.num_levels
+    assert L >= 3
+    for rep in range(2):
+        vv = v2 if rep == 0 else v2 * 0.5
+        a_d.update_values_device(torch.tensor(vv, device="cuda"))
+        m_d.recompute_device(a_d)
+        a_h.update_values(vv)
+        m_h.recompute(a_h)
+        for lvl in range(L):
+            assert np.array_equal(_bits(m_d.level(lvl, "A")[2]), _bits(m_h.level(lvl, "A")[2])), lvl
+        r = P.uniform(70 + rep, n)
+        z_d, z_h = np.empty(n), np.empty(n)
+        m_d.apply_to(r, z_d)
+        m_h.apply_to(r, z_h)
+        assert np.array_equal(_bits(z_d), _bits(z_h))
+        rt = torch.tensor(r, dtype=torch.float32, device="cuda")
+        zd32, zh32 = torch.empty_like(rt), torch.empty_like(rt)
+        m_d.apply_to(rt, zd32)
+        m_h.apply_to(rt, zh32)
+        assert torch.equal(zd32, zh32)
+
+
+def test_corrections_match_oracle():
+    torch = _torch()
+    nx, ny, dx, dy, relax = 70, 50, 1 / 69, 1 / 49, 0.3
+    u, v, p, du, dv, _ = _fields(nx, ny, 81)
+    pp = P.uniform(82, nx * ny)
+    dev = {k: torch.tensor(a, device="cuda") for k, a in dict(u=u, v=v, p=p, du=du, dv=dv, pp=pp).items()}
+    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, WEST_OUTLET)
+    sysm.apply_corrections_device(dev["pp"], dev["du"], dev["dv"], dev["p"], dev["u"], dev["v"], relax)
+    uo, vo, po = u.copy(), v.copy(), p.copy()
+    O.simple_corrections(nx, ny, dx, dy, relax, pp, du, dv, po, uo, vo)
+    for k, o in (("u", uo), ("v", vo), ("p", po)):
+        assert np.array_equal(_bits(dev[k].cpu().numpy()), _bits(o)), k
+
+
+def test_simple_iterations_device_equal_host():
+    # three SIMPLE pressure steps: refresh, AMG-preconditioned BiCGSTAB to
+    # 1e-10 (pressure.rs:452-485 solves the same system with BiCGSTAB),
+    # corrections.  The device path never copies the matrix; the host path
+    # assembles on the CPU and recomputes the AMG there.  Same bits.
+    torch = _torch()
+    nx, ny = 96, 72
+    dx, dy, relax = 1 / 95, 1 / 71, 0.3
+    n = nx * ny
+    u, v, p, du, dv, rho = _fields(nx, ny, 90)
+    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, WEST_OUTLET, fields=(u, v, p, du, dv, rho))
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 200
+    amg_d = ca.AlgebraicMultigrid(sysm.matrix, cfg)
+    rp, ci, v0, _, _ = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, p, du, dv, rho)
+    a_h = ca.CsrMatrix.from_parts(v0, ci, rp, n, n)
+    amg_h = ca.AlgebraicMultigrid(a_h, cfg)
+    scfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(2000)
+    dev = {k: torch.tensor(a, device="cuda") for k, a in dict(u=u, v=v, p=p, du=du, dv=dv, rho=rho).items()}
+    hu, hv, hp = u.copy(), v.copy(), p.copy()
+    rhs = torch.empty(n, dtype=torch.float64, device="cuda")
+    for it in range(3):
+        mr = sysm.assemble_device(dev["u"], dev["v"], dev["p"], dev["du"], dev["dv"], dev["rho"], rhs)
+        amg_d.recompute_device(sysm.matrix)
+        pp = torch.zeros(n, dtype=torch.float64, device="cuda")
+        rep_d = ca.bicgstab_preconditioned(sysm.matrix, rhs, amg_d, pp, scfg)
+        sysm.apply_corrections_device(pp, dev["du"], dev["dv"], dev["p"], dev["u"], dev["v"], relax)
+
+        _, _, vh, bh, mrh = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, hu, hv, hp, du, dv, rho)
+        a_h.update_values(vh)
+        amg_h.recompute(a_h)
+        pph = np.zeros(n)
+        rep_h = ca.bicgstab_preconditioned(a_h, bh, amg_h, pph, scfg)
+        O.simple_corrections(nx, ny, dx, dy, relax, pph, du, dv, hp, hu, hv)
+
+        assert mr == mrh
+        assert rep_d.termination == ca.Termination.Converged and rep_h.termination == ca.Termination.Converged
+        assert rep_d.iterations == rep_h.iterations
+        assert np.array_equal(_bits(pp.cpu().numpy()), _bits(pph)), it
+        for k, o in (("u", hu), ("v", hv), ("p", hp)):
+            assert np.array_equal(_bits(dev[k].cpu().numpy()), _bits(o)), (it, k)

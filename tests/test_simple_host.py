@@ -1,0 +1,175 @@
+"""SIMPLE pressure-correction system: oracle pinning and host assembly.
+
+The oracle restates crates/cfd-2d/src/solvers/simple/pressure.rs:18-240 and
+:487-520.  It is pinned by the reference's own known answers (the tests at
+pressure.rs:533-637) and by closed forms of the interior stencil; the
+product's host generator (cfdla_problem_simple_pressure) must then equal the
+oracle bit for bit.
+"""
+import numpy as np
+import pytest
+
+import cfdrs_amd as ca
+from cfdrs_amd import problems as P
+from cfdrs_amd import simple as S
+from oracle import oracle as O
+
+WEST_OUTLET = (S.SIDE_PRESSURE, S.SIDE_PLAIN, S.SIDE_PLAIN, S.SIDE_PLAIN)  # outlet west, wall east, outflow N/S
+NO_ANCHOR = (S.SIDE_PLAIN,) * 4
+
+
+def _value(a, r, c):
+    rp, ci, v = a.arrays()
+    for k in range(rp[r], rp[r + 1]):
+        if ci[k] == c:
+            return v[k]
+    return 0.0
+
+
+def _fields(nx, ny, seed=0):
+    rng = np.random.default_rng(seed)
+    n = nx * ny
+    return ([rng.uniform(-1, 1, n) for _ in range(3)] + [rng.uniform(0.5, 1.5, n) for _ in range(2)]
+            + [rng.uniform(0.8, 1.2, n)])
+
+
+def test_reference_rhs_known_answer():
+    # pressure.rs:533-572: u = i dx, v = j dy on 3x3; max residual > 0, the
+    # centre has an imbalance, the anchored corner none
+    nx = ny = 3
+    dx = dy = 0.5
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    u, v = (i * dx).ravel().astype(float), (j * dy).ravel().astype(float)
+    z, one = np.zeros(9), np.ones(9)
+    _, rhs, mr = O.assemble_simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, z, z, z, one)
+    assert mr > 0.0
+    assert abs(rhs[4]) > 0.0
+    assert abs(rhs[0]) < 1e-12
+    # with d = 0 the faces are linear interpolations: imbalance = rho (du/dx + dv/dy) dx dy
+    assert rhs[4] == pytest.approx(2 * dx * dy, rel=1e-15)
+
+
+def test_reference_boundary_rows_known_answer():
+    # pressure.rs:574-600: west outlet anchors (0,0); east wall row (5) = p5 - p4
+    z, one = np.zeros(9), np.ones(9)
+    a, _, _ = O.assemble_simple_pressure(3, 3, 0.5, 0.5, WEST_OUTLET, z, z, z, z, z, one)
+    assert _value(a, 0, 0) == 1.0
+    assert _value(a, 5, 5) == 1.0
+    assert _value(a, 5, 4) == -1.0
+
+
+def test_reference_anchor_fallback_known_answer():
+    # pressure.rs:619-637: no pressure side -> node 0 pinned; corner 6 (west
+    # wall, north outflow) averages nodes 7 and 3
+    z, one = np.zeros(9), np.ones(9)
+    a, _, _ = O.assemble_simple_pressure(3, 3, 0.5, 0.5, NO_ANCHOR, z, z, z, z, z, one)
+    assert _value(a, 0, 0) == 1.0
+    assert _value(a, 6, 6) == 1.0
+    assert _value(a, 6, 7) == -0.5
+    assert _value(a, 6, 3) == -0.5
+
+
+def test_interior_stencil_closed_form_zero_pressure():
+    # p = 0, constant d: a_E = rho d dy/dx, a_N = rho d dx/dy, a_P = -sum;
+    # imbalance = rho dy (u_E - u_W)/2 + rho dx (v_N - v_S)/2
+    nx, ny, dx, dy, d, rho = 8, 7, 0.1, 0.25, 0.3, 1.7
+    rng = np.random.default_rng(1)
+    u, v = rng.uniform(-1, 1, nx * ny), rng.uniform(-1, 1, nx * ny)
+    a, rhs, _ = O.assemble_simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, np.zeros(nx * ny),
+                                           np.full(nx * ny, d), np.full(nx * ny, d), np.full(nx * ny, rho))
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            c = j * nx + i
+            ae, an = rho * d * dy / dx, rho * d * dx / dy
+            assert _value(a, c, c + 1) == pytest.approx(ae, rel=1e-14)
+            assert _value(a, c, c - nx) == pytest.approx(an, rel=1e-14)
+            assert _value(a, c, c) == pytest.approx(-2 * (ae + an), rel=1e-14)
+            want = rho * dy * (u[c + 1] - u[c - 1]) / 2 + rho * dx * (v[c + nx] - v[c - nx]) / 2
+            assert rhs[c] == pytest.approx(want, rel=1e-12, abs=1e-14)
+
+
+def test_rhie_chow_vanishes_for_linear_pressure():
+    # the averaged cell gradient equals the face gradient for linear p (also
+    # across the one-sided extrapolation at the boundary): the imbalance is
+    # the p = 0 one
+    nx, ny, dx, dy = 9, 8, 0.2, 0.15
+    u, v, _, du, dv, rho = _fields(nx, ny, 2)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    p = (3.0 * i * dx - 2.0 * j * dy).ravel()
+    _, r0, _ = O.assemble_simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, np.zeros(nx * ny), du, dv, rho)
+    _, r1, _ = O.assemble_simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, p, du, dv, rho)
+    assert np.allclose(r1, r0, rtol=0, atol=1e-12)
+
+
+def test_rhie_chow_cubic_pressure_closed_form():
+    # p = x^3, u = v = 0: each face velocity is d_face * 1.5 dx^2 (averaged
+    # cell gradient minus face gradient), so the imbalance at nodes two away
+    # from the x boundary is rho dy 1.5 dx^2 (d_e - d_w) with d_e - d_w =
+    # (d_{i+1} - d_{i-1}) / 2.  A dropped or sign-flipped correction fails it.
+    nx, ny, dx, dy = 10, 5, 0.1, 0.2
+    _, _, _, du, dv, rho = _fields(nx, ny, 3)
+    x = (np.arange(nx) * dx)[None, :].repeat(ny, 0).ravel()
+    z = np.zeros(nx * ny)
+    _, rhs, _ = O.assemble_simple_pressure(nx, ny, dx, dy, WEST_OUTLET, z, z, x ** 3, du, dv, rho)
+    for j in range(1, ny - 1):
+        for i in range(2, nx - 2):
+            c = j * nx + i
+            want = rho[c] * dy * 1.5 * dx * dx * (du[c + 1] - du[c - 1]) / 2
+            assert rhs[c] == pytest.approx(want, rel=1e-6, abs=1e-15)
+
+
+def test_corrections_closed_form():
+    # p' = g x: dp'/dx = g, dp'/dy = 0 at interior nodes
+    nx, ny, dx, dy, g, relax = 6, 5, 0.1, 0.2, 2.5, 0.3
+    _, _, _, du, dv, _ = _fields(nx, ny, 4)
+    x = (np.arange(nx) * dx)[None, :].repeat(ny, 0).ravel()
+    pp = g * x
+    p, u, v = np.ones(nx * ny), np.full(nx * ny, 0.5), np.full(nx * ny, -0.5)
+    O.simple_corrections(nx, ny, dx, dy, relax, pp, du, dv, p, u, v)
+    inner = np.zeros((ny, nx), bool)
+    inner[1:-1, 1:-1] = True
+    inner = inner.ravel()
+    assert np.allclose(p[inner], 1.0 + relax * pp[inner], rtol=1e-15)
+    assert np.allclose(u[inner], 0.5 - du[inner] * g * dx, rtol=1e-13)
+    assert np.array_equal(v[inner], np.full(inner.sum(), -0.5))
+    assert np.array_equal(p[~inner], np.ones((~inner).sum()))
+
+
+@pytest.mark.parametrize("side", [WEST_OUTLET, NO_ANCHOR, (2, 2, 0, 1), (0, 1, 2, 2), (2, 2, 2, 2), (1, 1, 1, 1)])
+def test_host_assembly_matches_oracle_bit_for_bit(side):
+    nx, ny, dx, dy = 11, 9, 0.1, 0.13
+    f = _fields(nx, ny, 5)
+    rp, ci, v, rhs, mr = P.simple_pressure(nx, ny, dx, dy, side, *f)
+    a, orhs, omr = O.assemble_simple_pressure(nx, ny, dx, dy, side, *f)
+    orp, oci, ov = a.arrays()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert np.array_equal(v.view(np.uint64), ov.view(np.uint64))
+    assert np.array_equal(rhs.view(np.uint64), orhs.view(np.uint64))
+    assert mr == omr
+
+
+def test_periodic_rows_couple_to_partner():
+    # boundary.rs:52-66: a periodic west node averages across to the east
+    # boundary node of its row
+    nx, ny = 5, 4
+    z, one = np.zeros(nx * ny), np.ones(nx * ny)
+    a, _, _ = O.assemble_simple_pressure(nx, ny, 0.1, 0.1, (2, 2, 1, 1), z, z, z, z, z, one)
+    r = 2 * nx  # (0, 2)
+    assert _value(a, r, r + nx - 1) == -1.0 and _value(a, r, r) == 1.0
+    rp, ci, v, _, _ = P.simple_pressure(nx, ny, 0.1, 0.1, (2, 2, 1, 1), z, z, z, z, z, one)
+    assert list(ci[rp[r]:rp[r + 1]]) == [r, r + nx - 1]
+
+
+def test_invalid_sides_rejected():
+    z = np.zeros(9)
+    with pytest.raises(ca.CfdlaError) as e:
+        P.simple_pressure(3, 3, 1.0, 1.0, (0, 0, 0, 3), z, z, z, z, z, z)
+    assert e.value.status == ca._native.ERR_CONFIG
+    with pytest.raises(ca.CfdlaError):
+        P.simple_pressure(3, 3, 1.0, 1.0, (2, 0, 0, 0), z, z, z, z, z, z)  # unpaired periodic
+    with pytest.raises(ca.CfdlaError):
+        P.simple_pressure(2, 3, 1.0, 1.0, (0, 0, 0, 0), z[:6], z[:6], z[:6], z[:6], z[:6], z[:6])
+    with pytest.raises(ca.CfdlaError):
+        S.sides({"west": "pressure_outlet", "north": "outflow", "south": "outflow"})  # pressure.rs:603-617
+    assert S.sides({"west": "pressure_outlet", "east": "wall", "north": "outflow", "south": "outflow"}) == \
+        WEST_OUTLET
