@@ -164,6 +164,12 @@ def main() -> None:
 
     copy = copy_calibration(args.copy_calibration)
 
+    # AMG recompute with new fine values (same values here), on the device
+    # against the host Galerkin products plus upload
+    recompute = {"device_ms": round(_time_solves(lambda: amg.recompute_device(a), 3)[0], 3),
+                 "host_ms": round(_time_solves(lambda: amg.recompute(a), 1, warmup=0)[0], 1)}
+    log(f"AMG recompute {nn}^3: {recompute}")
+
     others = None if args.no_configs else secondary_configs(args)
 
     cpu = None
@@ -200,6 +206,7 @@ def main() -> None:
         "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
         "spmv_fp32_us": round(spmv[4]["us"], 3),
         "copy_calibration": copy,
+        "amg_recompute": recompute,
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv[8]["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": SPMV_FP64_TRAFFIC_BYTES.get(spmv["format"]) if nn == 128 else None,
@@ -415,6 +422,69 @@ def secondary_configs(args) -> dict:
     netres["edges"] = int(nets.edge_i.size)
     out["networks_4096"] = netres
     log(f"networks: {netres}")
+
+    # SIMPLE pressure-correction step on a 512^2 channel grid (walls south
+    # and north, inflow west, pressure outlet east; solvers/simple/pressure.rs):
+    # matrix and right-hand side refreshed from the fields, AMG hierarchy
+    # recomputed, BiCGSTAB-AMG to 1e-10, corrections -- all on the device.
+    # The host path of the same refresh (CPU assembly, values over PCIe, host
+    # Galerkin recompute) is timed beside it; its solve would be the same.
+    from cfdrs_amd import simple as S
+
+    nx = ny = 512
+    ns = nx * ny
+    hs = 1.0 / (nx - 1)
+    fields = [P.uniform(301, ns), P.uniform(302, ns), P.uniform(303, ns), 1e-2 * P.uniform(304, ns, 0.5, 1.5),
+              1e-2 * P.uniform(305, ns, 0.5, 1.5), np.ones(ns)]
+    side = (S.SIDE_PLAIN, S.SIDE_PRESSURE, S.SIDE_PLAIN, S.SIDE_PLAIN)
+    sysm = S.SimplePressureCorrection(nx, ny, hs, hs, side, fields=fields)
+    amg_s = ca.AlgebraicMultigrid(sysm.matrix, ca.AMGConfig.smoothed_aggregation())
+    fd = [torch.from_numpy(f).to(dev) for f in fields]
+    pc, uc, vc = fd[2].clone(), fd[0].clone(), fd[1].clone()  # corrected copies keep the system stationary
+    rhs_s = torch.empty(ns, dtype=torch.float64, device=dev)
+    pp = torch.zeros(ns, dtype=torch.float64, device=dev)
+    scfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(2000)
+
+    def s_refresh():
+        sysm.assemble_device(*fd, rhs_s, want_max=False)
+
+    def s_recompute():
+        amg_s.recompute_device(sysm.matrix)
+
+    def s_solve():
+        pp.zero_()
+        return ca.bicgstab_preconditioned(sysm.matrix, rhs_s, amg_s, pp, scfg)
+
+    def s_correct():
+        sysm.apply_corrections_device(pp, fd[3], fd[4], pc, uc, vc, 0.3)
+
+    def s_step():
+        s_refresh()
+        s_recompute()
+        rep = s_solve()
+        s_correct()
+        return rep
+
+    ms_step, rep_s = _time_solves(s_step, steps)
+    simple = {"grid": f"{nx}x{ny}", "unknowns": ns, "amg_levels": amg_s.statistics.level_sizes,
+              "step_ms": round(ms_step, 3), "iterations": rep_s.iterations,
+              "termination": rep_s.termination.name}
+    for name, fn in (("refresh_ms", s_refresh), ("amg_recompute_device_ms", s_recompute), ("solve_ms", s_solve),
+                     ("corrections_ms", s_correct)):
+        simple[name] = round(_time_solves(fn, steps)[0], 3)
+    a_h = ca.CsrMatrix.from_parts(sysm.matrix.values, sysm.matrix.col_indices, sysm.matrix.row_ptr, ns, ns)
+    amg_h = ca.AlgebraicMultigrid(a_h, ca.AMGConfig.smoothed_aggregation())
+    amg_h.apply_to(np.zeros(ns), np.empty(ns))  # hierarchy on the device, as in the device path
+
+    def h_assemble():
+        return P.simple_pressure(nx, ny, hs, hs, side, *fields)[2]
+
+    vals = h_assemble()
+    simple["host_assembly_ms"] = round(_time_solves(h_assemble, 1, warmup=0)[0], 3)
+    simple["host_upload_ms"] = round(_time_solves(lambda: a_h.update_values(vals), 1, warmup=0)[0], 3)
+    simple["host_amg_recompute_ms"] = round(_time_solves(lambda: amg_h.recompute(a_h), 1, warmup=0)[0], 3)
+    out["simple_pressure_step_512x512_channel"] = simple
+    log(f"SIMPLE pressure step: {simple}")
 
     # adjoint gradient of the outlet flux, 256^2 GRF permeability
     prob = ADJ.FluxProblem(256, 256)

@@ -13,6 +13,7 @@ from .linear_solver import (IterativeSolverConfig, KrylovWorkspace, Precision, S
 from .multigrid import (AlgebraicMultigrid, AMGConfig, AMGStatistics, CoarseningStrategy, CoarseSolver,
                         InterpolationStrategy, JacobiPreconditioner, SmootherType, SuccessiveOverRelaxation)
 from .sparse import CsrMatrix, SparseMatrixBuilder
+from . import simple
 
 __all__ = [
     "AlgebraicMultigrid", "AMGConfig", "AMGStatistics", "CfdlaError", "CoarseningStrategy", "CoarseSolver",

@@ -127,6 +127,47 @@ HostCsr csr_multiply(const HostCsr &a, const HostCsr &b) {
   return c;
 }
 
+ProductPlan product_plan(const HostCsr &a, const HostCsr &b, const HostCsr &c) {
+  CFDLA_REQUIRE(a.n == c.n && a.m == b.n && b.m == c.m, CFDLA_ERR_DIMENSION, "product plan: shapes differ");
+  const int64_t nslots = c.nnz();
+  std::vector<int64_t> ptr((size_t)nslots + 1, 0);
+  // pass 1: products per slot (the slots of a row belong to one worker)
+  parallel_rows(a.n, [&](int64_t lo, int64_t hi) {
+    std::vector<int64_t> slot((size_t)b.m, -1);
+    for (int64_t i = lo; i < hi; ++i) {
+      for (int64_t k = c.rp[i]; k < c.rp[i + 1]; ++k) slot[(size_t)c.ci[k]] = k;
+      for (int64_t ka = a.rp[i]; ka < a.rp[i + 1]; ++ka) {
+        const int32_t r = a.ci[ka];
+        for (int64_t kb = b.rp[r]; kb < b.rp[r + 1]; ++kb) ptr[(size_t)slot[(size_t)b.ci[kb]] + 1]++;
+      }
+    }
+  });
+  for (int64_t k = 0; k < nslots; ++k) ptr[(size_t)k + 1] += ptr[(size_t)k];
+  CFDLA_REQUIRE(ptr.back() < (int64_t)INT32_MAX && a.nnz() < (int64_t)INT32_MAX && b.nnz() < (int64_t)INT32_MAX,
+                CFDLA_ERR_INVALID_ARG, "product plan exceeds 32-bit indices");
+  ProductPlan p;
+  p.pa.resize((size_t)ptr.back());
+  p.pb.resize((size_t)ptr.back());
+  // pass 2: the pairs, per slot in the order the row traversal meets them
+  std::vector<int64_t> next(ptr.begin(), ptr.end() - 1);
+  parallel_rows(a.n, [&](int64_t lo, int64_t hi) {
+    std::vector<int64_t> slot((size_t)b.m, -1);
+    for (int64_t i = lo; i < hi; ++i) {
+      for (int64_t k = c.rp[i]; k < c.rp[i + 1]; ++k) slot[(size_t)c.ci[k]] = k;
+      for (int64_t ka = a.rp[i]; ka < a.rp[i + 1]; ++ka) {
+        const int32_t r = a.ci[ka];
+        for (int64_t kb = b.rp[r]; kb < b.rp[r + 1]; ++kb) {
+          const int64_t q = next[(size_t)slot[(size_t)b.ci[kb]]]++;
+          p.pa[(size_t)q] = (int32_t)ka;
+          p.pb[(size_t)q] = (int32_t)kb;
+        }
+      }
+    }
+  });
+  p.ptr.assign(ptr.begin(), ptr.end());
+  return p;
+}
+
 // Strong connections: |a_ij| >= theta * max_{k != i} |a_ik| (diagonal excluded).
 HostCsr strong_connections(const HostCsr &a, double theta) {
   HostCsr s;

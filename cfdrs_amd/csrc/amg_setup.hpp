@@ -22,6 +22,14 @@ struct HostHierarchy {
   std::vector<HostLevel> levels;
 };
 
+// Numeric plan of C = A B on C's fixed pattern: slot k of C sums
+// A.v[pa[q]] * B.v[pb[q]] for q in [ptr[k], ptr[k + 1]) in that order, the
+// order in which csr_multiply's row accumulator meets the products.
+struct ProductPlan {
+  std::vector<int32_t> ptr, pa, pb;
+};
+ProductPlan product_plan(const HostCsr &a, const HostCsr &b, const HostCsr &c);
+
 HostCsr csr_transpose(const HostCsr &a);
 HostCsr csr_multiply(const HostCsr &a, const HostCsr &b);
 HostCsr strong_connections(const HostCsr &a, double theta);

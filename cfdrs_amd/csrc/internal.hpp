@@ -269,9 +269,10 @@ void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, const double *
 double simple_max_read();
 void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
                         const double *du, const double *dv, double *p, double *u, double *v);
-// cv = A B on C's fixed sorted pattern, the host product's summation order.
-void spgemm_numeric(int nrows, const int32_t *arp, const int32_t *aci, const double *av, const int32_t *brp,
-                    const int32_t *bci, const double *bv, const int32_t *crp, const int32_t *cci, double *cv);
+// cv[k] = sum over q in [ptr[k], ptr[k+1]) of av[pa[q]] * bv[pb[q]], in q
+// order from -0.0: a sparse product replayed from its per-slot plan.
+void product_numeric(int nslots, const int32_t *ptr, const int32_t *pa, const int32_t *pb, const double *av,
+                     const double *bv, double *cv);
 // n x n inverse of a CSR matrix by Gauss-Jordan with partial pivoting, the
 // host's operation order (throws CFDLA_ERR_NUMERICAL when singular).
 void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const double *v, double *inv);

@@ -7,16 +7,18 @@
 // * The SIMPLE pressure-correction rows and right-hand side
 //   (crates/cfd-2d/src/solvers/simple/pressure.rs:244-450) and the velocity /
 //   pressure corrections (:487-520).
-// * The AMG Galerkin refresh (amg.rs:179-214): numeric sparse products on
-//   fixed patterns and the dense coarsest inverse.
+// * The AMG Galerkin refresh (amg.rs:179-214): numeric sparse products
+//   replayed from per-slot plans and the dense coarsest inverse.
 //
 // Everything that replays a host computation does the host's floating-point
 // operations in the host's order with contraction off, so results are equal
-// bit for bit: a product C = A B accumulates each row's products in the
-// traversal order of the host's sparse accumulator, starting from -0.0 (the
+// bit for bit: each slot of a product C = A B sums its products in the
+// order the host's row accumulator meets them, starting from -0.0 (the
 // additive identity of IEEE round-to-nearest: -0.0 + x == x for every x,
 // signed zeros included), which equals the host's "first product assigned,
 // later ones added".
+#include <climits>
+
 #include "internal.hpp"
 #include "simple_stencil.hpp"
 #include "solver.hpp"
@@ -166,36 +168,34 @@ __global__ __launch_bounds__(kBlock) void k_simple_corrections(int64_t nx, int64
 
 // ---- Galerkin refresh ------------------------------------------------------
 
-// C = A B on C's fixed pattern (sorted rows), one lane per row: products in
-// the order (entry of A's row) x (entry of B's row), each added to C's slot
-// of its column (binary search in the row).
-__global__ __launch_bounds__(kBlock) void k_spgemm_numeric(int nrows, const int32_t *__restrict__ arp,
-                                                           const int32_t *__restrict__ aci,
-                                                           const double *__restrict__ av,
-                                                           const int32_t *__restrict__ brp,
-                                                           const int32_t *__restrict__ bci,
-                                                           const double *__restrict__ bv,
-                                                           const int32_t *__restrict__ crp,
-                                                           const int32_t *__restrict__ cci, double *cv) {
+// Each block owns kBlock consecutive slots of C, whose products are one
+// contiguous range of the plan.  The range is staged through LDS in tiles:
+// the block forms the rounded products with coalesced plan loads, then every
+// lane adds the products of its own slot, in plan order (the order of the
+// host's row accumulator), to a sum started at -0.0.
+constexpr int kProdTile = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_product_numeric(int nslots, const int32_t *__restrict__ ptr,
+                                                            const int32_t *__restrict__ pa,
+                                                            const int32_t *__restrict__ pb,
+                                                            const double *__restrict__ av,
+                                                            const double *__restrict__ bv, double *cv) {
 #pragma clang fp contract(off)
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrows; i += gridDim.x * kBlock) {
-    const int c0 = crp[i], c1 = crp[i + 1];
-    for (int k = c0; k < c1; ++k) cv[k] = -0.0;
-    for (int ka = arp[i]; ka < arp[i + 1]; ++ka) {
-      const int r = aci[ka];
-      const double a = av[ka];
-      for (int kb = brp[r]; kb < brp[r + 1]; ++kb) {
-        const int col = bci[kb];
-        int lo = c0, hi = c1;  // first slot with cci >= col; present by construction
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (cci[mid] < col) lo = mid + 1;
-          else hi = mid;
-        }
-        const double prod = a * bv[kb];
-        cv[lo] = cv[lo] + prod;
-      }
+  __shared__ double prod[kProdTile];
+  for (int s0 = blockIdx.x * kBlock; s0 < nslots; s0 += gridDim.x * kBlock) {
+    const int s = s0 + (int)threadIdx.x;
+    const int q_begin = ptr[s0], q_end = ptr[min(s0 + kBlock, nslots)];
+    const int my_b = s < nslots ? ptr[s] : q_end, my_e = s < nslots ? ptr[s + 1] : q_end;
+    double sum = -0.0;
+    for (int t0 = q_begin; t0 < q_end; t0 += kProdTile) {
+      const int t1 = min(q_end, t0 + kProdTile);
+      for (int q = t0 + (int)threadIdx.x; q < t1; q += kBlock) prod[q - t0] = av[pa[q]] * bv[pb[q]];
+      __syncthreads();
+      const int lo = max(my_b, t0), hi = min(my_e, t1);
+      for (int q = lo; q < hi; ++q) sum = sum + prod[q - t0];
+      __syncthreads();
     }
+    if (s < nslots) cv[s] = sum;
   }
 }
 
@@ -210,37 +210,60 @@ __global__ __launch_bounds__(kBlock) void k_dense_fill(int n, double *m) {
   }
 }
 
+// One block per row; the columns of a CSR row are distinct.
 __global__ __launch_bounds__(kBlock) void k_dense_scatter(int n, const int32_t *__restrict__ rp,
                                                           const int32_t *__restrict__ ci,
                                                           const double *__restrict__ v, double *m) {
-  for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock)
-    for (int k = rp[r]; k < rp[r + 1]; ++k) m[(int64_t)r * 2 * n + ci[k]] = m[(int64_t)r * 2 * n + ci[k]] + v[k];
+  for (int r = blockIdx.x; r < n; r += gridDim.x)
+    for (int k = rp[r] + (int)threadIdx.x; k < rp[r + 1]; k += kBlock)
+      m[(int64_t)r * 2 * n + ci[k]] = m[(int64_t)r * 2 * n + ci[k]] + v[k];
 }
 
-// Partial-pivot search, restated for a block of rows: the host scans rows
-// c, c+1, ... replacing the best only when |m_rc| is strictly larger, so
-// row c is the initial candidate (even when it is NaN) and later NaNs never
-// win.  A block not holding row c starts empty (-1); merging partials in
-// block order with the same strict rule gives the host's row.
+// Partial pivoting as the host does it: scanning rows c, c+1, ... and
+// replacing the best only on a strictly larger |m_rc| picks the lowest row of
+// the largest value, and a NaN never wins -- except at row c, the initial
+// best, where it makes the column singular.  On the device, row c's NaN is
+// tested directly and the other candidates are reduced with "larger value,
+// then lower row" (NaN entered as -1), an associative and commutative merge,
+// so any reduction order finds the host's row.
 struct Piv {
   double val;
   int row;
 };
 
-__device__ inline Piv piv_merge(Piv a, Piv b) { return b.val > a.val ? b : a; }
+__device__ inline Piv piv_better(Piv a, Piv b) {
+  return (b.val > a.val || (b.val == a.val && b.row < a.row)) ? b : a;
+}
+
+__device__ inline Piv piv_of(double x, int row) { return Piv{x == x ? fabs(x) : -1.0, row}; }
 
 constexpr int kGjRows = 4;  // rows per block of the elimination step
+
+// Block-wide merge of one candidate per thread; the result is valid in
+// thread 0.
+__device__ Piv piv_block(Piv mine) {
+  __shared__ double sv[kBlock];
+  __shared__ int sr[kBlock];
+  sv[threadIdx.x] = mine.val;
+  sr[threadIdx.x] = mine.row;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const Piv o = piv_better(Piv{sv[threadIdx.x], sr[threadIdx.x]}, Piv{sv[threadIdx.x + w], sr[threadIdx.x + w]});
+      sv[threadIdx.x] = o.val;
+      sr[threadIdx.x] = o.row;
+    }
+    __syncthreads();
+  }
+  return Piv{sv[0], sr[0]};
+}
 
 // Column 0's partials of the filled matrix.
 __global__ __launch_bounds__(kBlock) void k_gj_init(int n, const double *m, double *pval, int *prow) {
   const int r0 = blockIdx.x * kGjRows;
   if (threadIdx.x == 0) {
-    Piv acc{-1.0, -1};
-    for (int r = r0; r < min(n, r0 + kGjRows); ++r) {
-      const double a = fabs(m[(int64_t)r * 2 * n]);
-      if (r == 0) acc = Piv{a, 0};
-      else acc = piv_merge(acc, Piv{a, r});
-    }
+    Piv acc{-1.0, INT_MAX};
+    for (int r = r0; r < min(n, r0 + kGjRows); ++r) acc = piv_better(acc, piv_of(m[(int64_t)r * 2 * n], r));
     pval[blockIdx.x] = acc.val;
     prow[blockIdx.x] = acc.row;
   }
@@ -249,18 +272,21 @@ __global__ __launch_bounds__(kBlock) void k_gj_init(int n, const double *m, doub
 // One Gauss-Jordan column step on [A | I], ping-pong src -> dst: pivot row p
 // from the partials, rows c and p swapped, row c scaled by d = 1 / m_pc,
 // every other row r with f = m_rc != 0 updated m_rj - f (m_pj d).  Writes
-// the next column's pivot partials.  A singular column (best pivot 0 or not
-// finite) sets *bad; every later step then returns at once.
+// the next column's pivot partials.  A singular column (NaN at row c, best
+// pivot 0 or not finite) sets *bad; every later step then returns at once.
 __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *src, double *dst,
                                                     const double *pin, const int *rin, double *pout, int *rout,
                                                     int nparts, int *bad) {
 #pragma clang fp contract(off)
   __shared__ int s_p;
   __shared__ double s_next[kGjRows];
+  const int64_t w = 2 * (int64_t)n;
+  Piv mine{-1.0, INT_MAX};
+  for (int b = threadIdx.x; b < nparts; b += kBlock) mine = piv_better(mine, Piv{pin[b], rin[b]});
+  const Piv best = piv_block(mine);
   if (threadIdx.x == 0) {
-    Piv acc{-1.0, -1};
-    for (int b = 0; b < nparts; ++b) acc = piv_merge(acc, Piv{pin[b], rin[b]});
-    s_p = (*bad || !(acc.val > 0.0 && acc.val <= 1.7976931348623157e308)) ? -1 : acc.row;
+    const double mcc = src[(int64_t)c * w + c];
+    s_p = (*bad || mcc != mcc || !(best.val > 0.0 && best.val <= 1.7976931348623157e308)) ? -1 : best.row;
   }
   __syncthreads();
   const int p = s_p;
@@ -268,7 +294,6 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
     if (threadIdx.x == 0) *bad = 1;
     return;
   }
-  const int64_t w = 2 * (int64_t)n;
   const double d = 1.0 / src[(int64_t)p * w + c];
   const int r0 = blockIdx.x * kGjRows;
   for (int q = 0; q < kGjRows; ++q) {
@@ -289,18 +314,16 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
         out = srow[j] - t;
       }
       drow[j] = out;
-      if (j == c + 1) s_next[q] = fabs(out);
+      if (j == c + 1) s_next[q] = out;
     }
   }
   __syncthreads();
   if (threadIdx.x == 0 && c + 1 < n) {
-    Piv acc{-1.0, -1};
+    Piv acc{-1.0, INT_MAX};
     for (int q = 0; q < kGjRows; ++q) {
       const int r = r0 + q;
       if (r >= n) break;
-      if (r < c + 1) continue;
-      if (r == c + 1) acc = Piv{s_next[q], r};
-      else acc = piv_merge(acc, Piv{s_next[q], r});
+      if (r >= c + 1) acc = piv_better(acc, piv_of(s_next[q], r));
     }
     pout[blockIdx.x] = acc.val;
     rout[blockIdx.x] = acc.row;
@@ -366,11 +389,11 @@ void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double rel
                      relax, pp, du, dv, p, u, v);
 }
 
-void spgemm_numeric(int nrows, const int32_t *arp, const int32_t *aci, const double *av, const int32_t *brp,
-                    const int32_t *bci, const double *bv, const int32_t *crp, const int32_t *cci, double *cv) {
-  if (nrows == 0) return;
-  hipLaunchKernelGGL(k_spgemm_numeric, dim3(std::max(1, std::min(kMaxBlocks, (nrows + kBlock - 1) / kBlock))),
-                     dim3(kBlock), 0, stream(), nrows, arp, aci, av, brp, bci, bv, crp, cci, cv);
+void product_numeric(int nslots, const int32_t *ptr, const int32_t *pa, const int32_t *pb, const double *av,
+                     const double *bv, double *cv) {
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_product_numeric, dim3(std::max(1, std::min(kMaxBlocks, (nslots + kBlock - 1) / kBlock))),
+                     dim3(kBlock), 0, stream(), nslots, ptr, pa, pb, av, bv, cv);
 }
 
 void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const double *v, double *inv) {
@@ -386,7 +409,8 @@ void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const dou
   }
   int *bad = flag_reset();
   hipLaunchKernelGGL(k_dense_fill, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
-  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(n)), dim3(kBlock), 0, stream(), n, rp, ci, v, buf[0].p);
+  hipLaunchKernelGGL(k_dense_scatter, dim3(std::min(n, kMaxBlocks)), dim3(kBlock), 0, stream(), n, rp, ci, v,
+                     buf[0].p);
   hipLaunchKernelGGL(k_gj_init, dim3(nb), dim3(kBlock), 0, stream(), n, buf[0].p, pv[0].p, pr[0].p);
   for (int c = 0; c < n; ++c)
     hipLaunchKernelGGL(k_gj_step, dim3(nb), dim3(kBlock), 0, stream(), n, c, buf[c & 1].p, buf[(c + 1) & 1].p,

@@ -618,27 +618,32 @@ void AmgPrecond::update_values_device(const double *fine_values) {
     CFDLA_HIP(hipMemcpyAsync(a0.v64.p, fine_values, (size_t)a0.nnz * sizeof(double), hipMemcpyDeviceToDevice,
                              stream()));
   refresh_level(a0, host.levels[0].A, true);
-  if (ra.size() != L - 1) {
-    // the patterns of the host's intermediate products R_l A_l
-    ra.clear();
-    ra.resize(L - 1);
+  if (gplan.size() != L - 1) {
+    // per-slot product lists of the host's two products per level (host
+    // work once per hierarchy; the intermediate R_l A_l pattern is the
+    // host product's)
+    gplan.clear();
+    gplan.resize(L - 1);
+    auto put = [](DevPlan &d, const ProductPlan &p) {
+      d.nslots = (int)p.ptr.size() - 1;
+      d.ptr.upload(p.ptr.data(), p.ptr.size());
+      d.pa.upload(p.pa.data(), p.pa.size());
+      d.pb.upload(p.pb.data(), p.pb.size());
+      sync_stream();
+    };
     for (size_t l = 0; l + 1 < L; ++l) {
-      const HostCsr t = csr_multiply(host.levels[l].R, host.levels[l].A);
-      std::vector<int32_t> rp32(t.rp.size());
-      for (size_t k = 0; k < t.rp.size(); ++k) rp32[k] = (int32_t)t.rp[k];
-      ra[l].rp.upload(rp32.data(), rp32.size());
-      ra[l].ci.upload(t.ci.data(), t.ci.size());
-      ra[l].v.alloc(t.ci.size());
+      const HostLevel &h = host.levels[l];
+      const HostCsr ra_pat = csr_multiply(h.R, h.A);
+      put(gplan[l].ra, product_plan(h.R, h.A, ra_pat));
+      put(gplan[l].rap, product_plan(ra_pat, h.P, host.levels[l + 1].A));
+      gplan[l].ra_v.alloc((size_t)ra_pat.nnz());
     }
-    sync_stream();
   }
   for (size_t l = 0; l + 1 < L; ++l) {
     DevLevel &f = lv[l], &c = lv[l + 1];
-    const int nc = (int)host.levels[l].R.n;
-    spgemm_numeric(nc, f.R.rp.p, f.R.ci.p, f.R.v64.p, f.A.rp.p, f.A.ci.p, f.A.v64.p, ra[l].rp.p, ra[l].ci.p,
-                   ra[l].v.p);
-    spgemm_numeric(nc, ra[l].rp.p, ra[l].ci.p, ra[l].v.p, f.P.rp.p, f.P.ci.p, f.P.v64.p, c.A.rp.p, c.A.ci.p,
-                   c.A.v64.p);
+    GalerkinPlan &g = gplan[l];
+    product_numeric(g.ra.nslots, g.ra.ptr.p, g.ra.pa.p, g.ra.pb.p, f.R.v64.p, f.A.v64.p, g.ra_v.p);
+    product_numeric(g.rap.nslots, g.rap.ptr.p, g.rap.pa.p, g.rap.pb.p, g.ra_v.p, f.P.v64.p, c.A.v64.p);
     refresh_level(c.A, host.levels[l + 1].A, true);
   }
   if (cfg.coarse_solver == 1) {

@@ -78,10 +78,17 @@ struct DevLevel {
   template <typename T> T *r();
 };
 
-// Device pattern and values of an intermediate product (R_l A_l).
-struct DevPattern {
-  DevBuf<int32_t> rp, ci;
-  DevBuf<double> v;
+// Device copy of a ProductPlan.
+struct DevPlan {
+  int nslots = 0;
+  DevBuf<int32_t> ptr, pa, pb;
+};
+
+// Numeric plans of one level's Galerkin product A_{l+1} = (R_l A_l) P_l and
+// the values of the intermediate R_l A_l.
+struct GalerkinPlan {
+  DevPlan ra, rap;
+  DevBuf<double> ra_v;
 };
 
 struct AmgPrecond : Precond {
@@ -93,7 +100,7 @@ struct AmgPrecond : Precond {
   double setup_time = 0.0;
   bool on_device = false;
   bool host_stale = false;  // a device refresh left newer level values than host
-  std::vector<DevPattern> ra;  // pattern of R_l A_l per level, made on the first device refresh
+  std::vector<GalerkinPlan> gplan;  // per level, made on the first device refresh
   // AlgebraicMultigrid::recompute (amg.rs:179-214) on the device from
   // level-0 values already on the device (CSR order of the fine matrix).
   void update_values_device(const double *fine_values);

@@ -236,3 +236,24 @@ def test_simple_iterations_device_equal_host():
         assert np.array_equal(_bits(pp.cpu().numpy()), _bits(pph)), it
         for k, o in (("u", hu), ("v", hv), ("p", hp)):
             assert np.array_equal(_bits(dev[k].cpu().numpy()), _bits(o)), (it, k)
+
+
+@pytest.mark.parametrize("bad", ["zeros", "nan"])
+def test_amg_recompute_singular_coarse_raises_like_host(bad):
+    # a coarsest operator that is zero, or NaN at its first pivot, is
+    # singular for the host Gauss-Jordan; the device one reports the same
+    torch = _torch()
+    nn = 96
+    n = nn * nn
+    rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(61, n)), 0b10)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 100
+    a_d, a_h, m_d, m_h = _amg_pair(rp, ci, v, n, cfg)
+    v2 = np.zeros_like(v) if bad == "zeros" else np.where(np.arange(v.size) == 0, np.nan, v)
+    a_d.update_values_device(torch.tensor(v2, device="cuda"))
+    a_h.update_values(v2)
+    with pytest.raises(ca.CfdlaError) as eh:
+        m_h.recompute(a_h)
+    with pytest.raises(ca.CfdlaError) as ed:
+        m_d.recompute_device(a_d)
+    assert ed.value.status == eh.value.status == ca._native.ERR_NUMERICAL
