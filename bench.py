@@ -437,7 +437,7 @@ def secondary_configs(args) -> dict:
     fields = [P.uniform(301, ns), P.uniform(302, ns), P.uniform(303, ns), 1e-2 * P.uniform(304, ns, 0.5, 1.5),
               1e-2 * P.uniform(305, ns, 0.5, 1.5), np.ones(ns)]
     side = (S.SIDE_PLAIN, S.SIDE_PRESSURE, S.SIDE_PLAIN, S.SIDE_PLAIN)
-    sysm = S.SimplePressureCorrection(nx, ny, hs, hs, side, fields=fields)
+    sysm = S.SimplePressureCorrection(nx, ny, hs, hs, side, fields=fields, positive_diagonal=True)
     amg_s = ca.AlgebraicMultigrid(sysm.matrix, ca.AMGConfig.smoothed_aggregation())
     fd = [torch.from_numpy(f).to(dev) for f in fields]
     pc, uc, vc = fd[2].clone(), fd[0].clone(), fd[1].clone()  # corrected copies keep the system stationary
@@ -466,18 +466,38 @@ def secondary_configs(args) -> dict:
         return rep
 
     ms_step, rep_s = _time_solves(s_step, steps)
-    simple = {"grid": f"{nx}x{ny}", "unknowns": ns, "amg_levels": amg_s.statistics.level_sizes,
+    simple = {"grid": f"{nx}x{ny}", "unknowns": ns, "form": "positive diagonal (interior rows negated)",
+              "amg_levels": amg_s.statistics.level_sizes,
               "step_ms": round(ms_step, 3), "iterations": rep_s.iterations,
               "termination": rep_s.termination.name}
     for name, fn in (("refresh_ms", s_refresh), ("amg_recompute_device_ms", s_recompute), ("solve_ms", s_solve),
                      ("corrections_ms", s_correct)):
         simple[name] = round(_time_solves(fn, steps)[0], 3)
+    # the reference's sign (a_P = -sum a_nb beside +1 boundary rows): its
+    # own solver, unpreconditioned BiCGSTAB (pressure.rs:452-485), and the
+    # same AMG recipe
+    ref_sys = S.SimplePressureCorrection(nx, ny, hs, hs, side, fields=fields)
+    rhs_r = torch.empty(ns, dtype=torch.float64, device=dev)
+    ref_sys.assemble_device(*fd, rhs_r, want_max=False)
+    amg_r = ca.AlgebraicMultigrid(ref_sys.matrix, ca.AMGConfig.smoothed_aggregation())
+
+    def r_plain():
+        pp.zero_()
+        return ca.bicgstab(ref_sys.matrix, rhs_r, pp, scfg)
+
+    def r_amg():
+        pp.zero_()
+        return ca.bicgstab_preconditioned(ref_sys.matrix, rhs_r, amg_r, pp, scfg)
+
+    for name, fn in (("reference_sign_bicgstab_unpreconditioned", r_plain), ("reference_sign_bicgstab_amg", r_amg)):
+        ms_r, rep_r = _time_solves(fn, 1)
+        simple[name] = {"ms": round(ms_r, 3), "iterations": rep_r.iterations, "termination": rep_r.termination.name}
     a_h = ca.CsrMatrix.from_parts(sysm.matrix.values, sysm.matrix.col_indices, sysm.matrix.row_ptr, ns, ns)
     amg_h = ca.AlgebraicMultigrid(a_h, ca.AMGConfig.smoothed_aggregation())
     amg_h.apply_to(np.zeros(ns), np.empty(ns))  # hierarchy on the device, as in the device path
 
     def h_assemble():
-        return P.simple_pressure(nx, ny, hs, hs, side, *fields)[2]
+        return P.simple_pressure(nx, ny, hs, hs, side, *fields, positive_diagonal=True)[2]
 
     vals = h_assemble()
     simple["host_assembly_ms"] = round(_time_solves(h_assemble, 1, warmup=0)[0], 3)

@@ -114,12 +114,14 @@ SIGNATURES = {
                                            C.c_double, _VP, _VP, _I64P, _VP, _VP, _VP]),
     "cfdla_problem_pressure_rhs": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                              _VP, _VP, C.c_int32, _VP]),
-    "cfdla_problem_simple_pressure": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _VP, _VP, _VP,
-                                                _VP, _VP, _VP, _I64P, _VP, _VP, _VP, _VP, _F64P]),
-    "cfdla_simple_pressure_refresh_device": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _VP,
-                                                       _VP, _VP, _VP, _VP, _VP, _F64P]),
+    "cfdla_problem_simple_pressure": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, C.c_int32, _VP,
+                                                _VP, _VP, _VP, _VP, _VP, _I64P, _VP, _VP, _VP, _VP, _F64P]),
+    "cfdla_simple_pressure_refresh_device": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                                       C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _F64P]),
     "cfdla_simple_corrections_device": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, _VP,
                                                   _VP, _VP, _VP, _VP, _VP]),
+    "cfdla_pressure_rhs_device": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            _VP, _VP, C.c_int32, _VP]),
     "cfdla_problem_pressure_correction": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, _VP, _I64P,
                                                     _VP, _VP, _VP]),
     "cfdla_problem_varcoef": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_double, _VP, C.c_int32, _VP, _I64P,

@@ -281,7 +281,8 @@ cfdla_status cfdla_matrix_get_values(cfdla_matrix m, double *values) {
 }
 
 cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, double dx, double dy,
-                                                  const double *u, const double *v, const double *p,
+                                                  int32_t positive_diagonal, const double *u, const double *v,
+                                                  const double *p,
                                                   const double *d_u, const double *d_v, const double *rho,
                                                   double *rhs_dev, double *max_residual) {
   return guarded([&] {
@@ -306,9 +307,20 @@ cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, in
       m->simple_ny = ny;
     }
     DevCsr &d = devmat(m);
-    simple_refresh(nx, ny, dx, dy, u, v, p, d_u, d_v, rho, d.rp.p, d.v64.p, rhs_dev, max_residual != nullptr);
+    simple_refresh(nx, ny, dx, dy, positive_diagonal != 0, u, v, p, d_u, d_v, rho, d.rp.p, d.v64.p, rhs_dev,
+                   max_residual != nullptr);
     device_values_changed(m);
     if (max_residual) *max_residual = simple_max_read();
+  });
+}
+
+cfdla_status cfdla_pressure_rhs_device(int64_t nx, int64_t ny, double dx, double dy, double rho, double dt,
+                                       const double *u, const double *v, int32_t pin_reference, double *rhs) {
+  return guarded([&] {
+    CFDLA_REQUIRE(nx >= 3 && ny >= 3, CFDLA_ERR_INVALID_ARG, "pressure grid needs at least 3x3 nodes");
+    CFDLA_REQUIRE(u && v && rhs, CFDLA_ERR_INVALID_ARG, "u, v and rhs must not be NULL");
+    ensure_init();
+    pressure_rhs(nx, ny, dx, dy, rho / dt, u, v, pin_reference != 0, rhs);
   });
 }
 

@@ -263,12 +263,15 @@ void scatter_formats(DevCsr &d);
 // rp[idx]) and rhs; with want_max, max |imbalance| is reduced on the device
 // for simple_max_read (which synchronizes).  The row kernel's block maxima
 // sit in the reduction partials until the next reducing launch.
-void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, const double *u, const double *v, const double *p,
-                    const double *du, const double *dv, const double *rho, const int32_t *rp, double *v64,
-                    double *rhs, bool want_max);
+void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, const double *u, const double *v,
+                    const double *p, const double *du, const double *dv, const double *rho, const int32_t *rp,
+                    double *v64, double *rhs, bool want_max);
 double simple_max_read();
 void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
                         const double *du, const double *dv, double *p, double *u, double *v);
+// Cell-centred pressure-correction right-hand side on device arrays.
+void pressure_rhs(int64_t nx, int64_t ny, double dx, double dy, double coeff, const double *u, const double *v,
+                  bool pin, double *rhs);
 // cv[k] = sum over q in [ptr[k], ptr[k+1]) of av[pa[q]] * bv[pb[q]], in q
 // order from -0.0: a sparse product replayed from its per-slot plan.
 void product_numeric(int nslots, const int32_t *ptr, const int32_t *pa, const int32_t *pb, const double *av,

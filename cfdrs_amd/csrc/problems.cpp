@@ -152,10 +152,10 @@ cfdla_status cfdla_problem_pressure_rhs(int64_t nx, int64_t ny, double dx, doubl
 }
 
 cfdla_status cfdla_problem_simple_pressure(int64_t nx, int64_t ny, double dx, double dy, const int32_t *side,
-                                           const double *u, const double *v, const double *p, const double *d_u,
-                                           const double *d_v, const double *rho, int64_t *nnz, int64_t *row_ptr,
-                                           int64_t *col_indices, double *values, double *rhs,
-                                           double *max_residual) {
+                                           int32_t positive_diagonal, const double *u, const double *v,
+                                           const double *p, const double *d_u, const double *d_v, const double *rho,
+                                           int64_t *nnz, int64_t *row_ptr, int64_t *col_indices, double *values,
+                                           double *rhs, double *max_residual) {
   return run([&] {
     CFDLA_REQUIRE(nx >= 3 && ny >= 3, CFDLA_ERR_INVALID_ARG, "SIMPLE pressure grid needs at least 3x3 nodes");
     CFDLA_REQUIRE(side && nnz, CFDLA_ERR_INVALID_ARG, "side and nnz must not be NULL");
@@ -198,9 +198,9 @@ cfdla_status cfdla_problem_simple_pressure(int64_t nx, int64_t ny, double dx, do
           const double a[5] = {s.as, s.aw, s.ap, s.ae, s.an};
           for (int k = 0; k < 5; ++k) {
             col_indices[o] = c[k];
-            values[o++] = a[k];
+            values[o++] = positive_diagonal ? -a[k] : a[k];
           }
-          rhs[idx] = s.imbalance;
+          rhs[idx] = positive_diagonal ? -s.imbalance : s.imbalance;
           // strict comparison: a NaN imbalance never becomes the maximum (pressure.rs:140-143)
           if (std::fabs(s.imbalance) > maxr) maxr = std::fabs(s.imbalance);
         }

@@ -98,7 +98,7 @@ int grid_for(int64_t units) {
 // One lane per node: interior rows (5 slots from rp[idx], columns sorted)
 // and the right-hand side; boundary nodes get rhs 0.  Block maxima of
 // |imbalance| (a NaN never wins the strict comparison) go to partials.
-__global__ __launch_bounds__(kBlock) void k_simple_refresh(int64_t nx, int64_t ny, double dx, double dy,
+__global__ __launch_bounds__(kBlock) void k_simple_refresh(int64_t nx, int64_t ny, double dx, double dy, int negate,
                                                            const double *u, const double *v, const double *p,
                                                            const double *du, const double *dv, const double *rho,
                                                            const int32_t *__restrict__ rp, double *v64, double *rhs,
@@ -113,12 +113,12 @@ __global__ __launch_bounds__(kBlock) void k_simple_refresh(int64_t nx, int64_t n
     }
     const SimpleNode s = simple_node(i, j, nx, ny, dx, dy, u, v, p, du, dv, rho);
     double *row = v64 + rp[idx];
-    row[0] = s.as;
-    row[1] = s.aw;
-    row[2] = s.ap;
-    row[3] = s.ae;
-    row[4] = s.an;
-    rhs[idx] = s.imbalance;
+    row[0] = negate ? -s.as : s.as;
+    row[1] = negate ? -s.aw : s.aw;
+    row[2] = negate ? -s.ap : s.ap;
+    row[3] = negate ? -s.ae : s.ae;
+    row[4] = negate ? -s.an : s.an;
+    rhs[idx] = negate ? -s.imbalance : s.imbalance;
     const double a = fabs(s.imbalance);
     if (a > mx) mx = a;
   }
@@ -163,6 +163,25 @@ __global__ __launch_bounds__(kBlock) void k_simple_corrections(int64_t nx, int64
     const double dpy = (pp[idx + nx] - pp[idx - nx]) / (two * dy);
     u[idx] = u[idx] - du[idx] * dpx * dx;
     v[idx] = v[idx] - dv[idx] * dpy * dy;
+  }
+}
+
+// Pressure-correction right-hand side of the cell-centred system
+// (crates/cfd-2d/src/pressure_velocity/correction.rs:345-356): cell (i, j)
+// of the interior is row (i-1)(ny-2) + (j-1), one less with the first cell
+// pinned; rhs = -(rho/dt) (central du/dx + dv/dy), the host generator's
+// operations in its order.
+__global__ __launch_bounds__(kBlock) void k_pressure_rhs(int64_t nx, int64_t ny, double dx, double dy, double coeff,
+                                                         const double *__restrict__ u, const double *__restrict__ v,
+                                                         int pin, double *rhs) {
+#pragma clang fp contract(off)
+  const int64_t cells = (nx - 2) * (ny - 2);
+  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < cells; idx += (int64_t)gridDim.x * kBlock) {
+    if (pin && idx == 0) continue;
+    const int64_t i = idx / (ny - 2) + 1, j = idx % (ny - 2) + 1;
+    const double div = (u[j * nx + i + 1] - u[j * nx + i - 1]) / (2.0 * dx) +
+                       (v[(j + 1) * nx + i] - v[(j - 1) * nx + i]) / (2.0 * dy);
+    rhs[pin ? idx - 1 : idx] = -coeff * div;
   }
 }
 
@@ -364,13 +383,13 @@ void scatter_formats(DevCsr &d) {
                        d.sdinv64.p, d.sdinv32.p);
 }
 
-void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, const double *u, const double *v, const double *p,
-                    const double *du, const double *dv, const double *rho, const int32_t *rp, double *v64,
-                    double *rhs, bool want_max) {
+void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, const double *u, const double *v,
+                    const double *p, const double *du, const double *dv, const double *rho, const int32_t *rp,
+                    double *v64, double *rhs, bool want_max) {
   const int grid = grid_for(nx * ny);
   RedWs &ws = ctx().ws;
-  hipLaunchKernelGGL(k_simple_refresh, dim3(grid), dim3(kBlock), 0, stream(), nx, ny, dx, dy, u, v, p, du, dv, rho,
-                     rp, v64, rhs, ws.partials.p);
+  hipLaunchKernelGGL(k_simple_refresh, dim3(grid), dim3(kBlock), 0, stream(), nx, ny, dx, dy, negate ? 1 : 0, u, v,
+                     p, du, dv, rho, rp, v64, rhs, ws.partials.p);
   if (!want_max) return;
   g_max.ensure(1);
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, stream(), grid, ws.partials.p, g_max.p);
@@ -387,6 +406,12 @@ void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double rel
                         const double *du, const double *dv, double *p, double *u, double *v) {
   hipLaunchKernelGGL(k_simple_corrections, dim3(grid_for(nx * ny)), dim3(kBlock), 0, stream(), nx, ny, dx, dy,
                      relax, pp, du, dv, p, u, v);
+}
+
+void pressure_rhs(int64_t nx, int64_t ny, double dx, double dy, double coeff, const double *u, const double *v,
+                  bool pin, double *rhs) {
+  hipLaunchKernelGGL(k_pressure_rhs, dim3(grid_for((nx - 2) * (ny - 2))), dim3(kBlock), 0, stream(), nx, ny, dx, dy,
+                     coeff, u, v, pin ? 1 : 0, rhs);
 }
 
 void product_numeric(int nslots, const int32_t *ptr, const int32_t *pa, const int32_t *pb, const double *av,

@@ -46,25 +46,29 @@ def pressure_correction_2d(nxc: int, nyc: int, dx: float, dy: float, dirichlet=(
     return rp, ci, v
 
 
-def simple_pressure(nx: int, ny: int, dx: float, dy: float, side, u, v, p, d_u, d_v, rho):
+def simple_pressure(nx: int, ny: int, dx: float, dy: float, side, u, v, p, d_u, d_v, rho,
+                    positive_diagonal: bool = False):
     """(row_ptr, col, values, rhs, max_residual) of the SIMPLE
     pressure-correction system on an nx x ny node grid
     (crates/cfd-2d/src/solvers/simple/pressure.rs:18-240); side = (west,
-    east, south, north) of 0 plain / 1 pressure / 2 periodic."""
+    east, south, north) of 0 plain / 1 pressure / 2 periodic.
+    positive_diagonal negates the interior rows and their rhs (same
+    solution; the form to precondition with AMG)."""
     sd = np.asarray(side, np.int32)
     f = [np.ascontiguousarray(a, np.float64) for a in (u, v, p, d_u, d_v, rho)]
     for a in f:
         if a.size != nx * ny:
             raise N.CfdlaError(N.ERR_DIMENSION, "fields must have nx * ny entries")
     nnz = C.c_int64()
-    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), *(N.ptr(a) for a in f), C.byref(nnz), None,
-           None, None, None, None)
+    pos = 1 if positive_diagonal else 0
+    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), pos, *(N.ptr(a) for a in f), C.byref(nnz),
+           None, None, None, None, None)
     rp = np.empty(nx * ny + 1, np.int64)
     ci = np.empty(nnz.value, np.int64)
     val = np.empty(nnz.value)
     rhs = np.empty(nx * ny)
     mr = C.c_double()
-    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), *(N.ptr(a) for a in f), C.byref(nnz),
+    N.call("cfdla_problem_simple_pressure", nx, ny, dx, dy, N.ptr(sd), pos, *(N.ptr(a) for a in f), C.byref(nnz),
            N.ptr(rp), N.ptr(ci), N.ptr(val), N.ptr(rhs), C.byref(mr))
     return rp, ci, val, rhs, mr.value
 
@@ -148,6 +152,23 @@ def pressure_rhs(n, dx, dy, rho, dt, u, v, pin=True):
     N.call("cfdla_problem_pressure_rhs", n, n, dx, dy, rho, dt, N.ptr(np.ascontiguousarray(u, np.float64)),
            N.ptr(np.ascontiguousarray(v, np.float64)), 1 if pin else 0, N.ptr(out))
     return out
+
+
+def pressure_rhs_device(nx, ny, dx, dy, rho, dt, u, v, rhs, pin=True) -> None:
+    """pressure_rhs on CUDA float64 tensors (u, v of nx * ny nodes; rhs of
+    the interior cells, one fewer when pinned), bit for bit the host's."""
+    import torch
+
+    cells = (nx - 2) * (ny - 2) - (1 if pin else 0)
+    for t, m in ((u, nx * ny), (v, nx * ny), (rhs, cells)):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                and t.numel() == m):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "pressure_rhs_device takes contiguous CUDA float64 tensors")
+    from .sparse import bind_torch_stream
+
+    bind_torch_stream()
+    N.call("cfdla_pressure_rhs_device", nx, ny, dx, dy, rho, dt, u.data_ptr(), v.data_ptr(), 1 if pin else 0,
+           rhs.data_ptr())
 
 
 def config3(n: int = 512, re: float = 1000.0, dt: float = 0.05):

@@ -43,9 +43,9 @@ def sides(boundaries: dict) -> tuple:
     return tuple(out)
 
 
-def assemble_pressure_correction(nx, ny, dx, dy, side, u, v, p, d_u, d_v, rho):
+def assemble_pressure_correction(nx, ny, dx, dy, side, u, v, p, d_u, d_v, rho, positive_diagonal=False):
     """Host assembly: (CsrMatrix, rhs, max |mass imbalance|)."""
-    rp, ci, val, rhs, mr = simple_pressure(nx, ny, dx, dy, side, u, v, p, d_u, d_v, rho)
+    rp, ci, val, rhs, mr = simple_pressure(nx, ny, dx, dy, side, u, v, p, d_u, d_v, rho, positive_diagonal)
     return CsrMatrix.from_parts(val, ci, rp, nx * ny, nx * ny), rhs, mr
 
 
@@ -59,17 +59,24 @@ def _dev(a, name):
 
 class SimplePressureCorrection:
     """Pressure-correction system of one grid, assembled once on the host
-    (the first-build branch) and refreshed on the device afterwards."""
+    (the first-build branch) and refreshed on the device afterwards.
 
-    def __init__(self, nx: int, ny: int, dx: float, dy: float, side, fields=None):
+    ``positive_diagonal`` stores the interior rows negated (a_P = +sum
+    a_nb, rhs negated; the solution is unchanged).  The reference's sign
+    (a_P = -sum a_nb beside +1 boundary rows) suits its unpreconditioned
+    BiCGSTAB; an AMG hierarchy needs rows of one sign, since its Galerkin
+    product sums the rows of an aggregate."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, side, fields=None, positive_diagonal=False):
         self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
         self.side = tuple(side)
+        self.positive_diagonal = bool(positive_diagonal)
         n = nx * ny
         if fields is None:
             z = np.zeros(n)
             fields = (z, z, z, z, z, np.ones(n))
-        self.matrix, self.rhs_host, self.max_residual = assemble_pressure_correction(nx, ny, dx, dy, self.side,
-                                                                                     *fields)
+        self.matrix, self.rhs_host, self.max_residual = assemble_pressure_correction(
+            nx, ny, dx, dy, self.side, *fields, positive_diagonal=self.positive_diagonal)
 
     def assemble_device(self, u, v, p, d_u, d_v, rho, rhs, want_max: bool = True):
         """Refresh the matrix values and write rhs (CUDA float64 tensors of
@@ -83,7 +90,7 @@ class SimplePressureCorrection:
 
         mr = C.c_double()
         N.call("cfdla_simple_pressure_refresh_device", self.matrix.handle, self.nx, self.ny, self.dx, self.dy,
-               _dev(u, "u"), _dev(v, "v"), _dev(p, "p"), _dev(d_u, "d_u"), _dev(d_v, "d_v"), _dev(rho, "rho"),
+               1 if self.positive_diagonal else 0, _dev(u, "u"), _dev(v, "v"), _dev(p, "p"), _dev(d_u, "d_u"), _dev(d_v, "d_v"), _dev(rho, "rho"),
                _dev(rhs, "rhs"), C.byref(mr) if want_max else None)
         self.matrix._values = None
         return mr.value if want_max else None

@@ -378,6 +378,13 @@ cfdla_status cfdla_problem_momentum_u(int64_t nx, int64_t ny, double dx, double 
 cfdla_status cfdla_problem_pressure_rhs(int64_t nx, int64_t ny, double dx, double dy, double rho, double dt,
                                         const double *u, const double *v, int32_t pin_reference,
                                         double *rhs);
+/* Same right-hand side from device arrays u, v into a device rhs (the
+ * host generator's operations, bit for bit): with the constant-coefficient
+ * matrix of cfdla_problem_pressure_correction kept on the device, a
+ * pressure-correction iteration needs no host assembly. */
+cfdla_status cfdla_pressure_rhs_device(int64_t nx, int64_t ny, double dx, double dy, double rho, double dt,
+                                       const double *u_dev, const double *v_dev, int32_t pin_reference,
+                                       double *rhs_dev);
 /* Rectangular-duct conductance G = 1/R (rectangular.rs:95-157,209-245) and,
  * if dg_dw is non-NULL, dG/dw at fixed h, L, mu. */
 cfdla_status cfdla_rect_conductance(int64_t n, const double *width, const double *height,
@@ -405,25 +412,30 @@ cfdla_status cfdla_synth_grf(int64_t nx, int64_t ny, double corr_len, uint64_t s
  * sides: the partner node); node 0 is an identity row when no side is a
  * pressure side.  side[4] = {west, east, south, north} of CFDLA_SIDE_*.
  * Fields are nx*ny arrays.  rhs is 0 at boundary nodes; *max_residual
- * (nullable) = max |mass imbalance|.  Two-phase: with row_ptr NULL only *nnz
- * is written and the fields may be NULL. */
+ * (nullable) = max |mass imbalance|.  positive_diagonal = 1 negates every
+ * interior row and its rhs entry (exactly; the solution is unchanged), so
+ * all diagonals are positive: the form to precondition with AMG, whose
+ * Galerkin sums over an aggregate must not mix rows of opposite sign.
+ * Two-phase: with row_ptr NULL only *nnz is written and the fields may be
+ * NULL. */
 #define CFDLA_SIDE_PLAIN 0    /* wall, outflow, velocity inlet */
 #define CFDLA_SIDE_PRESSURE 1 /* Dirichlet / pressure inlet / pressure outlet: anchors p' */
 #define CFDLA_SIDE_PERIODIC 2 /* periodic with the opposite side */
 cfdla_status cfdla_problem_simple_pressure(int64_t nx, int64_t ny, double dx, double dy, const int32_t *side,
-                                           const double *u, const double *v, const double *p, const double *d_u,
-                                           const double *d_v, const double *rho, int64_t *nnz, int64_t *row_ptr,
-                                           int64_t *col_indices, double *values, double *rhs,
-                                           double *max_residual);
+                                           int32_t positive_diagonal, const double *u, const double *v,
+                                           const double *p, const double *d_u, const double *d_v, const double *rho,
+                                           int64_t *nnz, int64_t *row_ptr, int64_t *col_indices, double *values,
+                                           double *rhs, double *max_residual);
 /* Per-iteration refresh of that system on the device (the refresh branch,
  * pressure.rs:244-450): m must hold cfdla_problem_simple_pressure's pattern
  * for the same nx, ny; its interior rows are recomputed from device fields
- * (the same operations as the host assembly, bit for bit), boundary rows
- * keep their values, rhs_dev (nx*ny) receives the imbalance and 0 at
- * boundary nodes.  *max_residual (host, nullable; non-NULL synchronizes)
- * receives max |mass imbalance|. */
+ * (the same operations as the host assembly, bit for bit, negated with
+ * positive_diagonal = 1), boundary rows keep their values, rhs_dev (nx*ny)
+ * receives the imbalance and 0 at boundary nodes.  *max_residual (host,
+ * nullable; non-NULL synchronizes) receives max |mass imbalance|. */
 cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, double dx, double dy,
-                                                  const double *u, const double *v, const double *p,
+                                                  int32_t positive_diagonal, const double *u, const double *v,
+                                                  const double *p,
                                                   const double *d_u, const double *d_v, const double *rho,
                                                   double *rhs_dev, double *max_residual);
 /* SimpleAlgorithm::apply_corrections (pressure.rs:487-520) on device arrays,

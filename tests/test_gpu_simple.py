@@ -45,13 +45,16 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float64).view(np.uint64)
 
 
-@pytest.mark.parametrize("nx,ny,side,fmt", [(67, 45, WEST_OUTLET, "csr_rowblock"), (96, 80, WEST_OUTLET, "dia"),
-                                            (96, 80, (0, 0, 0, 0), "dia"), (90, 70, (2, 2, 0, 1), None)],
-                         ids=["csr", "dia", "dia-no-anchor", "periodic"])
-def test_refresh_matches_host_assembly(nx, ny, side, fmt):
+@pytest.mark.parametrize("nx,ny,side,fmt,pos", [(67, 45, WEST_OUTLET, "csr_rowblock", False),
+                                                (96, 80, WEST_OUTLET, "dia", False),
+                                                (96, 80, (0, 0, 0, 0), "dia", False),
+                                                (90, 70, (2, 2, 0, 1), None, False),
+                                                (96, 80, WEST_OUTLET, "dia", True)],
+                         ids=["csr", "dia", "dia-no-anchor", "periodic", "dia-positive-diagonal"])
+def test_refresh_matches_host_assembly(nx, ny, side, fmt, pos):
     torch = _torch()
     dx, dy = 1.0 / (nx - 1), 1.0 / (ny - 1)
-    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, side, fields=_fields(nx, ny, 1))
+    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, side, fields=_fields(nx, ny, 1), positive_diagonal=pos)
     if fmt:
         assert sysm.matrix.storage()["format"] == fmt
     for seed in (2, 3):  # two refreshes in a row
@@ -59,7 +62,7 @@ def test_refresh_matches_host_assembly(nx, ny, side, fmt):
         dev = [torch.tensor(a, device="cuda") for a in f]
         rhs = torch.full((nx * ny,), 7.0, dtype=torch.float64, device="cuda")
         mr = sysm.assemble_device(*dev, rhs)
-        rp, ci, v, hrhs, hmr = P.simple_pressure(nx, ny, dx, dy, side, *f)
+        rp, ci, v, hrhs, hmr = P.simple_pressure(nx, ny, dx, dy, side, *f, positive_diagonal=pos)
         assert np.array_equal(_bits(sysm.matrix.values), _bits(v))
         assert np.array_equal(_bits(rhs.cpu().numpy()), _bits(hrhs))
         assert mr == hmr
@@ -77,7 +80,7 @@ def test_refresh_rejects_foreign_pattern():
     z = torch.zeros(n, dtype=torch.float64, device="cuda")
     for gx, gy in ((nx, ny), (nx + 1, ny)):  # wrong pattern, wrong size
         with pytest.raises(ca.CfdlaError):
-            ca._native.call("cfdla_simple_pressure_refresh_device", m.handle, gx, gy, 0.1, 0.1,
+            ca._native.call("cfdla_simple_pressure_refresh_device", m.handle, gx, gy, 0.1, 0.1, 0,
                             *(z.data_ptr(),) * 7, None)
 
 
@@ -195,7 +198,8 @@ def test_corrections_match_oracle():
         assert np.array_equal(_bits(dev[k].cpu().numpy()), _bits(o)), k
 
 
-def test_simple_iterations_device_equal_host():
+@pytest.mark.parametrize("pos", [False, True], ids=["reference-sign", "positive-diagonal"])
+def test_simple_iterations_device_equal_host(pos):
     # three SIMPLE pressure steps: refresh, AMG-preconditioned BiCGSTAB to
     # 1e-10 (pressure.rs:452-485 solves the same system with BiCGSTAB),
     # corrections.  The device path never copies the matrix; the host path
@@ -205,11 +209,12 @@ def test_simple_iterations_device_equal_host():
     dx, dy, relax = 1 / 95, 1 / 71, 0.3
     n = nx * ny
     u, v, p, du, dv, rho = _fields(nx, ny, 90)
-    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, WEST_OUTLET, fields=(u, v, p, du, dv, rho))
+    sysm = S.SimplePressureCorrection(nx, ny, dx, dy, WEST_OUTLET, fields=(u, v, p, du, dv, rho),
+                                      positive_diagonal=pos)
     cfg = ca.AMGConfig.smoothed_aggregation()
     cfg.min_coarse_size = 200
     amg_d = ca.AlgebraicMultigrid(sysm.matrix, cfg)
-    rp, ci, v0, _, _ = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, p, du, dv, rho)
+    rp, ci, v0, _, _ = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, u, v, p, du, dv, rho, pos)
     a_h = ca.CsrMatrix.from_parts(v0, ci, rp, n, n)
     amg_h = ca.AlgebraicMultigrid(a_h, cfg)
     scfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(2000)
@@ -223,7 +228,7 @@ def test_simple_iterations_device_equal_host():
         rep_d = ca.bicgstab_preconditioned(sysm.matrix, rhs, amg_d, pp, scfg)
         sysm.apply_corrections_device(pp, dev["du"], dev["dv"], dev["p"], dev["u"], dev["v"], relax)
 
-        _, _, vh, bh, mrh = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, hu, hv, hp, du, dv, rho)
+        _, _, vh, bh, mrh = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, hu, hv, hp, du, dv, rho, pos)
         a_h.update_values(vh)
         amg_h.recompute(a_h)
         pph = np.zeros(n)
@@ -257,3 +262,20 @@ def test_amg_recompute_singular_coarse_raises_like_host(bad):
     with pytest.raises(ca.CfdlaError) as ed:
         m_d.recompute_device(a_d)
     assert ed.value.status == eh.value.status == ca._native.ERR_NUMERICAL
+
+
+@pytest.mark.parametrize("pin", [True, False])
+def test_pressure_rhs_device_matches_host(pin):
+    # correction.rs:345-356 on the device; the matrix of that path is
+    # constant, so this is its whole per-iteration assembly
+    torch = _torch()
+    for nx, ny in ((64, 64), (37, 53)):
+        rng = np.random.default_rng(nx + ny)
+        u, v = rng.uniform(-1, 1, nx * ny), rng.uniform(-1, 1, nx * ny)
+        host = np.empty((nx - 2) * (ny - 2) - (1 if pin else 0))
+        ca._native.call("cfdla_problem_pressure_rhs", nx, ny, 0.1, 0.07, 1.3, 0.05, ca._native.ptr(u),
+                        ca._native.ptr(v), 1 if pin else 0, ca._native.ptr(host))
+        out = torch.full((host.size,), 9.0, dtype=torch.float64, device="cuda")
+        P.pressure_rhs_device(nx, ny, 0.1, 0.07, 1.3, 0.05, torch.tensor(u, device="cuda"),
+                              torch.tensor(v, device="cuda"), out, pin)
+        assert np.array_equal(_bits(out.cpu().numpy()), _bits(host))

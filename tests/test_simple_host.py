@@ -173,3 +173,26 @@ def test_invalid_sides_rejected():
         S.sides({"west": "pressure_outlet", "north": "outflow", "south": "outflow"})  # pressure.rs:603-617
     assert S.sides({"west": "pressure_outlet", "east": "wall", "north": "outflow", "south": "outflow"}) == \
         WEST_OUTLET
+
+
+def test_positive_diagonal_negates_interior_rows_exactly():
+    # same system up to the sign of the interior rows and their rhs entries:
+    # every diagonal positive, boundary rows untouched, bits flipped only in
+    # the sign
+    nx, ny, dx, dy = 12, 9, 0.1, 0.12
+    f = _fields(nx, ny, 6)
+    rp, ci, v, rhs, mr = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, *f)
+    rp2, ci2, v2, rhs2, mr2 = P.simple_pressure(nx, ny, dx, dy, WEST_OUTLET, *f, positive_diagonal=True)
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and mr == mr2
+    rows = np.repeat(np.arange(nx * ny), np.diff(rp))
+    i, j = rows % nx, rows // nx
+    inner = (i > 0) & (i < nx - 1) & (j > 0) & (j < ny - 1)
+    assert np.array_equal(v2[inner], -v[inner]) and np.array_equal(v2[~inner], v[~inner])
+    node_inner = np.zeros((ny, nx), bool)
+    node_inner[1:-1, 1:-1] = True
+    node_inner = node_inner.ravel()
+    assert np.array_equal(rhs2[node_inner], -rhs[node_inner])
+    assert np.all(v2[ci2 == rows] > 0)
+    a = O.Csr.from_arrays(rp, ci, v, nx * ny).dense()
+    a2 = O.Csr.from_arrays(rp2, ci2, v2, nx * ny).dense()
+    assert np.allclose(np.linalg.solve(a2, rhs2), np.linalg.solve(a, rhs), rtol=1e-9, atol=1e-12)
