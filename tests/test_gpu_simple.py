@@ -138,7 +138,8 @@ def _amg_pair(rp, ci, v, n, cfg):
     return a_d, a_h, m_d, m_h
 
 
-@pytest.mark.parametrize("case", ["2d-dense-coarse", "2d-sweeps", "3d-dense-coarse", "simple-nonsymmetric"])
+@pytest.mark.parametrize("case", ["2d-dense-coarse", "2d-sweeps", "3d-dense-coarse", "3d-sliced-ell-level",
+                                  "simple-nonsymmetric"])
 def test_amg_recompute_device_matches_host(case):
     torch = _torch()
     cfg = ca.AMGConfig.smoothed_aggregation()
@@ -150,8 +151,8 @@ def test_amg_recompute_device_matches_host(case):
         cfg.min_coarse_size = 100
         if case == "2d-sweeps":
             cfg.coarse_solver = ca.CoarseSolver.ReferenceSweeps
-    elif case == "3d-dense-coarse":
-        nn = 24
+    elif case.startswith("3d"):
+        nn = 24 if case == "3d-dense-coarse" else 40  # 40^3: level 1 (~8,000 rows) in sliced ELL
         n = nn ** 3
         rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
         _, _, v2, _ = P.varcoef(nn, nn, nn, float(nn * nn), np.exp(P.uniform(63, n)), 1 << 4)
@@ -279,3 +280,23 @@ def test_pressure_rhs_device_matches_host(pin):
         P.pressure_rhs_device(nx, ny, 0.1, 0.07, 1.3, 0.05, torch.tensor(u, device="cuda"),
                               torch.tensor(v, device="cuda"), out, pin)
         assert np.array_equal(_bits(out.cpu().numpy()), _bits(host))
+
+
+def test_update_values_device_sliced_ell():
+    # the first Galerkin level of a 3D problem is stored sliced-ELL; new
+    # values reach its padded slots through the slot map
+    torch = _torch()
+    nn = 40
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
+    amg = ca.AlgebraicMultigrid(ca.CsrMatrix.from_parts(v, ci, rp, nn ** 3, nn ** 3),
+                                ca.AMGConfig.smoothed_aggregation())
+    rp1, ci1, v1, n1, _ = amg.level(1, "A")
+    m = ca.CsrMatrix.from_parts(v1, ci1, rp1, n1, n1)
+    assert m.storage()["format"] == "sell64"
+    v2 = v1 * (1.0 + 0.25 * P.uniform(91, v1.size))
+    m.update_values_device(torch.tensor(v2, device="cuda"))
+    x = P.uniform(92, n1)
+    yo = O.Csr.from_arrays(rp1, ci1, v2, n1).spmv(x)
+    assert np.allclose(m.spmv(x), yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+    y32 = m.spmv(torch.tensor(x, dtype=torch.float32, device="cuda")).cpu().numpy()
+    assert np.allclose(y32, yo, rtol=1e-4, atol=1e-5 * np.abs(yo).max())
