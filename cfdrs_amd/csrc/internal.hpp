@@ -348,6 +348,11 @@ void axpy_scal(int n, const T *x, T *y, Scal *s, int which);  // y += s->{alpha}
 template <typename T>
 void sor_colour(int count, const int *perm, const int *rp, const int *ci, const T *v, const T *piv, const T *r, T *z,
                 Scal *s, bool check_done);
+// The first colour from z = 0, which sees no nonzero neighbour:
+// z_i = piv_i r_i where first[i], 0 elsewhere (all 0 when check_done and
+// s->done != 0).
+template <typename T>
+void sor_first(int n, const uint8_t *first, const T *piv, const T *r, T *z, Scal *s, bool check_done);
 
 // Scalars helpers
 void scal_reset(Scal *s, double thr2);

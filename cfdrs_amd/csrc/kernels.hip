@@ -594,6 +594,18 @@ __global__ __launch_bounds__(kBlock) void k_sor_colour(int count, const int *__r
   }
 }
 
+// The first colour of a sweep from z = 0 sees only zero neighbours:
+// z_i = piv_i r_i there and 0 elsewhere, one streaming pass instead of a
+// fill and a sweep over the matrix rows.  (After convergence, z = 0.)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sor_first(int n, const uint8_t *__restrict__ first,
+                                                      const T *__restrict__ piv, const T *__restrict__ r, T *z,
+                                                      Scal *s, int check_done) {
+  const bool skip = check_done && s->done;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    z[i] = (!skip && first[i]) ? piv[i] * r[i] : T(0);
+}
+
 __global__ void k_scal_reset(Scal *s, double thr2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     for (int i = 0; i < 8; ++i) s->red[i] = 0.0;
@@ -809,6 +821,16 @@ void sor_colour(int count, const int *perm, const int *rp, const int *ci, const 
                      rp, ci, v, piv, r, z, s, check_done ? 1 : 0);
   CFDLA_HIP(hipGetLastError());
 }
+template <typename T>
+void sor_first(int n, const uint8_t *first, const T *piv, const T *r, T *z, Scal *s, bool check_done) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_sor_first<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, first, piv, r,
+                     z, s, check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+template void sor_first<double>(int, const uint8_t *, const double *, const double *, double *, Scal *, bool);
+template void sor_first<float>(int, const uint8_t *, const float *, const float *, float *, Scal *, bool);
+
 template void sor_colour<double>(int, const int *, const int *, const int *, const double *, const double *,
                                  const double *, double *, Scal *, bool);
 template void sor_colour<float>(int, const int *, const int *, const int *, const float *, const float *,

@@ -503,6 +503,9 @@ ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
   std::vector<float> v32h(a.v.size()), piv32h(piv.size());
   for (size_t k = 0; k < a.v.size(); ++k) v32h[k] = (float)a.v[k];
   for (size_t k = 0; k < piv.size(); ++k) piv32h[k] = (float)piv[k];
+  std::vector<uint8_t> first_h((size_t)n);
+  for (int i = 0; i < n; ++i) first_h[(size_t)i] = colour[(size_t)i] == 0 ? 1 : 0;
+  first.upload(first_h.data(), first_h.size());
   perm.upload(order.data(), order.size());
   rp.upload(rp32.data(), rp32.size());
   ci.upload(a.ci.data(), a.ci.size());
@@ -514,15 +517,15 @@ ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
 }
 
 void ColoredSorPrecond::apply(const double *r, double *z, Scal *s, bool chk) {
-  fill_vec<double>(n, 0.0, z);
-  for (int c = 0; c < ncolours; ++c)
+  sor_first<double>(n, first.p, piv64.p, r, z, s, chk);
+  for (int c = 1; c < ncolours; ++c)
     sor_colour<double>(colour_off[(size_t)c + 1] - colour_off[(size_t)c], perm.p + colour_off[(size_t)c], rp.p, ci.p,
                        v64.p, piv64.p, r, z, s, chk);
 }
 
 void ColoredSorPrecond::apply(const float *r, float *z, Scal *s, bool chk) {
-  fill_vec<float>(n, 0.0f, z);
-  for (int c = 0; c < ncolours; ++c)
+  sor_first<float>(n, first.p, piv32.p, r, z, s, chk);
+  for (int c = 1; c < ncolours; ++c)
     sor_colour<float>(colour_off[(size_t)c + 1] - colour_off[(size_t)c], perm.p + colour_off[(size_t)c], rp.p, ci.p,
                       v32.p, piv32.p, r, z, s, chk);
 }

@@ -60,6 +60,7 @@ struct ColoredSorPrecond : Precond {
   int n = 0, ncolours = 0;
   std::vector<int32_t> colour_off;  // ncolours + 1 offsets into perm
   DevBuf<int32_t> perm, rp, ci;      // rows grouped by colour; the matrix pattern
+  DevBuf<uint8_t> first;             // 1 on the rows of the first colour
   DevBuf<double> v64, piv64;         // values; omega / pivot (unit pivot where the diagonal is absent or zero)
   DevBuf<float> v32, piv32;
   ColoredSorPrecond(const HostCsr &a, double omega);

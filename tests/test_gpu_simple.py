@@ -300,3 +300,37 @@ def test_update_values_device_sliced_ell():
     assert np.allclose(m.spmv(x), yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
     y32 = m.spmv(torch.tensor(x, dtype=torch.float32, device="cuda")).cpu().numpy()
     assert np.allclose(y32, yo, rtol=1e-4, atol=1e-5 * np.abs(yo).max())
+
+
+def test_full_size_refresh_and_recompute_bit_exact():
+    # the workloads' own sizes: the 512^2 SIMPLE system refreshed on the
+    # device against the host assembly, and the 128^3 hierarchy recomputed
+    # on the device against the host recompute (3.2 GB of product plans)
+    torch = _torch()
+    nx = ny = 512
+    h = 1.0 / (nx - 1)
+    side = (S.SIDE_PLAIN, S.SIDE_PRESSURE, S.SIDE_PLAIN, S.SIDE_PLAIN)
+    f = _fields(nx, ny, 100)
+    sysm = S.SimplePressureCorrection(nx, ny, h, h, side, positive_diagonal=True)
+    rhs = torch.empty(nx * ny, dtype=torch.float64, device="cuda")
+    mr = sysm.assemble_device(*[torch.tensor(a, device="cuda") for a in f], rhs)
+    _, _, v, hrhs, hmr = P.simple_pressure(nx, ny, h, h, side, *f, positive_diagonal=True)
+    assert np.array_equal(_bits(sysm.matrix.values), _bits(v))
+    assert np.array_equal(_bits(rhs.cpu().numpy()), _bits(hrhs)) and mr == hmr
+
+    nn = 128
+    n = nn ** 3
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
+    a_d, a_h, m_d, m_h = _amg_pair(rp, ci, v, n, ca.AMGConfig.smoothed_aggregation())
+    v2 = v * (1.0 + 0.5 * np.abs(P.uniform(101, v.size)))  # not symmetric: the DIA layouts turn general
+    a_d.update_values_device(torch.tensor(v2, device="cuda"))
+    m_d.recompute_device(a_d)
+    a_h.update_values(v2)
+    m_h.recompute(a_h)
+    for lvl in range(m_d.statistics.num_levels):
+        assert np.array_equal(_bits(m_d.level(lvl, "A")[2]), _bits(m_h.level(lvl, "A")[2])), lvl
+    r = P.uniform(102, n)
+    z_d, z_h = np.empty(n), np.empty(n)
+    m_d.apply_to(r, z_d)
+    m_h.apply_to(r, z_h)
+    assert np.array_equal(_bits(z_d), _bits(z_h))
