@@ -258,23 +258,23 @@ __device__ inline Piv piv_of(double x, int row) { return Piv{x == x ? fabs(x) : 
 
 constexpr int kGjRows = 4;  // rows per block of the elimination step
 
-// Block-wide merge of one candidate per thread; the result is valid in
-// thread 0.
+// Block-wide merge of one candidate per thread (shuffles inside each wave,
+// then the 4 wave results); the result is returned to every thread.
 __device__ Piv piv_block(Piv mine) {
-  __shared__ double sv[kBlock];
-  __shared__ int sr[kBlock];
-  sv[threadIdx.x] = mine.val;
-  sr[threadIdx.x] = mine.row;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      const Piv o = piv_better(Piv{sv[threadIdx.x], sr[threadIdx.x]}, Piv{sv[threadIdx.x + w], sr[threadIdx.x + w]});
-      sv[threadIdx.x] = o.val;
-      sr[threadIdx.x] = o.row;
-    }
-    __syncthreads();
+  __shared__ double sv[kBlock / 64];
+  __shared__ int sr[kBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    mine = piv_better(mine, Piv{__shfl_xor(mine.val, o, 64), __shfl_xor(mine.row, o, 64)});
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = mine.val;
+    sr[threadIdx.x >> 6] = mine.row;
   }
-  return Piv{sv[0], sr[0]};
+  __syncthreads();
+  Piv best{sv[0], sr[0]};
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) best = piv_better(best, Piv{sv[w], sr[w]});
+  return best;
 }
 
 // Column 0's partials of the filled matrix.
@@ -290,8 +290,9 @@ __global__ __launch_bounds__(kBlock) void k_gj_init(int n, const double *m, doub
 
 // One Gauss-Jordan column step on [A | I], ping-pong src -> dst: pivot row p
 // from the partials, rows c and p swapped, row c scaled by d = 1 / m_pc,
-// every other row r with f = m_rc != 0 updated m_rj - f (m_pj d).  Writes
-// the next column's pivot partials.  A singular column (NaN at row c, best
+// every other row r with f = m_rc != 0 updated m_rj - f (m_pj d) for the
+// columns j > c (the left-half columns up to c are dead).  Writes the next
+// column's pivot partials.  A singular column (NaN at row c, best
 // pivot 0 or not finite) sets *bad; every later step then returns at once.
 __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *src, double *dst,
                                                     const double *pin, const int *rin, double *pout, int *rout,
@@ -308,6 +309,9 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
     s_p = (*bad || mcc != mcc || !(best.val > 0.0 && best.val <= 1.7976931348623157e308)) ? -1 : best.row;
   }
   __syncthreads();
+  // Columns 0..c of the left half are never read after this step (pivots
+  // and multipliers come from later columns), so they are not written.
+  const int64_t j0 = c + 1;
   const int p = s_p;
   if (p < 0) {
     if (threadIdx.x == 0) *bad = 1;
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
     const double *prow = src + (int64_t)p * w;
     double *drow = dst + (int64_t)r * w;
     const double f = srow[c];
-    for (int64_t j = threadIdx.x; j < w; j += kBlock) {
+    for (int64_t j = j0 + threadIdx.x; j < w; j += kBlock) {
       double out;
       if (r == c) out = prow[j] * d;
       else if (f == 0.0) out = srow[j];
