@@ -79,65 +79,80 @@ __global__ __launch_bounds__(kBlock) void k_mdot_fin(int nb, const double *__res
   }
 }
 
-// w -= sum_{k<=j} hp[k] V_k
-template <typename T>
+// w -= sum_{k<=j} hp[k] V_k; with NORM, also the block partials of ||w||^2
+// of the updated w (the last Gram-Schmidt pass: the norm needs no pass of
+// its own).  (Summing the multi-dot partials here instead of in k_mdot_fin,
+// with 64 partials per vector, was slower: 22.1 -> 23.2 ms for the 512^2
+// momentum GMRES.)
+template <typename T, bool NORM>
 __global__ __launch_bounds__(kBlock) void k_maxpy(int n, int jp1, const T *__restrict__ V, size_t ld,
-                                                  const double *__restrict__ hp, T *__restrict__ w, Scal *s) {
+                                                  const double *__restrict__ hp, T *__restrict__ w, Scal *s,
+                                                  double *partials) {
   if (s->done) return;
   __shared__ double h[257];
   for (int k = threadIdx.x; k < jp1; k += kBlock) h[k] = hp[k];
   __syncthreads();
+  double v[1] = {0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     double acc = (double)w[i];
     for (int k = 0; k < jp1; ++k) acc -= h[k] * (double)V[(size_t)k * ld + i];
-    w[i] = (T)acc;
+    const T wi = (T)acc;
+    w[i] = wi;
+    if (NORM) v[0] += (double)wi * (double)wi;
   }
-}
-
-// block partials of ||w||^2
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gm_norm(int n, const T *__restrict__ w, Scal *s, double *partials) {
-  if (s->done) return;
-  double v[1] = {0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += (double)w[i] * (double)w[i];
-  block_partials<1>(v, partials);
+  if (NORM) block_partials<1>(v, partials);
 }
 
 // One block: ||w||, Hessenberg column j, Givens rotation, residual estimate.
+// The column and the earlier rotations are staged in LDS by the whole block,
+// so thread 0's rotation chain waits on arithmetic, not on global memory.
 __global__ __launch_bounds__(kBlock) void k_gm_arnoldi(int j, int nblocks, const double *partials, Scal *s,
                                                        GmLayout L) {
   if (s->done) return;
+  __shared__ double col[258], cs[256], sn[256];
+  for (int k = threadIdx.x; k <= j; k += kBlock) col[k] = L.hc[k];
+  for (int k = threadIdx.x; k < j; k += kBlock) {
+    cs[k] = L.cs[k];
+    sn[k] = L.sn[k];
+  }
   double tot[1];
-  sum_partials<1>(nblocks, partials, tot);
-  if (threadIdx.x != 0) return;
-  {
-    const int m = L.m;
+  sum_partials<1>(nblocks, partials, tot);  // synchronizes the block
+  const int m = L.m;
+  __shared__ int broke;
+  if (threadIdx.x == 0) {
     const double hn = sqrt(tot[0]);
     s->hn = hn;
-    for (int k = 0; k <= j; ++k) L.H[(size_t)k * m + j] = L.hc[k];
-    L.H[(size_t)(j + 1) * m + j] = hn;
+    col[j + 1] = hn;
     for (int k = 0; k < j; ++k) {
-      double t0 = L.H[(size_t)k * m + j], t1 = L.H[(size_t)(k + 1) * m + j];
-      L.H[(size_t)k * m + j] = L.cs[k] * t0 + L.sn[k] * t1;
-      L.H[(size_t)(k + 1) * m + j] = -L.sn[k] * t0 + L.cs[k] * t1;
+      const double t0 = col[k], t1 = col[k + 1];
+      col[k] = cs[k] * t0 + sn[k] * t1;
+      col[k + 1] = -sn[k] * t0 + cs[k] * t1;
     }
-    double h0 = L.H[(size_t)j * m + j], h1 = L.H[(size_t)(j + 1) * m + j];
-    double den = hypot(h0, h1);
-    if (den == 0.0 || !isfinite(den)) {
+    const double h0 = col[j], h1 = col[j + 1];
+    const double den = hypot(h0, h1);
+    broke = den == 0.0 || !isfinite(den);
+    if (broke) {
       s->done = 4;  // breakdown: the cycle ends with kstep = j
-      return;
+    } else {
+      const double c = h0 / den, sj = h1 / den;
+      L.cs[j] = c;
+      L.sn[j] = sj;
+      col[j] = den;
+      col[j + 1] = 0.0;
+      const double gj = L.g[j];
+      L.g[j + 1] = -sj * gj;
+      L.g[j] = c * gj;
+      s->kstep = j + 1;
+      s->iters += 1;
+      s->alpha = hn > 0.0 ? 1.0 / hn : 0.0;
+      if (fabs(L.g[j + 1]) <= sqrt(s->thr2) || hn == 0.0) s->done = 1;
     }
-    L.cs[j] = h0 / den;
-    L.sn[j] = h1 / den;
-    L.H[(size_t)j * m + j] = den;
-    L.H[(size_t)(j + 1) * m + j] = 0.0;
-    L.g[j + 1] = -L.sn[j] * L.g[j];
-    L.g[j] = L.cs[j] * L.g[j];
-    s->kstep = j + 1;
-    s->iters += 1;
-    s->alpha = hn > 0.0 ? 1.0 / hn : 0.0;
-    if (fabs(L.g[j + 1]) <= sqrt(s->thr2) || hn == 0.0) s->done = 1;
   }
+  __syncthreads();
+  // the column as the host-visible Hessenberg H(0:j+1, j); on breakdown the
+  // rotated entries up to j + 1, as before the rotation that failed
+  for (int k = threadIdx.x; k <= j + 1; k += kBlock) L.H[(size_t)k * m + j] = col[k];
+  (void)broke;
 }
 
 // v_{j+1} = w / hn (w already lives in V_{j+1})
@@ -250,9 +265,13 @@ SolveOut gmres(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_
         hipLaunchKernelGGL((k_mdot<T>), dim3(nb, j + 1), dim3(kBlock), 0, stream(), n, V, ld, w, L.part, s);
         hipLaunchKernelGGL(k_mdot_fin, dim3(j + 1), dim3(kBlock), 0, stream(), nb, L.part, L.hp, L.hc,
                            pass == 0 ? 1 : 0, s);
-        hipLaunchKernelGGL((k_maxpy<T>), dim3(eb), dim3(kBlock), 0, stream(), n, j + 1, V, ld, L.hp, w, s);
+        if (pass == 0)
+          hipLaunchKernelGGL((k_maxpy<T, false>), dim3(eb), dim3(kBlock), 0, stream(), n, j + 1, V, ld, L.hp, w, s,
+                             nullptr);
+        else
+          hipLaunchKernelGGL((k_maxpy<T, true>), dim3(eb), dim3(kBlock), 0, stream(), n, j + 1, V, ld, L.hp, w, s,
+                             ws.partials.p);
       }
-      hipLaunchKernelGGL((k_gm_norm<T>), dim3(eb), dim3(kBlock), 0, stream(), n, w, s, ws.partials.p);
       hipLaunchKernelGGL(k_gm_arnoldi, dim3(1), dim3(kBlock), 0, stream(), j, eb, ws.partials.p, s, L);
       hipLaunchKernelGGL((k_gm_scale<T>), dim3(eb), dim3(kBlock), 0, stream(), n, w, s);
     }
