@@ -236,6 +236,7 @@ enum FinalizeOp : int {
   OP_BI_RR = 12,       // BiCGSTAB: rr = red0; iters++
   OP_RELIABLE = 13,    // rr_true = rr = red0 (FP64 residual); done = 4 if rr <= thr_final2, else 0;
                        // restart_dir if it exceeds 4x the recursive rr it replaces
+  OP_BI_RR_RHO = 14,   // BiCGSTAB: OP_BI_RR on red0, then (still running) OP_BI_RHO on red1 = rhat . r
 };
 
 enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4, EPI_RJ0 = 5 };
@@ -336,9 +337,12 @@ template <typename T>
 void bicg_p_update(int n, const T *r, const T *v, T *p, Scal *s);  // p = r + beta (p - omega v)
 template <typename T>
 void bicg_s_update(int n, const T *r, const T *v, T *sv, Scal *s, RedWs &ws);  // s = r - alpha v; ||s||^2
+// x += alpha ph + omega sh; r = s - omega t; ||r||^2 -> OP_BI_RR.  With rhat
+// non-NULL also rhat . r -> OP_BI_RR_RHO: the next iteration's rho and beta
+// come out of the same finalize.
 template <typename T>
-void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s,
-                     RedWs &ws);  // x += alpha ph + omega sh; r = s - omega t; ||r||^2
+void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s, RedWs &ws,
+                     const T *rhat = nullptr);
 template <typename T>
 void axpy_scal(int n, const T *x, T *y, Scal *s, int which);  // y += s->{alpha} x
 

@@ -127,7 +127,8 @@ __device__ void finalize(Scal *s, int op) {
         s->nrestart += s->restart_dir;
       }
     } break;
-    case OP_BI_RR: {
+    case OP_BI_RR:
+    case OP_BI_RR_RHO: {
       double rr = s->red[0];
       s->rr = rr;
       if (!isfinite(rr)) {
@@ -138,6 +139,17 @@ __device__ void finalize(Scal *s, int op) {
       } else if (s->omega == 0.0 || !isfinite(s->omega)) {
         s->term = CFDLA_TERM_BREAKDOWN;
         s->done = 2;
+      } else if (op == OP_BI_RR_RHO) {
+        // the next iteration's OP_BI_RHO, with rho_new = rhat . r
+        double rho_new = s->red[1];
+        if (rho_new == 0.0 || !isfinite(rho_new)) {
+          s->term = CFDLA_TERM_BREAKDOWN;
+          s->done = 2;
+        } else {
+          s->beta = s->restart_dir ? 0.0 : (rho_new / s->rho) * (s->alpha / s->omega);
+          s->restart_dir = 0;
+          s->rho = rho_new;
+        }
       }
     } break;
     default:
@@ -552,22 +564,24 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int n, const T *__restrict__ 
   reduce_and_finalize<1>(acc, s, OP_BI_SNORM, partials, counter);
 }
 
-// BiCGSTAB: x += alpha ph + omega sh; r = s - omega t; ||r||^2
-template <typename T>
+// BiCGSTAB: x += alpha ph + omega sh; r = s - omega t; ||r||^2 (and, with
+// RHO, rhat . r for the next iteration)
+template <typename T, bool RHO>
 __global__ __launch_bounds__(kBlock) void k_bicg_xr(int n, const T *__restrict__ ph, const T *__restrict__ sh,
                                                     const T *__restrict__ sv, const T *__restrict__ t,
-                                                    T *__restrict__ x, T *__restrict__ r, Scal *s,
-                                                    double *partials, unsigned *counter) {
+                                                    T *__restrict__ x, T *__restrict__ r, const T *__restrict__ rhat,
+                                                    Scal *s, double *partials, unsigned *counter) {
   if (s->done) return;
   const T alpha = (T)s->alpha, omega = (T)s->omega;
-  double acc[1] = {0.0};
+  double acc[RHO ? 2 : 1] = {};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] += alpha * ph[i] + omega * sh[i];
     T ri = sv[i] - omega * t[i];
     r[i] = ri;
     acc[0] += (double)ri * (double)ri;
+    if constexpr (RHO) acc[1] += (double)rhat[i] * (double)ri;
   }
-  reduce_and_finalize<1>(acc, s, OP_BI_RR, partials, counter);
+  reduce_and_finalize<RHO ? 2 : 1>(acc, s, RHO ? OP_BI_RR_RHO : OP_BI_RR, partials, counter);
 }
 
 template <typename T>
@@ -799,11 +813,18 @@ void bicg_s_update(int n, const T *r, const T *v, T *sv, Scal *s, RedWs &ws) {
 }
 
 template <typename T>
-void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s, RedWs &ws) {
+void bicg_x_r_update(int n, const T *ph, const T *sh, const T *sv, const T *t, T *x, T *r, Scal *s, RedWs &ws,
+                     const T *rhat) {
   const int blocks = blocks_for(n, kBlock * 4);
-  hipLaunchKernelGGL((k_bicg_xr<T>), dim3(blocks), dim3(kBlock), 0, stream(), n, ph, sh, sv, t, x, r, s,
-                     ws.partials.p, ws.counter.p);
-  finish(1, s, OP_BI_RR, blocks, ws, true);
+  if (rhat) {
+    hipLaunchKernelGGL((k_bicg_xr<T, true>), dim3(blocks), dim3(kBlock), 0, stream(), n, ph, sh, sv, t, x, r, rhat,
+                       s, ws.partials.p, ws.counter.p);
+    finish(2, s, OP_BI_RR_RHO, blocks, ws, true);
+  } else {
+    hipLaunchKernelGGL((k_bicg_xr<T, false>), dim3(blocks), dim3(kBlock), 0, stream(), n, ph, sh, sv, t, x, r,
+                       nullptr, s, ws.partials.p, ws.counter.p);
+    finish(1, s, OP_BI_RR, blocks, ws, true);
+  }
   CFDLA_HIP(hipGetLastError());
 }
 
@@ -877,7 +898,8 @@ void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, co
   template void dense_gemv<T>(int, const double *, const T *, T *, Scal *, bool);                             \
   template void bicg_p_update<T>(int, const T *, const T *, T *, Scal *);                                      \
   template void bicg_s_update<T>(int, const T *, const T *, T *, Scal *, RedWs &);                             \
-  template void bicg_x_r_update<T>(int, const T *, const T *, const T *, const T *, T *, T *, Scal *, RedWs &); \
+  template void bicg_x_r_update<T>(int, const T *, const T *, const T *, const T *, T *, T *, Scal *, RedWs &,  \
+                                   const T *);                                                               \
   template void axpy_scal<T>(int, const T *, T *, Scal *, int);
 
 CFDLA_INSTANTIATE(double)

@@ -1003,11 +1003,15 @@ SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int
     o.term = CFDLA_TERM_INITIAL_RESIDUAL;
     return o;
   }
+  // rho = rhat . r is formed by the x/r update of the previous iteration
+  // (OP_BI_RR_RHO), except right after a (re)start
+  bool need_rho = true;
   auto restart = [&]() {
     copy_vec<T>(n, r, rhat, s, false);
     fill_vec<T>(n, T(0), p);
     fill_vec<T>(n, T(0), v);
     scal_bicg_restart(s);
+    need_rho = true;
   };
   restart();
   if (chunk <= 0) chunk = auto_chunk(M, n);
@@ -1019,7 +1023,8 @@ SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int
     if (enq < max_it) {
       int64_t steps = std::min<int64_t>(chunk, max_it - enq);
       for (int64_t k = 0; k < steps; ++k) {
-        dot2<T>(n, rhat, r, nullptr, nullptr, s, true, OP_BI_RHO, ws);
+        if (need_rho) dot2<T>(n, rhat, r, nullptr, nullptr, s, true, OP_BI_RHO, ws);
+        need_rho = false;
         bicg_p_update<T>(n, r, v, p, s);
         M->apply(p, ph, s, true);
         csr_apply<T>(A, EPI_Y, ph, nullptr, nullptr, 0.0, v, s, true, OP_BI_ALPHA, rhat, ws);
@@ -1027,7 +1032,7 @@ SolveOut bicgstab(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int
         M->apply(sv, sh, s, true);
         csr_apply<T>(A, EPI_Y, sh, nullptr, nullptr, 0.0, t, s, true, OP_NONE, nullptr, ws);
         dot2<T>(n, t, sv, t, t, s, true, OP_BI_OMEGA, ws);
-        bicg_x_r_update<T>(n, ph, sh, sv, t, x, r, s, ws);
+        bicg_x_r_update<T>(n, ph, sh, sv, t, x, r, s, ws, rhat);
       }
       enq += steps;
       poll.post();

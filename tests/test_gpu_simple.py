@@ -334,3 +334,28 @@ def test_full_size_refresh_and_recompute_bit_exact():
     m_d.apply_to(r, z_d)
     m_h.apply_to(r, z_h)
     assert np.array_equal(_bits(z_d), _bits(z_h))
+
+
+def test_amg_recompute_device_single_level():
+    # a hierarchy that is only the dense coarsest solve: no Galerkin
+    # products, the inverse alone is recomputed
+    torch = _torch()
+    nn = 12
+    n = nn * nn
+    rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(111, n)), 0b10)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 1000  # n = 144 rows: no coarsening
+    a_d, a_h, m_d, m_h = _amg_pair(rp, ci, v, n, cfg)
+    assert m_d.statistics.num_levels == 1
+    v2 = v * (1.0 + 0.3 * np.abs(P.uniform(112, v.size)))
+    a_d.update_values_device(torch.tensor(v2, device="cuda"))
+    m_d.recompute_device(a_d)
+    a_h.update_values(v2)
+    m_h.recompute(a_h)
+    r = P.uniform(113, n)
+    z_d, z_h = np.empty(n), np.empty(n)
+    m_d.apply_to(r, z_d)
+    m_h.apply_to(r, z_h)
+    assert np.array_equal(_bits(z_d), _bits(z_h))
+    d2 = O.Csr.from_arrays(rp, ci, v2, n).dense()
+    assert np.allclose(z_d, np.linalg.solve(d2, r), rtol=1e-10, atol=1e-12)
