@@ -55,7 +55,8 @@ class AmgConfig(C.Structure):
                 ("post_smooth_iterations", C.c_int32), ("coarse_solver", C.c_int32),
                 ("relaxation_factor", C.c_double), ("strength_threshold", C.c_double),
                 ("smoothing_omega", C.c_double), ("max_aggregate_size", C.c_int32),
-                ("max_interpolation_points", C.c_int32)]
+                ("max_interpolation_points", C.c_int32), ("galerkin_on_device", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class AmgStatistics(C.Structure):

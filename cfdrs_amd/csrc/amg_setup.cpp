@@ -493,7 +493,7 @@ std::vector<double> dense_inverse(const HostCsr &a) {
   return inv;
 }
 
-void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHierarchy &h) {
+void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHierarchy &h, const GalerkinFn &galerkin) {
   h.levels.clear();
   h.levels.emplace_back();
   h.levels[0].A = fine;
@@ -511,7 +511,7 @@ void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHiera
     else if (cfg.interpolation == 1) P = piecewise_constant_p(split.fine_to_coarse, cur.A.n, nc);
     else P = smooth_prolongator(cur.A, piecewise_constant_p(split.fine_to_coarse, cur.A.n, nc), cfg.smoothing_omega);
     HostCsr R = csr_transpose(P);
-    HostCsr Ac = csr_multiply(csr_multiply(R, cur.A), P);
+    HostCsr Ac = galerkin ? galerkin(h.levels.size() - 1, R, cur.A, P) : csr_multiply(csr_multiply(R, cur.A), P);
     cur.P = std::move(P);
     cur.R = std::move(R);
     cur.fine_to_coarse = std::move(split.fine_to_coarse);

@@ -1,6 +1,7 @@
 // Host-side AMG setup of libcfdla (see amg_setup.cpp).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -42,7 +43,11 @@ HostCsr classical_p(const HostCsr &a, const std::vector<int32_t> &cpts, const Ho
 HostCsr smooth_prolongator(const HostCsr &a, const HostCsr &pt, double omega);
 std::vector<double> dense_inverse(const HostCsr &a);
 
-void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHierarchy &h);
+// The Galerkin product of level l, A_{l+1} = (R A) P; the host product when
+// none is given.
+using GalerkinFn = std::function<HostCsr(size_t level, const HostCsr &R, const HostCsr &A, const HostCsr &P)>;
+void build_hierarchy(const HostCsr &fine, const cfdla_amg_config &cfg, HostHierarchy &h,
+                     const GalerkinFn &galerkin = nullptr);
 void recompute_hierarchy(HostHierarchy &h, const double *fine_values);
 
 }  // namespace cfdla

@@ -560,7 +560,49 @@ AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
                 "the device V-cycle smooths with Jacobi only (smoother = 2); Gauss-Seidel sweeps are sequential");
   CFDLA_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, CFDLA_ERR_CONFIG, "AMG coarse_solver must be 0 or 1");
   auto t0 = std::chrono::steady_clock::now();
-  build_hierarchy(a, cfg, host);
+  GalerkinFn galerkin;
+  if (cfg.galerkin_on_device) {
+    ensure_init();
+    galerkin = [this](size_t l, const HostCsr &R, const HostCsr &A, const HostCsr &P) {
+      struct Dev {
+        DevBuf<int32_t> rp, ci;
+        DevBuf<double> v;
+        explicit Dev(const HostCsr &h) {
+          CFDLA_REQUIRE(h.nnz() < INT32_MAX && h.n < INT32_MAX, CFDLA_ERR_INVALID_ARG,
+                        "matrix too large for 32-bit device indices");
+          std::vector<int32_t> rp32(h.rp.begin(), h.rp.end());
+          rp.upload(rp32.data(), rp32.size());
+          ci.upload(h.ci.data(), h.ci.size());
+          v.upload(h.v.data(), h.v.size());
+          sync_stream();
+        }
+      };
+      const Dev r(R), a(A), p(P);
+      if (gplan.size() <= l) gplan.resize(l + 1);
+      GalerkinPlan &g = gplan[l];
+      DevBuf<int32_t> ra_rp, ra_ci, c_rp, c_ci;
+      DevBuf<double> c_v;
+      device_product((int)R.n, (int)A.m, r.rp.p, r.ci.p, r.v.p, a.rp.p, a.ci.p, a.v.p, g.ra, ra_rp, ra_ci, g.ra_v);
+      device_product((int)R.n, (int)P.m, ra_rp.p, ra_ci.p, g.ra_v.p, p.rp.p, p.ci.p, p.v.p, g.rap, c_rp, c_ci, c_v);
+      HostCsr ac;
+      ac.n = R.n;
+      ac.m = P.m;
+      std::vector<int32_t> rp32((size_t)ac.n + 1);
+      c_rp.download(rp32.data(), rp32.size());
+      sync_stream();
+      ac.rp.assign(rp32.begin(), rp32.end());
+      const size_t nnz = (size_t)ac.rp.back();
+      ac.ci.resize(nnz);
+      ac.v.resize(nnz);
+      c_ci.download(ac.ci.data(), nnz);
+      c_v.download(ac.v.data(), nnz);
+      sync_stream();
+      return ac;
+    };
+  }
+  build_hierarchy(a, cfg, host, galerkin);
+  // plans of products whose level was kept; a final empty product is dropped
+  if (galerkin) gplan.resize(host.levels.size() - 1);
   setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -92,6 +92,13 @@ struct GalerkinPlan {
   DevBuf<double> ra_v;
 };
 
+// C = A B on the device (galerkin.hip), bit-identical to csr_multiply: C's
+// row pointers, columns and values, and the per-slot plan of the product.
+// A is nr rows (int32 CSR), B has ncols columns.
+void device_product(int nr, int ncols, const int32_t *arp, const int32_t *aci, const double *av, const int32_t *brp,
+                    const int32_t *bci, const double *bv, DevPlan &plan, DevBuf<int32_t> &crp,
+                    DevBuf<int32_t> &cci, DevBuf<double> &cv);
+
 struct AmgPrecond : Precond {
   cfdla_amg_config cfg{};
   HostHierarchy host;

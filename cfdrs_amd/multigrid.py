@@ -72,6 +72,9 @@ class AMGConfig:
     coarse_solver: CoarseSolver = CoarseSolver.ReferenceSweeps
     smoothing_omega: float = 2.0 / 3.0
     max_aggregate_size: int = 8
+    # form the Galerkin products on the GPU (bit-identical to the host
+    # products; no reference counterpart)
+    galerkin_on_device: bool = False
 
     @classmethod
     def default(cls) -> "AMGConfig":
@@ -106,7 +109,7 @@ class AMGConfig:
                            int(self.interpolation_strategy), int(self.smoother_type), self.pre_smooth_iterations,
                            self.post_smooth_iterations, int(self.coarse_solver), self.relaxation_factor,
                            self.strength_threshold, self.smoothing_omega, self.max_aggregate_size,
-                           self.max_interpolation_points)
+                           self.max_interpolation_points, 1 if self.galerkin_on_device else 0, 0)
 
 
 @dataclass
@@ -207,7 +210,10 @@ class AlgebraicMultigrid(_Preconditioner):
         """AlgebraicMultigrid::recompute (amg.rs:179-214)."""
         if fine_matrix.nnz != self.matrix.nnz:
             raise N.CfdlaError(N.ERR_DIMENSION, "recompute requires the same sparsity pattern")
-        N.call("cfdla_amg_update_values", self.handle, N.ptr(np.ascontiguousarray(fine_matrix.creation_order_values())))
+        # keep the array referenced across the call: for a matrix created with
+        # unsorted rows creation_order_values() is a fresh array
+        values = np.ascontiguousarray(fine_matrix.creation_order_values(), dtype=np.float64)
+        N.call("cfdla_amg_update_values", self.handle, N.ptr(values))
 
     def recompute_device(self, fine_matrix: CsrMatrix) -> None:
         """AlgebraicMultigrid::recompute (amg.rs:179-214) from the current

@@ -149,8 +149,9 @@ def momentum_u(n, dx, dy, dt, mu, rho, u, v):
 
 def pressure_rhs(n, dx, dy, rho, dt, u, v, pin=True):
     out = np.empty((n - 2) * (n - 2) - (1 if pin else 0))
-    N.call("cfdla_problem_pressure_rhs", n, n, dx, dy, rho, dt, N.ptr(np.ascontiguousarray(u, np.float64)),
-           N.ptr(np.ascontiguousarray(v, np.float64)), 1 if pin else 0, N.ptr(out))
+    u = np.ascontiguousarray(u, np.float64)  # referenced until the call returns
+    v = np.ascontiguousarray(v, np.float64)
+    N.call("cfdla_problem_pressure_rhs", n, n, dx, dy, rho, dt, N.ptr(u), N.ptr(v), 1 if pin else 0, N.ptr(out))
     return out
 
 

@@ -128,6 +128,11 @@ typedef struct {
   double smoothing_omega;
   int32_t max_aggregate_size;
   int32_t max_interpolation_points;
+  /* 1: form the Galerkin products A_{l+1} = (R_l A_l) P_l on the device
+   * (bit-identical to the host products; their plans also serve
+   * cfdla_amg_update_values_device).  No reference counterpart. */
+  int32_t galerkin_on_device;
+  int32_t reserved;
 } cfdla_amg_config;
 
 #define CFDLA_MAX_LEVELS 32

@@ -359,3 +359,71 @@ def test_amg_recompute_device_single_level():
     assert np.array_equal(_bits(z_d), _bits(z_h))
     d2 = O.Csr.from_arrays(rp, ci, v2, n).dense()
     assert np.allclose(z_d, np.linalg.solve(d2, r), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["2d", "3d-40", "simple-nonsymmetric", "unsorted-random"])
+def test_device_galerkin_setup_matches_host(case):
+    # galerkin_on_device: A_{l+1} = (R_l A_l) P_l by expand-sort-compress on
+    # the GPU; every level's pattern and values equal the host products', and
+    # the plans it leaves behind drive the device recompute
+    torch = _torch()
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    if case == "2d":
+        nn = 96
+        n = nn * nn
+        rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.exp(P.uniform(121, n)), 0b10)
+        cfg.min_coarse_size = 100
+    elif case == "3d-40":
+        nn = 40
+        n = nn ** 3
+        rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
+    elif case == "simple-nonsymmetric":
+        nx, ny = 72, 64
+        n = nx * ny
+        rp, ci, v, _, _ = P.simple_pressure(nx, ny, 1 / 71, 1 / 63, WEST_OUTLET, *_fields(nx, ny, 122),
+                                            positive_diagonal=True)
+        cfg.min_coarse_size = 200
+    else:
+        # diagonally dominant random pattern, columns created unsorted
+        rng = np.random.default_rng(123)
+        n = 6000
+        rows = np.repeat(np.arange(n), 7)
+        cols = np.concatenate([np.sort(rng.choice(n, 7, replace=False)) for _ in range(n)])
+        dense_diag = np.zeros(n)
+        vals = -rng.uniform(0.1, 1.0, rows.size)
+        vals[rows == cols] = 0.0
+        np.add.at(dense_diag, rows, -vals)
+        keep = rows != cols
+        rows, cols, vals = rows[keep], cols[keep], vals[keep]
+        rows = np.concatenate([rows, np.arange(n)])
+        cols = np.concatenate([cols, np.arange(n)])
+        vals = np.concatenate([vals, dense_diag + 1.0])
+        order = np.lexsort((rng.random(rows.size), rows))
+        rows, cols, vals = rows[order], cols[order], vals[order]
+        rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))])
+        ci, v = cols, vals
+        cfg.min_coarse_size = 200
+    a_h = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    a_d = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    m_h = ca.AlgebraicMultigrid(a_h, cfg)
+    cfg_d = ca.AMGConfig(**{**cfg.__dict__, "galerkin_on_device": True})
+    m_d = ca.AlgebraicMultigrid(a_d, cfg_d)
+    L = m_h.statistics.num_levels
+    assert L >= 2 and m_d.statistics.num_levels == L
+    for lvl in range(L):
+        h, d = m_h.level(lvl, "A"), m_d.level(lvl, "A")
+        assert np.array_equal(h[0], d[0]) and np.array_equal(h[1], d[1]), lvl
+        assert np.array_equal(_bits(h[2]), _bits(d[2])), lvl
+    r = P.uniform(124, n)
+    z_h, z_d = np.empty(n), np.empty(n)
+    m_h.apply_to(r, z_h)
+    m_d.apply_to(r, z_d)
+    assert np.array_equal(_bits(z_h), _bits(z_d))
+    # the setup's plans serve the device recompute
+    v2 = a_h.creation_order_values() * (1.0 + 0.2 * np.abs(P.uniform(125, v.size)))
+    a_d.update_values_device(torch.tensor(v2, device="cuda"))
+    m_d.recompute_device(a_d)
+    a_h.update_values(v2)
+    m_h.recompute(a_h)
+    for lvl in range(L):
+        assert np.array_equal(_bits(m_d.level(lvl, "A")[2]), _bits(m_h.level(lvl, "A")[2])), lvl

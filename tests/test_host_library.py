@@ -225,3 +225,34 @@ def test_plugin_surface():
         register_backend("wgpu")
     with pytest.raises(ValueError):
         reg.register(p)
+
+
+def test_amg_recompute_with_unsorted_matrix_passes_creation_order():
+    # a matrix created with unsorted rows: recompute hands the library the
+    # values in creation order (a fresh array that must stay referenced
+    # across the call); level 0 then holds the new values in stored order
+    rng = np.random.default_rng(3)
+    n = 400
+    rows = np.repeat(np.arange(n), 5)
+    cols = np.concatenate([rng.choice(n, 5, replace=False) for _ in range(n)])
+    vals = -rng.uniform(0.1, 1.0, rows.size)
+    diag = np.zeros(n)
+    np.add.at(diag, rows, -vals)
+    keep = rows != cols
+    rows = np.concatenate([rows[keep], np.arange(n)])
+    cols = np.concatenate([cols[keep], np.arange(n)])
+    vals = np.concatenate([vals[keep], diag + 1.0])
+    order = np.lexsort((rng.random(rows.size), rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))])
+    a = ca.CsrMatrix.from_parts(vals, cols, rp, n, n)
+    assert a._order is not None  # the rows really were unsorted
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 20
+    amg = ca.AlgebraicMultigrid(a, cfg)
+    L = amg.statistics.num_levels
+    coarse0 = amg.level(L - 1, "A")[2].copy()
+    a.update_values(vals * 1.5)
+    amg.recompute(a)
+    assert np.array_equal(amg.level(0, "A")[2], a.values)
+    assert np.allclose(amg.level(L - 1, "A")[2], 1.5 * coarse0, rtol=1e-13, atol=0)
