@@ -39,16 +39,21 @@ __device__ __forceinline__ EpiIn<T> epi_load(int row, const T *__restrict__ x, c
   return e;
 }
 
+// The value an epilogue writes for row sum `sum`.
+template <typename T, int EPI>
+__device__ __forceinline__ T epi_out(const EpiIn<T> &e, T sum, T omega) {
+  if constexpr (EPI == EPI_Y || EPI == EPI_RJ0) return sum;
+  else if constexpr (EPI == EPI_RESID) return e.b - sum;
+  else if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) return e.x + omega * e.d * (e.b - sum);
+  else return e.y + sum;
+}
+
 // EPI_RJ0 also writes y2 = omega * dinv .* y (the next level's first Jacobi
 // iterate from zero, evaluated as k_jacobi_first does).
 template <typename T, int EPI, bool DOT>
 __device__ __forceinline__ void epi_store(const EpiIn<T> &e, int row, T sum, T omega, T *y, const T *dotw,
                                           double &dacc, T *y2 = nullptr) {
-  T out;
-  if constexpr (EPI == EPI_Y || EPI == EPI_RJ0) out = sum;
-  else if constexpr (EPI == EPI_RESID) out = e.b - sum;
-  else if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) out = e.x + omega * e.d * (e.b - sum);
-  else out = e.y + sum;
+  const T out = epi_out<T, EPI>(e, sum, omega);
   if constexpr (DOT) dacc += (double)(dotw == y ? out : e.w) * (double)out;
   y[row] = out;
   if constexpr (EPI == EPI_RJ0) y2[row] = omega * e.d * out;

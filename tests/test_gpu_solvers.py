@@ -45,18 +45,41 @@ def test_spmv_integer_bit_exact(row_len):
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
 
 
-@pytest.mark.parametrize("shape", [(37, 1, 1), (64, 64, 1), (33, 17, 9), (128, 128, 128)])
-def test_spmv_stencil_integer_bit_exact(shape):
-    # stencil matrices of >= 4096 rows take the diagonal kernel, smaller ones
-    # the row-block kernel; integer values and x make every partial sum exact
+def _integer_stencil(shape):
     nx, ny, nz = shape
     n = nx * ny * nz
     k = np.round(P.uniform(3, n, 1.0, 9.0))
     rp, ci, v, _ = P.varcoef(nx, ny, nz, 1.0, k, 0, np.zeros(6))
     v = np.round(v * 4.0)  # integer-valued, same pattern
     x = np.round(P.uniform(4, n, -20.0, 20.0))
+    return n, rp, ci, v, x
+
+
+# Shapes and the diagonal kernel they take (>= 4096 rows; smaller ones take
+# the row-block kernel).  The FP64 two-rows-per-lane kernel needs n and the
+# plane offsets nx, nx*ny to be even: (64, 64, 1), (32, 16, 12), (34, 30, 5)
+# and 128^3 take it, (33, 17, 9) does not.
+STENCIL_SHAPES = [(37, 1, 1), (64, 64, 1), (32, 16, 12), (34, 30, 5), (33, 17, 9), (128, 128, 128)]
+
+
+@pytest.mark.parametrize("shape", STENCIL_SHAPES)
+def test_spmv_stencil_integer_bit_exact(shape):
+    # integer values and x make every partial sum exact
+    n, rp, ci, v, x = _integer_stencil(shape)
     y = ca.CsrMatrix.from_parts(v, ci, rp, n, n).spmv(x)
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n).spmv(x))
+
+
+@pytest.mark.parametrize("shape", STENCIL_SHAPES[1:5])
+def test_spmv_stencil_fp32_integer_bit_exact(shape):
+    # |entries| <= 216 and |x| <= 20: every FP32 product and partial sum is an
+    # integer below 2^24, so the FP32 product is exact as well
+    import torch
+
+    n, rp, ci, v, x = _integer_stencil(shape)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    y32 = a.spmv(torch.from_numpy(x.astype(np.float32)).cuda()).cpu().numpy()
+    assert np.array_equal(y32.astype(np.float64), O.Csr.from_arrays(rp, ci, v, n).spmv(x))
 
 
 def test_dia_storage_selected():
@@ -179,9 +202,11 @@ AMG_CASES = [
 ]
 
 
+# nn = 20 runs the fine level on the two-rows-per-lane diagonal kernel,
+# nn = 19 on the one-row-per-lane kernel
+@pytest.mark.parametrize("nn", [20, 19])
 @pytest.mark.parametrize("case", AMG_CASES, ids=lambda c: f"{c['coarsening']}-{c['interp']}-{c['coarse']}")
-def test_amg_vcycle_matches_oracle(case):
-    nn = 20
+def test_amg_vcycle_matches_oracle(case, nn):
     n = nn ** 3
     k = P.config2_k(nn)
     rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), k, 1 << 4, np.zeros(6))
