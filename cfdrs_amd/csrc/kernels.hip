@@ -217,14 +217,13 @@ namespace {
 // solve, R 780 against 1008 us); the vector kernel won only FP32 level-2
 // sweeps (377 against 437 us).
 //
-template <typename T, int EPI, bool DOT>
+template <typename T, int EPI, bool DOT, int CAP>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const T *__restrict__ av,
                                                    const T *__restrict__ x, const T *__restrict__ b,
                                                    const T *__restrict__ dinv, T omega, T *y, const T *dotw,
                                                    Scal *s, int check_done, double *partials, T *y2) {
   if (check_done && s->done) return;
-  constexpr int CAP = rb_cap<T>();
   constexpr int U = CAP / kBlock;
   constexpr int RPT = kRbRows / kBlock;  // rows per thread when a block has more rows than threads
   __shared__ T prod[CAP];
@@ -339,9 +338,18 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
                    Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr) {
   const int nrb = A.nrb<T>();
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
-  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), nrb,
-                     reinterpret_cast<const int2 *>(A.rb<T>()), A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega,
-                     y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
+  const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
+  // An FP32 partition whose blocks all hold at most 2048 nonzeros (built so
+  // for short rows: the first prolongator) runs with 8 loads per thread
+  // instead of 16, half of them clamped re-reads otherwise.
+  if (sizeof(T) == 4 && A.rb32_max <= rb_cap<double>())
+    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, rb_cap<double>()>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb,
+                       A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
+                       ws.partials.p, y2);
+  else
+    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, rb_cap<T>()>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb,
+                       A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
+                       ws.partials.p, y2);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 
