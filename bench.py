@@ -8,10 +8,11 @@ A step is one solve from x = 0 with the matrix, hierarchy and right-hand side
 resident in HBM.  ``value`` is the FP32-inner / FP64-refined time-to-1e-10 in
 milliseconds; the plain FP64 time, their ratio and the SpMV bandwidth are
 reported beside it.  ``roofline`` is the FP64 y = A x of the fine operator
-in its device format (64-row sliced ELL with int16 column offsets for this
-stencil), timed with HIP events on the stream it is launched on, against the
-bytes that format must move; ``cpu_baseline`` is the CPU FP64 oracle solving
-the same system on one host core.
+in its device format (symmetric diagonal storage for this stencil: main and
+3 upper diagonals, no indices), timed with HIP events on the stream it is
+launched on, against the bytes that format must move; ``cpu_baseline`` is the
+CPU FP64 oracle solving the same system on the host's cores (up to 16
+threads), its one-thread time beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -33,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --spmv-only`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors
 # that make the counters read 512 MiB each for a 512 MiB device copy
 # (profiles/k_dia_spmv_varcoef128.md).  Keyed by the device format measured.
-SPMV_FP64_TRAFFIC_BYTES = {"dia": round((42327.151 * 2.0 + 16384.0) * 1024)}
+SPMV_FP64_TRAFFIC_BYTES = {"dia": round((42329.487 * 2.0 + 16384.0) * 1024)}
 
 
 def log(msg: str) -> None:

@@ -1,9 +1,9 @@
 // Device runtime, preconditioners and Krylov drivers of libcfdla.
 //
 // The Krylov recurrences are driven from the host but keep every scalar on
-// the device: reductions finish in the last block of the reducing kernel,
-// which updates alpha/beta/rho/omega and the convergence flag in a Scal
-// record.  The host only enqueues kernels and polls the flag one step
+// the device: a reducing kernel stores one partial per block, and a one-block
+// finalize kernel launched after it sums them in block order and updates
+// alpha/beta/rho/omega and the convergence flag in a Scal record.  The host only enqueues kernels and polls the flag one step
 // behind (a pinned copy of Scal plus an event per step), so the GPU never
 // waits for the host, and kernels of steps enqueued after convergence exit
 // immediately.
