@@ -257,7 +257,9 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
     dev = torch.device("cuda:0")
     mats = [a] + [ca.CsrMatrix.from_parts(v, ci, rp, n, n) for _ in range(args.spmv_copies - 1)]
     storage = a.storage()
-    kernel = {"dia": f"k_dia (symmetric diagonal storage, {storage['units']} stored diagonals x {n} rows)",
+    # FP64 takes the two-rows-per-lane kernel when n and the plane offsets are even
+    dia_kernel = "k_dia_vec, two rows per lane" if args.grid % 2 == 0 else "k_dia"
+    kernel = {"dia": f"{dia_kernel} (symmetric diagonal storage, {storage['units']} stored diagonals x {n} rows)",
               "sell64": f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)",
               "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)"}[storage["format"]]
     fn = N.lib().cfdla_spmv_device
