@@ -426,6 +426,10 @@ def secondary_configs(args) -> dict:
     batch = NW.channel_batch(nets)
     xb = torch.zeros(batch.n_rows, dtype=torch.float64, device=dev)
     netres = {"systems": nets.count, "rows": int(batch.n_rows)}
+    # per-system rows and nonzeros, for the bytes each CG iteration streams
+    net_rp = batch.export()[0]
+    net_n = np.diff(nets.node_offset).astype(np.float64)
+    net_nnz = (net_rp[nets.node_offset[1:]] - net_rp[nets.node_offset[:-1]]).astype(np.float64)
     for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED):
         cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(50000).with_precision(prec)
         # the row-bin schedule the autotuner picks for this size histogram
@@ -437,9 +441,22 @@ def secondary_configs(args) -> dict:
             return batch.solve(xb, cfg)[1]
 
         ms, reps = _time_solves(run, steps)
+        # Each team CG iteration re-reads its system's matrix (values, int32
+        # columns and row pointers) and the Jacobi and equilibration factors:
+        # nnz (s + 4) + 4 (n + 1) + n (s + 8) bytes, s = 8 (FP64) / 4 (FP32
+        # inner iterations); the FP32 form's FP64 residual replacements are
+        # not counted.  Most re-reads hit the L2 of the team's XCD, so the
+        # rate is an effective stream rate, quoted against the HBM peak.
+        sv = 8 if prec == ca.Precision.FP64 else 4
+        its = np.array([r.iterations for r in reps], np.float64)
+        stream = float(np.sum(its * (net_nnz * (sv + 4) + 4 * (net_n + 1) + net_n * (sv + 8))))
         netres[prec.name] = {"ms": round(ms, 3), "systems_per_s": round(nets.count * 1e3 / ms, 1),
                              "max_iterations": int(max(r.iterations for r in reps)),
+                             "mean_iterations": round(float(its.mean()), 1),
                              "all_converged": all(r.termination == ca.Termination.Converged for r in reps),
+                             "matrix_stream_gb": round(stream / 1e9, 2),
+                             "matrix_stream_gbs": round(stream / (ms * 1e-3) / 1e9, 1),
+                             "frac_of_hbm_peak": round(stream / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
                              "schedule": tuned["best"]}
     # design-loop refresh: widths -> conductances -> assembled values, right-
     # hand side and preconditioner factors, on the device against the host

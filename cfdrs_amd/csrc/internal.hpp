@@ -105,14 +105,15 @@ struct HostCsr {
 //
 // Every other matrix runs the row-block kernel: rows are packed in order into
 // blocks of at most kRbRows rows and rb_cap<T>() nonzeros (a longer row gets
-// a block of its own).  A partition per value type (the FP32 blocks hold
-// twice the nonzeros in the same LDS) is stored as nrb + 1 (first row, first
-// nonzero) pairs.
+// a block of its own).  A partition per value type is stored as nrb + 1
+// (first row, first nonzero) pairs.  Both types use 2048 nonzeros: FP32
+// blocks of 4096 (the same LDS bytes) ran the 128^3 refined solve 4 % slower
+// (5.10 against 4.88 ms), as their rows were summed by fewer threads.
 constexpr int16_t kSellPad = INT16_MIN;
 constexpr int kRbRows = 512;
 template <typename T>
 constexpr int rb_cap() {
-  return sizeof(T) == 8 ? 2048 : 4096;
+  return 2048;
 }
 // Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
 // one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
@@ -135,7 +136,6 @@ struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
   int nrb64 = 0, nrb32 = 0;
-  int rb32_max = 0;  // most nonzeros in one block of the FP32 partition
   int sell_chunk = 8;  // entries per unrolled step of the sliced-ELL kernel (8 or 16)
   DevBuf<int32_t> rp, ci, rb64, rb32;
   DevBuf<double> v64, sdinv64;

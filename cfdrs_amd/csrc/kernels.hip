@@ -198,14 +198,14 @@ int xcd_grid(int units, int units_per_block, int cap) {
 namespace {
 
 // ---- row-block CSR product ----------------------------------------------------
-// The host packs rows in order into blocks of at most kRbRows rows and kRbCap
+// The host packs rows in order into blocks of at most kRbRows rows and rb_cap
 // nonzeros.  A workgroup streams its block's nonzeros with coalesced,
-// independent loads (up to kRbCap / kBlock = 8 column/value pairs per thread,
+// independent loads (up to rb_cap / kBlock = 8 column/value pairs per thread,
 // all issued before the x gathers), parks the products in LDS, then sums each
 // row from LDS with 1..64 threads (the largest power of two that fits the
 // block's row count) and a shuffle tree.  Row lengths of 4..700 nonzeros (AMG
 // transfer operators and Galerkin products) all keep every lane busy with
-// 8 loads in flight.  A row longer than kRbCap is a block of its own and is
+// 8 loads in flight.  A row longer than rb_cap is a block of its own and is
 // summed block-strided straight from memory.
 //
 // Blocks are dealt to the 8 XCDs round-robin; XCD q walks the q-th
@@ -217,13 +217,14 @@ namespace {
 // solve, R 780 against 1008 us); the vector kernel won only FP32 level-2
 // sweeps (377 against 437 us).
 //
-template <typename T, int EPI, bool DOT, int CAP>
+template <typename T, int EPI, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const T *__restrict__ av,
                                                    const T *__restrict__ x, const T *__restrict__ b,
                                                    const T *__restrict__ dinv, T omega, T *y, const T *dotw,
                                                    Scal *s, int check_done, double *partials, T *y2) {
   if (check_done && s->done) return;
+  constexpr int CAP = rb_cap<T>();
   constexpr int U = CAP / kBlock;
   constexpr int RPT = kRbRows / kBlock;  // rows per thread when a block has more rows than threads
   __shared__ T prod[CAP];
@@ -338,18 +339,9 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
                    Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr) {
   const int nrb = A.nrb<T>();
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
-  const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
-  // An FP32 partition whose blocks all hold at most 2048 nonzeros (built so
-  // for short rows: the first prolongator) runs with 8 loads per thread
-  // instead of 16, half of them clamped re-reads otherwise.
-  if (sizeof(T) == 4 && A.rb32_max <= rb_cap<double>())
-    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, rb_cap<double>()>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb,
-                       A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
-                       ws.partials.p, y2);
-  else
-    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, rb_cap<T>()>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb,
-                       A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
-                       ws.partials.p, y2);
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), nrb,
+                     reinterpret_cast<const int2 *>(A.rb<T>()), A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega,
+                     y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 

@@ -289,13 +289,8 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
   if (want_fp32) {
-    // rows of at most 4 nonzeros on average fill 512-row blocks with ~2048
-    // nonzeros: those keep the FP64 cap, so the FP32 kernel issues 8 loads
-    // per thread instead of 16 (the first prolongator, 3.9 per row)
-    build_row_blocks(h, h.nnz() <= 4 * h.n ? rb_cap<double>() : rb_cap<float>(), blocks);
+    build_row_blocks(h, rb_cap<float>(), blocks);
     nrb32 = (int)blocks.size() / 2 - 1;
-    rb32_max = 0;
-    for (int q = 0; q < nrb32; ++q) rb32_max = std::max(rb32_max, blocks[2 * q + 3] - blocks[2 * q + 1]);
     rb32.upload(blocks.data(), blocks.size());
   }
   ci.upload(h.ci.data(), h.ci.size());
