@@ -136,7 +136,6 @@ struct DevCsr {
   int n = 0, m = 0;
   int64_t nnz = 0;
   int nrb64 = 0, nrb32 = 0;
-  int sell_chunk = 8;  // entries per unrolled step of the sliced-ELL kernel (8 or 16)
   DevBuf<int32_t> rp, ci, rb64, rb32;
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;

@@ -329,7 +329,6 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   if (sell) {
     nslices = (int)(sp.size() - 1);
     sell_entries = sp.back();
-    sell_chunk = (double)sell_entries > 12.0 * 64.0 * (double)nslices ? 16 : 8;
     sell_ptr.upload(sp.data(), sp.size());
     sell_delta.upload(sd.data(), sd.size());
     sell_v64.upload(sv.data(), sv.size());
