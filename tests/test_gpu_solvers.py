@@ -82,6 +82,44 @@ def test_spmv_stencil_fp32_integer_bit_exact(shape):
     assert np.array_equal(y32.astype(np.float64), O.Csr.from_arrays(rp, ci, v, n).spmv(x))
 
 
+def _banded_csr(n, lo, hi, band, seed, empty_every=0):
+    # rows of lo..hi integer entries within +-band of the diagonal (offsets fit
+    # int16, padding small enough for the sliced-ELL layout)
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        k = 0 if (empty_every and i % empty_every == 0) else int(rng.integers(lo, hi + 1))
+        c0, c1 = max(0, i - band), min(n, i + band + 1)
+        cols = np.sort(rng.choice(np.arange(c0, c1), size=min(k, c1 - c0), replace=False))
+        ci.extend(cols.tolist())
+        v.extend(rng.integers(-9, 10, size=cols.size).astype(float).tolist())
+        rp.append(len(ci))
+    return np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
+
+
+@pytest.mark.parametrize("n", [4096, 5003])
+def test_spmv_sell_integer_bit_exact(n):
+    # sliced ELL with lanes ordered by row length and bounds-checked loads over
+    # the padding: uneven rows (and, at 5003, empty rows and a short last
+    # slice), bit-exact in FP64 and FP32 on integer data; a non-finite x at a
+    # column a row does not hold must not reach that row through the padding
+    import torch
+
+    rp, ci, v = _banded_csr(n, 20, 30, 3000, seed=n, empty_every=97 if n % 2 else 0)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    assert a.storage()["format"] == "sell64"
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    x = np.round(P.uniform(9, n, -20.0, 20.0))
+    yo = orc.spmv(x)
+    assert np.array_equal(a.spmv(x), yo)
+    y32 = a.spmv(torch.from_numpy(x.astype(np.float32)).cuda()).cpu().numpy()
+    assert np.array_equal(y32.astype(np.float64), yo)
+    x[::7] = np.inf
+    y, yo = a.spmv(x), orc.spmv(x)
+    assert np.array_equal(np.isfinite(y), np.isfinite(yo))
+    assert np.array_equal(y[np.isfinite(yo)], yo[np.isfinite(yo)])
+
+
 def test_dia_storage_selected():
     # stencil operators take the diagonal layout: the symmetric 7-point
     # operator stores its main and 3 upper diagonals, the nonsymmetric
