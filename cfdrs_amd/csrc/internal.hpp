@@ -104,16 +104,22 @@ struct HostCsr {
 // epilogue accesses stop coalescing.)
 //
 // Every other matrix runs the row-block kernel: rows are packed in order into
-// blocks of at most kRbRows rows and rb_cap<T>() nonzeros (a longer row gets
+// blocks of at most rb_rows<T>() rows and rb_cap<T>() nonzeros (a longer row gets
 // a block of its own).  A partition per value type is stored as nrb + 1
 // (first row, first nonzero) pairs.  Both types use 2048 nonzeros: FP32
 // blocks of 4096 (the same LDS bytes) ran the 128^3 refined solve 4 % slower
 // (5.10 against 4.88 ms), as their rows were summed by fewer threads.
 constexpr int16_t kSellPad = INT16_MIN;
-constexpr int kRbRows = 512;
 template <typename T>
 constexpr int rb_cap() {
   return 2048;
+}
+// Rows per block: 512 in FP64, 256 in FP32 (measured on the 128^3 solves:
+// 256-row FP32 blocks 4.79-4.81 against 4.81-4.84 ms over three pairs of
+// runs, 256-row FP64 ones 6.73 against 6.69 ms).
+template <typename T>
+constexpr int rb_rows() {
+  return sizeof(T) == 8 ? 512 : 256;
 }
 // Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
 // one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
@@ -183,7 +189,7 @@ template <> inline const double *DevCsr::dia_vals<double>() const { return dia_v
 template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32.p; }
 
 
-void build_row_blocks(const HostCsr &h, int cap, std::vector<int32_t> &rb);
+void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb);
 
 // Device-resident scalars of one solve.  A reducing kernel writes block
 // partials; the single-block finalize kernel launched after it sums them into

@@ -198,7 +198,7 @@ int xcd_grid(int units, int units_per_block, int cap) {
 namespace {
 
 // ---- row-block CSR product ----------------------------------------------------
-// The host packs rows in order into blocks of at most kRbRows rows and rb_cap
+// The host packs rows in order into blocks of at most rb_rows rows and rb_cap
 // nonzeros.  A workgroup streams its block's nonzeros with coalesced,
 // independent loads (up to rb_cap / kBlock = 8 column/value pairs per thread,
 // all issued before the x gathers), parks the products in LDS, then sums each
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   if (check_done && s->done) return;
   constexpr int CAP = rb_cap<T>();
   constexpr int U = CAP / kBlock;
-  constexpr int RPT = kRbRows / kBlock;  // rows per thread when a block has more rows than threads
+  constexpr int RPT = rb_rows<T>() / kBlock;  // rows per thread when a block has more rows than threads
   __shared__ T prod[CAP];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;

@@ -132,12 +132,12 @@ struct Poller {
 // ---------------------------------------------------------------------------
 // Device CSR
 // ---------------------------------------------------------------------------
-void build_row_blocks(const HostCsr &h, int cap, std::vector<int32_t> &rb) {
+void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb) {
   rb.assign({0, 0});
   int64_t rows = 0, nz = 0;
   for (int64_t r = 0; r < h.n; ++r) {
     const int64_t len = h.rp[r + 1] - h.rp[r];
-    if (rows > 0 && (nz + len > cap || rows == kRbRows)) {
+    if (rows > 0 && (nz + len > cap || rows == max_rows)) {
       rb.push_back((int32_t)r);
       rb.push_back((int32_t)h.rp[r]);
       rows = nz = 0;
@@ -285,11 +285,11 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   // (Smaller caps for small matrices, to reach 8 blocks per CU, measured
   // slower on every AMG level of the 128^3 problem: 7.6 -> 10.4 us per level-2
   // sweep; the per-block descriptor loads and barriers dominate.)
-  build_row_blocks(h, rb_cap<double>(), blocks);
+  build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
   if (want_fp32) {
-    build_row_blocks(h, rb_cap<float>(), blocks);
+    build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks);
     nrb32 = (int)blocks.size() / 2 - 1;
     rb32.upload(blocks.data(), blocks.size());
   }
