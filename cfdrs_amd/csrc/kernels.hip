@@ -217,9 +217,11 @@ namespace {
 // solve, R 780 against 1008 us); the vector kernel won only FP32 level-2
 // sweeps (377 against 437 us).
 //
-template <typename T, int EPI, bool DOT>
+// C16: columns are 16-bit offsets from the block's base column (cb[q]).
+template <typename T, int EPI, bool DOT, bool C16>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
-                                                   const int *__restrict__ ci, const T *__restrict__ av,
+                                                   const int *__restrict__ ci, const uint16_t *__restrict__ c16,
+                                                   const int *__restrict__ cb, const T *__restrict__ av,
                                                    const T *__restrict__ x, const T *__restrict__ b,
                                                    const T *__restrict__ dinv, T omega, T *y, const T *dotw,
                                                    Scal *s, int check_done, double *partials, T *y2) {
@@ -237,23 +239,32 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   int q = xcd * chunk + local;
   // block descriptors (first row, first nonzero) of this and the next block
   int2 d0 = make_int2(0, 0), d1 = make_int2(0, 0);
+  int b0 = 0;
   if (q < q1) {
     d0 = rb[q];
     d1 = rb[q + 1];
+    if constexpr (C16) b0 = cb[q];
   }
+  // column of nonzero k of this block
+  auto col = [&](int k) -> int {
+    if constexpr (C16) return b0 + (int)c16[k];
+    else return ci[k];
+  };
   while (q < q1) {
     const int qn = q + per;
     int2 n0 = d0, n1 = d1;
+    int nb = b0;
     if (qn < q1) {  // prefetched while this block is processed
       n0 = rb[qn];
       n1 = rb[qn + 1];
+      if constexpr (C16) nb = cb[qn];
     }
     const int r0 = d0.x, e0 = d0.y, nr = d1.x - d0.x, ne = d1.y - d0.y;
     if (ne > CAP) {
       // a single long row, summed block-strided from memory
       const EpiIn<T> e = epi_load<T, EPI, DOT>(r0, x, b, dinv, omega, y, dotw);
       T acc = T(0);
-      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * gather<T, EPI>(ci[e0 + k], x, b, dinv, omega);
+      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * gather<T, EPI>(col(e0 + k), x, b, dinv, omega);
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
       if ((tid & 63) == 0) wsum[tid >> 6] = acc;
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int k = min(tid + u * kBlock, ne - 1);
-          c[u] = ci[e0 + k];
+          c[u] = col(e0 + k);
           v[u] = av[e0 + k];
         }
 #pragma unroll
@@ -326,6 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
     }
     d0 = n0;
     d1 = n1;
+    b0 = nb;
     q = qn;
   }
   if constexpr (DOT) {
@@ -339,9 +351,15 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
                    Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr) {
   const int nrb = A.nrb<T>();
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
-  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT>), dim3(blocks), dim3(kBlock), 0, stream(), nrb,
-                     reinterpret_cast<const int2 *>(A.rb<T>()), A.rp.p, A.ci.p, A.vals<T>(), x, b, dinv, (T)omega,
-                     y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
+  const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
+  if (A.c16)
+    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, true>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb, A.rp.p,
+                       nullptr, A.cols16<T>(), A.cbase<T>(), A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
+                       check_done ? 1 : 0, ws.partials.p, y2);
+  else
+    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, false>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb, A.rp.p,
+                       A.ci.p, nullptr, nullptr, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
+                       ws.partials.p, y2);
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 

@@ -152,6 +152,25 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
 }
 
 namespace {
+// 16-bit column offsets for a row-block partition: base[q] = the smallest
+// column of block q, c16[k] = ci[k] - base[q].  False when some block's
+// columns span more than 65536.
+bool build_cols16(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<uint16_t> &c16,
+                  std::vector<int32_t> &base) {
+  const size_t nrb = rb.size() / 2 - 1;
+  c16.assign(h.ci.size(), 0);
+  base.assign(nrb, 0);
+  for (size_t q = 0; q < nrb; ++q) {
+    const int64_t e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
+    if (e1 <= e0) continue;
+    const auto mm = std::minmax_element(h.ci.begin() + e0, h.ci.begin() + e1);
+    if ((int64_t)*mm.second - (int64_t)*mm.first > 65535) return false;
+    base[q] = *mm.first;
+    for (int64_t k = e0; k < e1; ++k) c16[(size_t)k] = (uint16_t)(h.ci[(size_t)k] - *mm.first);
+  }
+  return true;
+}
+
 void smoother_dinv(const HostCsr &h, std::vector<double> &d) {
   d.assign((size_t)h.n, 0.0);
   for (int64_t i = 0; i < h.n; ++i)
@@ -285,13 +304,30 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   // (Smaller caps for small matrices, to reach 8 blocks per CU, measured
   // slower on every AMG level of the 128^3 problem: 7.6 -> 10.4 us per level-2
   // sweep; the per-block descriptor loads and barriers dominate.)
+  std::vector<uint16_t> c16v;
+  std::vector<int32_t> cb;
   build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
+  c16 = build_cols16(h, blocks, c16v, cb);
+  if (c16) {
+    c16_64.upload(c16v.data(), c16v.size());
+    cbase64.upload(cb.data(), cb.size());
+    sync_stream();
+  }
   if (want_fp32) {
     build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks);
     nrb32 = (int)blocks.size() / 2 - 1;
     rb32.upload(blocks.data(), blocks.size());
+    std::vector<uint16_t> c16v32;
+    std::vector<int32_t> cb32;
+    if (c16 && build_cols16(h, blocks, c16v32, cb32)) {
+      c16_32.upload(c16v32.data(), c16v32.size());
+      cbase32.upload(cb32.data(), cb32.size());
+      sync_stream();
+    } else {
+      c16 = false;
+    }
   }
   ci.upload(h.ci.data(), h.ci.size());
   v64.upload(h.v.data(), h.v.size());

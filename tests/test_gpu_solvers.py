@@ -36,9 +36,12 @@ def random_csr(n, m, row_len_max, seed, integer=False, empty_every=0):
     return np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
 
 
+# m = 2777 keeps every row block's columns within 65536 (16-bit column
+# offsets from a per-block base); m = 90001 does not (32-bit columns)
+@pytest.mark.parametrize("m", [2777, 90001])
 @pytest.mark.parametrize("row_len", [1, 3, 7, 16, 40, 130])
-def test_spmv_integer_bit_exact(row_len):
-    n, m = 3001, 2777
+def test_spmv_integer_bit_exact(row_len, m):
+    n = 3001
     rp, ci, v = random_csr(n, m, row_len, seed=row_len, integer=True, empty_every=97)
     x = np.random.default_rng(1).integers(-50, 51, size=m).astype(float)
     y = ca.CsrMatrix.from_parts(v, ci, rp, n, m).spmv(x)
