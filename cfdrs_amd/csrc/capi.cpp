@@ -606,6 +606,8 @@ cfdla_status cfdla_solve_transpose_device(cfdla_matrix m, cfdla_precond p, const
   return guarded([&] {
     require_ptr(m, "matrix");
     require_ptr(config, "config");
+    CFDLA_REQUIRE(m->host.n == m->host.m, CFDLA_ERR_DIMENSION, "transpose solves need a square matrix");
+    CFDLA_REQUIRE(!p || p->m->host.n == m->host.n, CFDLA_ERR_DIMENSION, "preconditioner size differs from matrix");
     SolveOut o;
     if (is_symmetric(m)) {
       o = solve_device(devmat(m), p ? p->p.get() : nullptr, g_dev, lambda_dev, *config);

@@ -144,9 +144,10 @@ struct DevCsr {
   int nrb64 = 0, nrb32 = 0;
   DevBuf<int32_t> rp, ci, rb64, rb32;
   // Row-block columns as 16-bit offsets from a per-block base (the block's
-  // smallest column), one copy per partition, when every block's columns
-  // span at most 65536: 2 bytes per nonzero instead of 4.
-  bool c16 = false;
+  // smallest column), one copy per partition (FP64 / FP32 row blocks), when
+  // every block of that partition has columns spanning at most 65535: 2 bytes
+  // per nonzero instead of 4.  Each partition decides on its own.
+  bool c16_f64 = false, c16_f32 = false;
   DevBuf<uint16_t> c16_64, c16_32;
   DevBuf<int32_t> cbase64, cbase32;
   DevBuf<double> v64, sdinv64;
@@ -184,6 +185,7 @@ struct DevCsr {
   template <typename T> const T *dia_vals() const;
   template <typename T> const int *rb() const { return sizeof(T) == 8 ? rb64.p : rb32.p; }
   template <typename T> int nrb() const { return sizeof(T) == 8 ? nrb64 : nrb32; }
+  template <typename T> bool has_cols16() const { return sizeof(T) == 8 ? c16_f64 : c16_f32; }
   template <typename T> const uint16_t *cols16() const { return sizeof(T) == 8 ? c16_64.p : c16_32.p; }
   template <typename T> const int *cbase() const { return sizeof(T) == 8 ? cbase64.p : cbase32.p; }
 };

@@ -352,7 +352,7 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
   const int nrb = A.nrb<T>();
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
   const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
-  if (A.c16)
+  if (A.has_cols16<T>())
     hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, true>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb, A.rp.p,
                        nullptr, A.cols16<T>(), A.cbase<T>(), A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
                        check_done ? 1 : 0, ws.partials.p, y2);

@@ -153,8 +153,8 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
 
 namespace {
 // 16-bit column offsets for a row-block partition: base[q] = the smallest
-// column of block q, c16[k] = ci[k] - base[q].  False when some block's
-// columns span more than 65536.
+// column of block q, c16[k] = ci[k] - base[q].  False when some block has
+// max - min > 65535 over its columns.
 bool build_cols16(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<uint16_t> &c16,
                   std::vector<int32_t> &base) {
   const size_t nrb = rb.size() / 2 - 1;
@@ -309,25 +309,30 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
-  c16 = build_cols16(h, blocks, c16v, cb);
-  if (c16) {
+  c16_f64 = build_cols16(h, blocks, c16v, cb);
+  if (c16_f64) {
     c16_64.upload(c16v.data(), c16v.size());
     cbase64.upload(cb.data(), cb.size());
     sync_stream();
+  } else {
+    c16_64.release();
+    cbase64.release();
   }
+  c16_f32 = false;
   if (want_fp32) {
     build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks);
     nrb32 = (int)blocks.size() / 2 - 1;
     rb32.upload(blocks.data(), blocks.size());
-    std::vector<uint16_t> c16v32;
-    std::vector<int32_t> cb32;
-    if (c16 && build_cols16(h, blocks, c16v32, cb32)) {
-      c16_32.upload(c16v32.data(), c16v32.size());
-      cbase32.upload(cb32.data(), cb32.size());
+    c16_f32 = build_cols16(h, blocks, c16v, cb);
+    if (c16_f32) {
+      c16_32.upload(c16v.data(), c16v.size());
+      cbase32.upload(cb.data(), cb.size());
       sync_stream();
-    } else {
-      c16 = false;
     }
+  }
+  if (!c16_f32) {
+    c16_32.release();
+    cbase32.release();
   }
   ci.upload(h.ci.data(), h.ci.size());
   v64.upload(h.v.data(), h.v.size());

@@ -21,7 +21,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .sparse import CsrMatrix, _is_torch, bind_torch_stream
+from .sparse import CsrMatrix, _is_torch, bind_torch_stream, require_device_vectors
 
 log = logging.getLogger("cfdrs_amd")
 
@@ -146,12 +146,11 @@ def _solve(matrix: CsrMatrix, rhs, preconditioner, solution, config: IterativeSo
         if _is_torch(solution):
             import torch
 
+            require_device_vectors(rhs, solution)
             if solution.dtype != torch.float64 or rhs.dtype != torch.float64:
                 raise N.CfdlaError(N.ERR_INVALID_ARG, "device solves take float64 tensors")
             if solution.numel() != n or rhs.numel() != n:
                 raise N.CfdlaError(N.ERR_DIMENSION, "vector length differs from matrix dimension")
-            if not (solution.is_contiguous() and rhs.is_contiguous()):
-                raise N.CfdlaError(N.ERR_INVALID_ARG, "vectors must be contiguous")
             bind_torch_stream()
             fn = "cfdla_solve_transpose_device" if transpose else "cfdla_solve_device"
             N.call(fn, matrix.handle, prec, rhs.data_ptr(), solution.data_ptr(), C.byref(cfg), C.byref(rep))

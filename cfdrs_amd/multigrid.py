@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
-from .sparse import CsrMatrix, _is_torch, bind_torch_stream
+from .sparse import CsrMatrix, _is_torch, bind_torch_stream, require_device_vectors
 
 
 class CoarseningStrategy(enum.IntEnum):
@@ -136,8 +136,11 @@ class _Preconditioner:
         if _is_torch(r):
             import torch
 
-            dt = {torch.float64: N.DT_F64, torch.float32: N.DT_F32}[r.dtype]
-            if z.dtype != r.dtype or r.numel() != self.n or z.numel() != self.n:
+            require_device_vectors(r, z)
+            dt = {torch.float64: N.DT_F64, torch.float32: N.DT_F32}.get(r.dtype)
+            if dt is None or z.dtype != r.dtype:
+                raise N.CfdlaError(N.ERR_INVALID_ARG, "r and z must be float64 or float32 tensors of one dtype")
+            if r.numel() != self.n or z.numel() != self.n:
                 raise N.CfdlaError(N.ERR_DIMENSION, "AMG preconditioner vector length mismatch")
             bind_torch_stream()
             N.call("cfdla_precond_apply_device", self.handle, dt, r.data_ptr(), z.data_ptr())

@@ -19,6 +19,19 @@ def _is_torch(a) -> bool:
     return hasattr(a, "data_ptr") and hasattr(a, "is_cuda")
 
 
+def require_device_vectors(*tensors) -> None:
+    """Every tensor handed to a *_device entry point must be a contiguous
+    CUDA tensor: a host pointer or a strided view would be read or written
+    by the kernels as if it were a dense device array."""
+    for t in tensors:
+        if not _is_torch(t):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "device entry points take torch tensors")
+        if not t.is_cuda:
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "device entry points take CUDA tensors, got a CPU tensor")
+        if not t.is_contiguous():
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "device vectors must be contiguous")
+
+
 def bind_torch_stream() -> None:
     """Issue library work on torch's current HIP stream."""
     import torch
@@ -135,11 +148,12 @@ class CsrMatrix:
             return
         import torch
 
+        require_device_vectors(x, y)
         if x.numel() != self.ncols or y.numel() != self.nrows:
             raise N.CfdlaError(N.ERR_DIMENSION, "vector length differs from matrix shape")
-        dt = {torch.float64: N.DT_F64, torch.float32: N.DT_F32}[x.dtype]
-        if y.dtype != x.dtype or not (x.is_contiguous() and y.is_contiguous()):
-            raise N.CfdlaError(N.ERR_INVALID_ARG, "x and y must be contiguous and of one dtype")
+        dt = {torch.float64: N.DT_F64, torch.float32: N.DT_F32}.get(x.dtype)
+        if dt is None or y.dtype != x.dtype:
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "x and y must be float64 or float32 tensors of one dtype")
         bind_torch_stream()
         N.call("cfdla_spmv_device", self.handle, dt, x.data_ptr(), y.data_ptr())
 

@@ -276,10 +276,13 @@ cfdla_status cfdla_solve(cfdla_matrix m, cfdla_precond p, const double *b, doubl
 cfdla_status cfdla_solve_device(cfdla_matrix m, cfdla_precond p, const double *b_dev,
                                 double *x_dev, const cfdla_solver_config *config,
                                 cfdla_report *report);
-/* Adjoint solve A^T lambda = g (device FP64).  For a symmetric matrix the
- * forward preconditioner is reused; otherwise the transpose is formed once
- * and cached with an identity/Jacobi preconditioner.  No reference
- * counterpart (the reference has no adjoint code). */
+/* Adjoint solve A^T lambda = g (device FP64), m square.  p (nullable) must
+ * have been built for a matrix of m's size (else CFDLA_ERR_DIMENSION, before
+ * any device work).  For a symmetric matrix p is reused as the preconditioner
+ * of A^T = A; otherwise p is not used: the transpose is formed once on the
+ * host, cached until m's values change, and solved with its own Jacobi
+ * preconditioner.  No reference counterpart (the reference has no adjoint
+ * code). */
 cfdla_status cfdla_solve_transpose_device(cfdla_matrix m, cfdla_precond p, const double *g_dev,
                                           double *lambda_dev, const cfdla_solver_config *config,
                                           cfdla_report *report);

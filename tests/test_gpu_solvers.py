@@ -48,6 +48,41 @@ def test_spmv_integer_bit_exact(row_len, m):
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
 
 
+# Row blocks hold at most 2048 nonzeros and 512 (FP64) / 256 (FP32) rows, so
+# with 5 entries per row the FP64 blocks are rows [0, 409), [409, 818), ...
+# and the FP32 blocks rows [0, 256), [256, 512), ...  A partition stores 16-bit
+# column offsets when every block's columns satisfy max - min <= 65535.
+#   span65535: every row holds columns {0, 7, 1000, 30000, 65535}: 16-bit in
+#     both partitions, with the largest offset that fits;
+#   span65536: the last column is 65536: 32-bit in both;
+#   fp32_straddles: rows below 409 hold columns 0..4 and rows from 409 on
+#     70000..70004, so every FP64 block spans 4 (16-bit) while the FP32 block
+#     [256, 512) spans 70004 (32-bit FP32 partition only).
+@pytest.mark.parametrize("case", ["span65535", "span65536", "fp32_straddles"])
+def test_spmv_16bit_column_span_boundary(case):
+    import torch
+
+    n = 818 if case == "fp32_straddles" else 1500
+    if case == "fp32_straddles":
+        cols = [np.arange(5) + (0 if i < 409 else 70000) for i in range(n)]
+        m = 70005
+    else:
+        last = 65535 if case == "span65535" else 65536
+        cols = [np.array([0, 7, 1000, 30000, last])] * n
+        m = last + 1
+    rng = np.random.default_rng(len(case))
+    ci = np.concatenate(cols).astype(np.int64)
+    v = rng.integers(-9, 10, size=ci.size).astype(float)
+    rp = np.arange(0, ci.size + 1, 5, dtype=np.int64)
+    x = rng.integers(-50, 51, size=m).astype(float)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
+    assert a.storage()["format"] == "csr_rowblock"
+    yo = O.Csr.from_arrays(rp, ci, v, n, m).spmv(x)
+    assert np.array_equal(a.spmv(x), yo)
+    y32 = a.spmv(torch.from_numpy(x.astype(np.float32)).cuda()).cpu().numpy()
+    assert np.array_equal(y32.astype(np.float64), yo)
+
+
 def _integer_stencil(shape):
     nx, ny, nz = shape
     n = nx * ny * nz

@@ -256,3 +256,65 @@ def test_amg_recompute_with_unsorted_matrix_passes_creation_order():
     amg.recompute(a)
     assert np.array_equal(amg.level(0, "A")[2], a.values)
     assert np.allclose(amg.level(L - 1, "A")[2], 1.5 * coarse0, rtol=1e-13, atol=0)
+
+
+def test_solve_entry_points_reject_mismatched_preconditioner():
+    # cfdla_solve_device and cfdla_solve_transpose_device check the preconditioner's size before
+    # any device work: an AMG built for 16 rows against a 64-row matrix is a dimension error
+    rp, ci, v, _ = P.config1(8)
+    m = ca.CsrMatrix.from_parts(v, ci, rp, 64, 64)
+    rp2, ci2, v2, _ = P.config1(4)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 4
+    amg = ca.AlgebraicMultigrid(ca.CsrMatrix.from_parts(v2, ci2, rp2, 16, 16), cfg)
+    sc = ca.IterativeSolverConfig.new(1e-10).native(N.KIND_CG)
+    rep = N.Report()
+    for fn in ("cfdla_solve_device", "cfdla_solve_transpose_device"):
+        st = getattr(N.lib(), fn)(m.handle, amg.handle, None, None, C.byref(sc), C.byref(rep))
+        assert st == N.ERR_DIMENSION, fn
+        assert b"preconditioner size differs" in N.lib().cfdla_last_error()
+
+
+class _FakeTensor:
+    """Quacks like a torch tensor for the boundary checks; any native call
+    through data_ptr() would fail the test."""
+
+    def __init__(self, n, is_cuda=True, contiguous=True, dtype=None):
+        import torch
+
+        self.n, self.is_cuda, self._contig = n, is_cuda, contiguous
+        self.dtype = dtype or torch.float64
+        self.device = "cuda:0" if is_cuda else "cpu"
+
+    def numel(self):
+        return self.n
+
+    def is_contiguous(self):
+        return self._contig
+
+    def data_ptr(self):
+        raise AssertionError("a native call was about to be made")
+
+
+@pytest.mark.parametrize("bad", ["cpu", "strided"])
+def test_device_entry_points_reject_cpu_and_strided_tensors(bad):
+    # a CPU tensor or a strided view must never reach cfdla_solve_device, cfdla_spmv_device or
+    # cfdla_precond_apply_device: the kernels would read host memory or a dense stride
+    rp, ci, v, _ = P.config1(8)
+    m = ca.CsrMatrix.from_parts(v, ci, rp, 64, 64)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 4
+    amg = ca.AlgebraicMultigrid(m, cfg)
+    good = _FakeTensor(64)
+    wrong = _FakeTensor(64, is_cuda=bad != "cpu", contiguous=bad != "strided")
+    conf = ca.IterativeSolverConfig.new(1e-10)
+    for args in ((wrong, good), (good, wrong)):
+        with pytest.raises(ca.SolveError) as e:
+            ca.cg_preconditioned(m, args[0], amg, args[1], conf)
+        assert e.value.status == N.ERR_INVALID_ARG
+        with pytest.raises(N.CfdlaError) as e:
+            m.spmv_into(*args)
+        assert e.value.status == N.ERR_INVALID_ARG
+        with pytest.raises(N.CfdlaError) as e:
+            amg.apply_to(*args)
+        assert e.value.status == N.ERR_INVALID_ARG
