@@ -89,6 +89,7 @@ SIGNATURES = {
     "cfdla_matrix_get_values": (C.c_int, [_VP, _VP]),
     "cfdla_matrix_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
     "cfdla_matrix_storage": (C.c_int, [_VP, _I32P, _I64P, _I64P]),
+    "cfdla_matrix_set_storage": (C.c_int, [_VP, C.c_int32]),
     "cfdla_matrix_destroy": (C.c_int, [_VP]),
     "cfdla_spmv": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_spmv_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),

@@ -360,6 +360,18 @@ cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stor
   });
 }
 
+cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    CFDLA_REQUIRE(format == CFDLA_STORAGE_AUTO || format == CFDLA_STORAGE_CSR_ROWBLOCK ||
+                      format == CFDLA_STORAGE_SELL64 || format == CFDLA_STORAGE_DIA,
+                  CFDLA_ERR_INVALID_ARG, "unknown storage format");
+    if (m->dev.fmt_request == format) return;
+    m->dev.fmt_request = format;
+    if (m->on_device) m->dev.upload(host_values(m), true, false);
+  });
+}
+
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m) {
   return guarded([&] { delete m; });
 }

@@ -164,6 +164,10 @@ struct DevCsr {
   DiaDesc dia_desc;
   DevBuf<double> dia_v64;
   DevBuf<float> dia_v32;
+  // Device format requested by the caller (CFDLA_STORAGE_*): -1 picks the
+  // format from the matrix (diagonal, then sliced ELL, then row-block CSR);
+  // a fixed one skips the others, for measuring one product kernel.
+  int fmt_request = -1;
   void upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow_sell = true);
   // Device value refresh.  v64 (CSR order) is the truth; build_maps (once,
   // from the host pattern) records for every CSR slot its position in the

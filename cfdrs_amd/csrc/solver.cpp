@@ -352,7 +352,9 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     sdinv32.upload(df.data(), df.size());
   }
   std::vector<double> dvals;
-  dia = allow_sell && build_dia(h, dia_desc, dvals, !general_dia);
+  const bool want_dia = fmt_request == -1 || fmt_request == CFDLA_STORAGE_DIA;
+  const bool want_sell = fmt_request == -1 || fmt_request == CFDLA_STORAGE_SELL64;
+  dia = allow_sell && want_dia && build_dia(h, dia_desc, dvals, !general_dia);
   maps_ready = false;
   if (dia) {
     dia_v64.upload(dvals.data(), dvals.size());
@@ -363,10 +365,20 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
       sync_stream();
     }
   }
+  if (!dia) {
+    dia_v64.release();
+    dia_v32.release();
+  }
   std::vector<int32_t> sp;
   std::vector<int16_t> sd;
   std::vector<double> sv;
-  sell = allow_sell && !dia && build_sell(h, sp, sd, sv);
+  sell = allow_sell && want_sell && !dia && build_sell(h, sp, sd, sv);
+  if (!sell) {
+    sell_ptr.release();
+    sell_delta.release();
+    sell_v64.release();
+    sell_v32.release();
+  }
   if (sell) {
     nslices = (int)(sp.size() - 1);
     sell_entries = sp.back();

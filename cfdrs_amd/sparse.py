@@ -167,6 +167,17 @@ class CsrMatrix:
         return {"format": {0: "csr_rowblock", 1: "sell64", 2: "dia"}[fmt.value], "stored_entries": ent.value,
                 "units": units.value}
 
+    STORAGE_FORMATS = {"auto": -1, "csr_rowblock": 0, "sell64": 1, "dia": 2}
+
+    def set_storage(self, fmt: str) -> dict:
+        """Restrict the device format ("auto", "csr_rowblock", "sell64",
+        "dia"); a format the matrix cannot take falls back to row-block CSR.
+        Returns storage().  Meant for timing one product kernel."""
+        if fmt not in self.STORAGE_FORMATS:
+            raise ValueError(f"unknown storage format {fmt!r}")
+        N.call("cfdla_matrix_set_storage", self.handle, self.STORAGE_FORMATS[fmt])
+        return self.storage()
+
     def diagonal(self):
         d = np.zeros(self.nrows)
         for i in range(self.nrows):

@@ -204,6 +204,15 @@ cfdla_status cfdla_matrix_info(cfdla_matrix m, int64_t *nrows, int64_t *ncols, i
 #define CFDLA_STORAGE_SELL64 1
 #define CFDLA_STORAGE_DIA 2
 cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stored_entries, int64_t *units);
+/* Restrict the device format of m to one CFDLA_STORAGE_* value, or
+ * CFDLA_STORAGE_AUTO (the default) to let the matrix pick.  A format the
+ * matrix cannot take (DIA for a non-stencil matrix, SELL64 for rows too long
+ * or offsets beyond int16) falls back to CSR_ROWBLOCK; cfdla_matrix_storage
+ * reports the result.  No counterpart in the reference: it exists to time
+ * one product kernel (for example the row-block CSR product on a stencil
+ * matrix that would otherwise use diagonal storage). */
+#define CFDLA_STORAGE_AUTO (-1)
+cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format);
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
 
 /* y = A x on host vectors.  Replaces cfd_math::sparse::spmv / try_spmv
