@@ -510,13 +510,40 @@ void BatchSolver::upload() {
   on_device = true;
 }
 
-void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *reports, const double *rhs_dev) {
+namespace {
+// ---- scratch of the allocation-free batched solve ---------------------------
+// One 256-byte slot for the work-queue counter, then report staging for at
+// most kNoAllocMaxSystems systems: 256 + 16384 x 48 bytes (768.25 KiB), held
+// by every GPU the library runs on until the process ends.  Host code reaches
+// it only through the address hipGetSymbolAddress returns for the current
+// device (g_scratch), never through the array's name.
+constexpr int64_t kNoAllocMaxSystems = 16384;
+constexpr size_t kNoAllocQueueBytes = 256;
+constexpr size_t kNoAllocBytes = kNoAllocQueueBytes + (size_t)kNoAllocMaxSystems * sizeof(BatchReport);
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+void *g_scratch[kScratchDevices] = {};
+
+void check_batch_config(const cfdla_solver_config &cfg) {
   CFDLA_REQUIRE(cfg.kind == CFDLA_CG, CFDLA_ERR_CONFIG, "batched networks are solved with CG (SPD Laplacians)");
   CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
                 "unknown precision");
   CFDLA_REQUIRE(cfg.max_iter >= 1 && cfg.max_iter < INT32_MAX, CFDLA_ERR_CONFIG, "max_iter out of range");
-  if (!on_device) upload();
-  if (nsys == 0) return;
+}
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  CFDLA_REQUIRE(device >= 0 && device < kScratchDevices, CFDLA_ERR_DEVICE,
+                "device index beyond the allocation-free solve's table");
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  CFDLA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CFDLA_REQUIRE(p != nullptr, CFDLA_ERR_DEVICE, "no device address for the allocation-free solve's scratch");
+  g_scratch[device] = p;
+}
+
+void BatchSolver::launch(double *x, const cfdla_solver_config &cfg, const double *rhs_dev, unsigned *queue,
+                         BatchReport *rep) {
   BArgs a;
   a.sys_row = d_sys_row.p;
   a.rp = d_rp.p;
@@ -528,9 +555,9 @@ void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *
   a.v32 = d_v32.p;
   a.jd32 = d_jd32.p;
   a.x = x;
-  a.rep = d_rep.p;
+  a.rep = rep;
   a.items = d_items.p;
-  a.queue = d_queue.p;
+  a.queue = queue;
   a.n_block_items = n_block_items;
   a.n_wave_items = n_wave_items;
   a.n_wave_sys = n_wave_sys;
@@ -543,11 +570,38 @@ void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *
   a.reliable = cfg.reliable_interval >= 0 ? 1 : 0;  // a replacement interval does not apply per team: delta only
   a.delta = cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1;
   const int grid = std::max(1, sched.blocks_per_cu) * num_cus();
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream(), d_queue.p);
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream(), queue);
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBlock), 0, stream(), a);
   CFDLA_HIP(hipGetLastError());
+}
+
+void BatchSolver::solve(double *x, const cfdla_solver_config &cfg, BatchReport *reports, const double *rhs_dev) {
+  check_batch_config(cfg);
+  if (!on_device) upload();
+  if (nsys == 0) return;
+  launch(x, cfg, rhs_dev, d_queue.p, d_rep.p);
   if (reports) {
     CFDLA_HIP(hipMemcpyAsync(reports, d_rep.p, (size_t)nsys * sizeof(BatchReport), hipMemcpyDeviceToHost, stream()));
+  }
+  sync_stream();
+}
+
+void BatchSolver::solve_noalloc(double *x, const cfdla_solver_config &cfg, BatchReport *reports,
+                                const double *rhs_dev) {
+  check_batch_config(cfg);
+  CFDLA_REQUIRE(on_device, CFDLA_ERR_CONFIG,
+                "batch is not resident on the device yet (its first cfdla_batch_solve_device uploads it)");
+  CFDLA_REQUIRE(nsys <= kNoAllocMaxSystems, CFDLA_ERR_DIMENSION,
+                "the allocation-free batched solve takes at most 16384 systems");
+  if (nsys == 0) return;
+  const int dev = library_device();
+  void *p = (dev >= 0 && dev < kScratchDevices) ? g_scratch[dev] : nullptr;
+  CFDLA_REQUIRE(p != nullptr, CFDLA_ERR_DEVICE, "allocation-free solve scratch not resolved on this device");
+  unsigned *queue = static_cast<unsigned *>(p);
+  BatchReport *rep = reinterpret_cast<BatchReport *>(static_cast<unsigned char *>(p) + kNoAllocQueueBytes);
+  launch(x, cfg, rhs_dev, queue, rep);  // k_reset_queue clears the counter the previous call left
+  if (reports) {
+    CFDLA_HIP(hipMemcpyAsync(reports, rep, (size_t)nsys * sizeof(BatchReport), hipMemcpyDeviceToHost, stream()));
   }
   sync_stream();
 }

@@ -951,6 +951,16 @@ cfdla_status cfdla_batch_solve_device(cfdla_batch b, const cfdla_solver_config *
   });
 }
 
+cfdla_status cfdla_batch_solve_device_noalloc(cfdla_batch b, const cfdla_solver_config *config,
+                                              const double *rhs_dev, double *x_dev, cfdla_report *reports) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    require_ptr(config, "config");
+    if (b->solver.nrows) require_ptr(x_dev, "x_dev");
+    b->solver.solve_noalloc(x_dev, *config, reinterpret_cast<BatchReport *>(reports), rhs_dev);
+  });
+}
+
 cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *config, int32_t n_candidates,
                                   const cfdla_batch_schedule *candidates, double *ms_out,
                                   cfdla_batch_schedule *best) {

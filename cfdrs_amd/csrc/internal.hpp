@@ -38,6 +38,11 @@ void set_last_error(const std::string &msg);
 // Library device/stream state.
 void init_device(int dev);
 void ensure_init();
+int library_device();  // the device init_device selected
+// Resolves, with `device` current, the device address of the allocation-free
+// batched solve's static scratch array (batch.hip); init_device calls it once
+// per device.
+void noalloc_scratch_init(int device);
 hipStream_t stream();
 void set_stream(hipStream_t s);
 void sync_stream();

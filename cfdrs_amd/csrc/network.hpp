@@ -101,6 +101,14 @@ struct BatchSolver {
   void download(std::vector<double> &values, std::vector<double> &rhs_out) const;
   // x: device FP64 packed solutions (initial guess in, solution out); reports: host array of nsys or null
   void solve(double *x, const cfdla_solver_config &cfg, BatchReport *reports, const double *rhs_override = nullptr);
+  // The same solve with the queue counter and report staging in the
+  // library's static scratch array instead of d_queue / d_rep: allocates
+  // nothing, so the batch must already be on the device, with at most
+  // 16384 systems.
+  void solve_noalloc(double *x, const cfdla_solver_config &cfg, BatchReport *reports,
+                     const double *rhs_override = nullptr);
+  void launch(double *x, const cfdla_solver_config &cfg, const double *rhs_override, unsigned *queue,
+              BatchReport *rep);
 };
 
 }  // namespace cfdla

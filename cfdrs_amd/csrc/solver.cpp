@@ -48,6 +48,7 @@ void init_device(int dev) {
   CFDLA_REQUIRE(count > 0, CFDLA_ERR_DEVICE, "no HIP device visible");
   CFDLA_REQUIRE(dev >= 0 && dev < count, CFDLA_ERR_INVALID_ARG, "device index out of range");
   CFDLA_HIP(hipSetDevice(dev));
+  noalloc_scratch_init(dev);
   hipDeviceProp_t prop;
   CFDLA_HIP(hipGetDeviceProperties(&prop, dev));
   d.cus = prop.multiProcessorCount;
@@ -66,6 +67,11 @@ void init_device(int dev) {
 
 void ensure_init() {
   if (!dstate().ready) init_device(0);
+}
+
+int library_device() {
+  ensure_init();
+  return dstate().device;
 }
 
 hipStream_t stream() {

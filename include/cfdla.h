@@ -359,6 +359,23 @@ cfdla_status cfdla_batch_solve(cfdla_batch b, const cfdla_solver_config *config,
 /* Same on device vectors; rhs_dev NULL uses the assembled right-hand side. */
 cfdla_status cfdla_batch_solve_device(cfdla_batch b, const cfdla_solver_config *config,
                                       const double *rhs_dev, double *x_dev, cfdla_report *reports);
+/* Allocation-free variant of cfdla_batch_solve_device: same arguments and
+ * results (reports is a host array of n_sys entries or NULL; the call
+ * synchronizes the library stream, as cfdla_batch_solve_device does).  No
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory during a call: tight serving and design loops, harnesses
+ * that forbid hipMalloc.  The work-queue counter and the per-system report
+ * staging live in a static device array of 256 B + 16384 x 48 B
+ * (768.25 KiB) that every GPU the library runs on holds until the process
+ * ends; its address is resolved once per device by cfdla_init.  Limits:
+ * at most 16384 systems (CFDLA_ERR_DIMENSION beyond), and the batch must
+ * already be resident on the device (CFDLA_ERR_CONFIG otherwise: its first
+ * cfdla_batch_solve_device or cfdla_batch_autotune uploads it).  The counter
+ * is cleared by a kernel at the start of every call.  All calls on one GPU
+ * share the array: calls that overlap on one GPU (other host threads or
+ * streams) race and must be serialized by the caller. */
+cfdla_status cfdla_batch_solve_device_noalloc(cfdla_batch b, const cfdla_solver_config *config,
+                                              const double *rhs_dev, double *x_dev, cfdla_report *reports);
 /* Time each candidate schedule with one full solve; ms_out[k] gets its time
  * and *best the fastest, which is also applied to the batch. */
 cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *config, int32_t n_candidates,
