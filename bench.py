@@ -6,13 +6,15 @@ residual of 1e-10.
 
 A step is one solve from x = 0 with the matrix, hierarchy and right-hand side
 resident in HBM.  ``value`` is the FP32-inner / FP64-refined time-to-1e-10 in
-milliseconds; the plain FP64 time, their ratio and the SpMV bandwidth are
-reported beside it.  ``roofline`` is the FP64 y = A x of the fine operator
-in its device format (symmetric diagonal storage for this stencil: main and
-3 upper diagonals, no indices), timed with HIP events on the stream it is
-launched on, against the bytes that format must move; ``cpu_baseline`` is the
-CPU FP64 oracle solving the same system on the host's cores (up to 16
-threads), its one-thread time beside it.
+milliseconds; the plain FP64 time, their ratio, each solution's error against
+the CPU FP64 oracle's solution of the same system and the solve-level
+bandwidth are reported beside it.  ``roofline`` is the FP64 CSR SpMV y = A x
+of the fine operator (row blocks, int32 row pointers and columns: the layout
+whose bytes nnz (s + 4) + n (2 s + 4) + 4 define the metric), timed with HIP
+events on the stream it is launched on; ``spmv_device_format`` is the same
+product in the format the solver uses for this stencil (symmetric diagonal
+storage).  ``cpu_baseline`` is the CPU FP64 oracle solving the same system
+on the host's cores (up to 16 threads), its one-thread time beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -29,11 +31,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the roofline kernel (FP64 y = A x on the 128^3
-# operator in diagonal storage), from two rocprofv3 counter passes over
-# `bench.py --spmv-only`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors
-# that make the counters read 512 MiB each for a 512 MiB device copy
-# (profiles/k_dia_spmv_varcoef128.md).  Keyed by the device format measured.
+# HBM bytes per launch of the SpMV kernels, from two rocprofv3 counter passes
+# over `bench.py --spmv-only --spmv-format F`: FETCH_SIZE x 2.0 + WRITE_SIZE x
+# 1.0 KiB, the factors that make the counters read 512 MiB each for a 512 MiB
+# device copy.  Keyed by the device format measured (128^3 operator).
 SPMV_FP64_TRAFFIC_BYTES = {"dia": round((42329.487 * 2.0 + 16384.0) * 1024)}
 
 
@@ -50,8 +51,9 @@ def spmv_bytes(n: int, nnz: int, s: int, storage: dict) -> int:
     if storage["format"] == "sell64":
         # padded slots: value + int16 column offset; one slice pointer per slice
         return storage["stored_entries"] * (s + 2) + (storage["units"] + 1) * 4 + 2 * n * s
-    # CSR: value + int32 column per nonzero, row pointers, row-block bounds
-    return nnz * (s + 4) + (n + 1) * 4 + (storage["units"] + 1) * 4 + 2 * n * s
+    # CSR with int32 columns: the metric's model nnz (s + 4) + n (2 s + 4) + 4
+    # (value + column per nonzero, row pointers, x read once, y written once)
+    return nnz * (s + 4) + n * (2 * s + 4) + 4
 
 
 def main() -> None:
@@ -76,6 +78,9 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV roofline (and copy calibration): the run profiled for byte counters")
+    ap.add_argument("--spmv-format", choices=["csr_int32", "auto", "both"], default="both",
+                    help="--spmv-only: the device format(s) timed (csr_int32: row-block CSR, int32 columns; "
+                         "auto: the format the solver uses)")
     ap.add_argument("--spmv-launches", type=int, default=200)
     ap.add_argument("--spmv-copies", type=int, default=4, help="device copies of the operator rotated in the SpMV timing")
     ap.add_argument("--no-configs", action="store_true",
@@ -104,12 +109,16 @@ def main() -> None:
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
     if args.spmv_only:
         bind_torch_stream()
-        spmv = spmv_roofline(a, (v, ci, rp), n, nnz, b_host, args)
-        copy = copy_calibration(args.copy_calibration)
-        print(json.dumps({"spmv_fp64_us": round(spmv[8]["us"], 3), "spmv_fp64_gbs": round(spmv[8]["gbs"], 1),
-                          "spmv_fp32_us": round(spmv[4]["us"], 3), "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
-                          "algorithmic_bytes_fp64": spmv[8]["bytes"], "algorithmic_bytes_fp32": spmv[4]["bytes"],
-                          "copy_calibration": copy}), flush=True)
+        fmts = ["csr_int32", "auto"] if args.spmv_format == "both" else [args.spmv_format]
+        res = {}
+        for f in fmts:
+            sp = spmv_roofline((v, ci, rp), n, nnz, b_host, args, f)
+            res[f] = {"format": sp["format"], "spmv_fp64_us": round(sp[8]["us"], 3),
+                      "spmv_fp64_gbs": round(sp[8]["gbs"], 1), "spmv_fp32_us": round(sp[4]["us"], 3),
+                      "spmv_fp32_gbs": round(sp[4]["gbs"], 1), "algorithmic_bytes_fp64": sp[8]["bytes"],
+                      "algorithmic_bytes_fp32": sp[4]["bytes"]}
+        res["copy_calibration"] = copy_calibration(args.copy_calibration)
+        print(json.dumps(res), flush=True)
         return
 
     cfg = ca.AMGConfig.smoothed_aggregation(args.theta, args.omega, smoothed=not args.unsmoothed)
@@ -156,16 +165,28 @@ def main() -> None:
 
     err = float(torch.linalg.norm(results["FP32_REFINED"]["x"] - results["FP64"]["x"]) /
                 torch.linalg.norm(results["FP64"]["x"]))
+    # solve-level bytes: the per-iteration model of the metric, B_spmv + B_prec
+    # + 8 n s, times the iterations, over the measured time
+    solve_bw = {}
+    for name, s_ in (("FP64", 8), ("FP32_REFINED", 4)):
+        per_it = pcg_iteration_bytes(amg, n, nnz, s_, cfg)
+        its = results[name]["iterations"]
+        solve_bw[name] = {"bytes_per_iteration": per_it, "iterations": its,
+                          "achieved_gbs": round(per_it * its / (results[name]["ms"] * 1e-3) / 1e9, 1)}
+        solve_bw[name]["frac_of_hbm_peak"] = round(solve_bw[name]["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    log(f"solve-level model bandwidth: {solve_bw}")
 
     # SpMV roofline: FP64 and FP32 y = A x on the fine operator, timed with
     # HIP events recorded on the stream the library launches on (torch's
-    # current stream, bound above).  The launches run back to back and rotate
-    # over --spmv-copies device copies of the operator and of x/y, so each
-    # launch streams a matrix last touched copies-1 launches earlier and the
-    # working set (4 x 180 MB for FP64) exceeds the 256 MB Infinity Cache,
-    # as inside a solve, where the V-cycle streams the other levels between
-    # two fine-level products.
-    spmv = spmv_roofline(a, (v, ci, rp), n, nnz, b_host, args)
+    # current stream, bound above), as row-block CSR with int32 columns (the
+    # layout of the metric's byte model) and in the solver's own format.  The
+    # launches run back to back and rotate over --spmv-copies device copies
+    # of the operator and of x/y, so each launch streams a matrix last
+    # touched copies-1 launches earlier and the working set exceeds the
+    # 256 MB Infinity Cache, as inside a solve, where the V-cycle streams the
+    # other levels between two fine-level products.
+    spmv = spmv_roofline((v, ci, rp), n, nnz, b_host, args, "csr_int32")
+    spmv_dev = spmv_roofline((v, ci, rp), n, nnz, b_host, args, "auto")
 
     copy = copy_calibration(args.copy_calibration)
 
@@ -186,10 +207,18 @@ def main() -> None:
     log(f"AMG recompute {nn}^3: {recompute}")
 
     others = None if args.no_configs else secondary_configs(args)
+    if others is not None and not args.no_cpu:
+        others["cpu_reference"] = cpu_secondary(args)
 
     cpu = None
+    oracle_err = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rp, ci, v, b_host, n, args, cfg)
+        cpu, x_orc = cpu_baseline(rp, ci, v, b_host, n, args, cfg)
+        # solution error of each GPU solve against the CPU FP64 oracle's
+        # solution of the same system (both stopped at relative residual 1e-10)
+        oracle_err = {name: float(np.linalg.norm(results[name]["x"].cpu().numpy() - x_orc) / np.linalg.norm(x_orc))
+                      for name in ("FP64", "FP32_REFINED")}
+        log(f"solution error against the CPU FP64 oracle: {oracle_err}")
 
     t64, tref = results["FP64"]["ms"], results["FP32_REFINED"]["ms"]
     line = {
@@ -218,9 +247,8 @@ def main() -> None:
         "rel_residual_fp64": results["FP64"]["rel_residual"],
         "rel_residual_refined": results["FP32_REFINED"]["rel_residual"],
         "refined_vs_fp64_solution_rel_diff": err,
-        "spmv_fp64_gbs": round(spmv[8]["gbs"], 1),
-        "spmv_fp32_gbs": round(spmv[4]["gbs"], 1),
-        "spmv_fp32_us": round(spmv[4]["us"], 3),
+        "solution_rel_error_vs_cpu_fp64_oracle": oracle_err,
+        "solve_level_bandwidth": solve_bw,
         "copy_calibration": copy,
         "amg_recompute": recompute,
         "roofline": {"bound": "hbm", "achieved": round(spmv[8]["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -233,19 +261,26 @@ def main() -> None:
                      "algorithmic_bytes": spmv[8]["bytes"],
                      "same_bytes_copy_gbs": round(spmv[8]["copy_gbs"], 1),
                      "frac_of_same_bytes_copy": round(spmv[8]["gbs"] / spmv[8]["copy_gbs"], 4),
-                     "csr_equivalent_bytes": spmv[8]["csr_bytes"],
-                     "csr_equivalent_gbs": round(spmv[8]["csr_bytes"] / (spmv[8]["us"] * 1e-6) / 1e9, 1)},
+                     "fp32_us": round(spmv[4]["us"], 3), "fp32_gbs": round(spmv[4]["gbs"], 1)},
+        "spmv_device_format": {"kernel": "FP64 y = A x, " + spmv_dev["kernel"], "avg_us": round(spmv_dev[8]["us"], 3),
+                               "format_bytes": spmv_dev[8]["bytes"], "achieved_gbs": round(spmv_dev[8]["gbs"], 1),
+                               "frac": round(spmv_dev[8]["gbs"] / HBM_PEAK_GBS, 4),
+                               "traffic": SPMV_FP64_TRAFFIC_BYTES.get(spmv_dev["format"]) if nn == 128 else None,
+                               "same_bytes_copy_gbs": round(spmv_dev[8]["copy_gbs"], 1),
+                               "fp32_us": round(spmv_dev[4]["us"], 3), "fp32_gbs": round(spmv_dev[4]["gbs"], 1)},
         "cpu_baseline": cpu,
         "other_workloads": others,
     }
     print(json.dumps(line), flush=True)
 
 
-def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
+def spmv_roofline(parts, n: int, nnz: int, b_host, args, fmt: str) -> dict:
     """Average duration and bandwidth of y = A x on the fine operator, FP64
-    and FP32.  K launches are enqueued back to back between two HIP events on
-    the library's stream, launch k on copy k % copies of the operator (and of
-    x and y), preceded by one untimed pass over all copies."""
+    and FP32, with the device format restricted to fmt ("csr_int32": row-block
+    CSR with int32 columns; "auto": the format the solver picks).  K launches
+    are enqueued back to back between two HIP events on the library's stream,
+    launch k on copy k % copies of the operator (and of x and y), preceded by
+    one untimed pass over all copies."""
     import ctypes as C
 
     import torch
@@ -255,13 +290,18 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
 
     v, ci, rp = parts
     dev = torch.device("cuda:0")
-    mats = [a] + [ca.CsrMatrix.from_parts(v, ci, rp, n, n) for _ in range(args.spmv_copies - 1)]
-    storage = a.storage()
+    mats = [ca.CsrMatrix.from_parts(v, ci, rp, n, n) for _ in range(args.spmv_copies)]
+    for m in mats:
+        storage = m.set_storage(fmt)
+    if fmt == "csr_int32":
+        storage = dict(storage, format="csr_int32")
     # FP64 takes the two-rows-per-lane kernel when n and the plane offsets are even
     dia_kernel = "k_dia_vec, two rows per lane" if args.grid % 2 == 0 else "k_dia"
     kernel = {"dia": f"{dia_kernel} (symmetric diagonal storage, {storage['units']} stored diagonals x {n} rows)",
               "sell64": f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)",
-              "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)"}[storage["format"]]
+              "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)",
+              "csr_int32": f"k_csr_rb, row-block CSR with int32 row pointers and columns ({storage['units']} row "
+                           f"blocks of at most 2048 nonzeros)"}[storage["format"]]
     fn = N.lib().cfdla_spmv_device
     out = {"kernel": kernel, "format": storage["format"]}
     for dt, s, code in ((torch.float64, 8, N.DT_F64), (torch.float32, 4, N.DT_F32)):
@@ -280,10 +320,9 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
         e1.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / args.spmv_launches
         nbytes = spmv_bytes(n, nnz, s, storage)
-        csr = spmv_bytes(n, nnz, s, {"format": "csr", "units": -1})  # nnz (s + 4) + n (2 s + 4) + 4
-        out[s] = dict(us=us, bytes=nbytes, gbs=nbytes / (us * 1e-6) / 1e9, csr_bytes=csr)
-        log(f"SpMV fp{8 * s}: {us:.2f} us/launch ({out[s]['gbs']:.1f} GB/s of {nbytes / 1e6:.2f} MB in the device "
-            f"format; CSR model {csr / 1e6:.2f} MB -> {csr / (us * 1e-6) / 1e9:.1f} GB/s)")
+        out[s] = dict(us=us, bytes=nbytes, gbs=nbytes / (us * 1e-6) / 1e9)
+        log(f"SpMV {storage['format']} fp{8 * s}: {us:.2f} us/launch ({out[s]['gbs']:.1f} GB/s of "
+            f"{nbytes / 1e6:.2f} MB)")
         del xs, ys
         if args.spmv_only:  # the counter runs calibrate on the copy kernel: no other copies there
             continue
@@ -304,7 +343,7 @@ def spmv_roofline(a, parts, n: int, nnz: int, b_host, args) -> dict:
         log(f"  same-bytes device copy: {cus:.2f} us ({out[s]['copy_gbs']:.1f} GB/s); SpMV at "
             f"{us and cus / us:.3f} of it")
         del srcs, dsts
-    for m in mats[1:]:
+    for m in mats:
         m.close()
     return out
 
@@ -445,8 +484,8 @@ def secondary_configs(args) -> dict:
         # columns and row pointers) and the Jacobi and equilibration factors:
         # nnz (s + 4) + 4 (n + 1) + n (s + 8) bytes, s = 8 (FP64) / 4 (FP32
         # inner iterations); the FP32 form's FP64 residual replacements are
-        # not counted.  Most re-reads hit the L2 of the team's XCD, so the
-        # rate is an effective stream rate, quoted against the HBM peak.
+        # not counted.  Most re-reads hit the L2 of the team's XCD, so this is
+        # an L2-served stream rate, not an HBM bandwidth.
         sv = 8 if prec == ca.Precision.FP64 else 4
         its = np.array([r.iterations for r in reps], np.float64)
         stream = float(np.sum(its * (net_nnz * (sv + 4) + 4 * (net_n + 1) + net_n * (sv + 8))))
@@ -455,8 +494,7 @@ def secondary_configs(args) -> dict:
                              "mean_iterations": round(float(its.mean()), 1),
                              "all_converged": all(r.termination == ca.Termination.Converged for r in reps),
                              "matrix_stream_gb": round(stream / 1e9, 2),
-                             "matrix_stream_gbs": round(stream / (ms * 1e-3) / 1e9, 1),
-                             "frac_of_hbm_peak": round(stream / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+                             "l2_served_matrix_stream_gbs": round(stream / (ms * 1e-3) / 1e9, 1),
                              "schedule": tuned["best"]}
     # design-loop refresh: widths -> conductances -> assembled values, right-
     # hand side and preconditioner factors, on the device against the host
@@ -562,16 +600,132 @@ def secondary_configs(args) -> dict:
     out["simple_pressure_step_512x512_channel"] = simple
     log(f"SIMPLE pressure step: {simple}")
 
-    # adjoint gradient of the outlet flux, 256^2 GRF permeability
+    # adjoint gradient of the outlet flux, 256^2 GRF permeability: the first
+    # call builds the AMG hierarchy on the host; later calls assemble on the
+    # device, recompute the hierarchy's Galerkin products on the device, and
+    # run the forward and adjoint solves and the gradient kernel
     prob = ADJ.FluxProblem(256, 256)
-    g = P.grf(256, 256)
-
-    def grad():
-        return ADJ.flux_and_gradient(prob, g)
-
-    ms, _ = _time_solves(grad, steps)
-    out["adjoint_flux_gradient_256x256"] = {"ms_including_amg_setup": round(ms, 3)}
+    g = torch.from_numpy(P.grf(256, 256)).to(dev)
+    first_ms, _ = _time_solves(lambda: ADJ.flux_and_gradient(prob, g), 1, warmup=0)
+    fwd_ms, _ = _time_solves(lambda: prob.forward(g), steps)
+    ms, _ = _time_solves(lambda: ADJ.flux_and_gradient(prob, g), steps)
+    out["adjoint_flux_gradient_256x256"] = {"ms": round(ms, 3), "forward_ms": round(fwd_ms, 3),
+                                            "first_call_ms_including_amg_setup": round(first_ms, 3)}
     log(f"adjoint: {out['adjoint_flux_gradient_256x256']}")
+    return out
+
+
+def cpu_secondary(args) -> dict:
+    """The CPU FP64 oracle (the reference's algorithms restated) on the four
+    other workloads, timed on the host's cores (up to 16 threads) and on one
+    thread, setup and assembly excluded: 2D pressure-correction Jacobi-PCG
+    (128^2), cavity momentum BiCGSTAB and GMRES(30) with Jacobi and the pinned
+    pressure AMG-PCG (512^2), the reference's per-network cascade
+    (linear_system.rs:114-267: row equilibration, dense LU up to 256 rows,
+    else Jacobi-CG and its LU fallback) over a seeded sample of the 4096
+    networks, and the forward + adjoint solves and gradient on the 256^2
+    permeability field."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from cfdrs_amd import network as NW
+    from cfdrs_amd import problems as P
+    from oracle import oracle as O
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    out = {"cores": threads, "kind": "port", "host": host_cpu()}
+    log(f"CPU oracle on the secondary workloads ({threads} threads, then 1)")
+
+    def timed(fn):
+        res = {}
+        for nt in (threads, 1):
+            O.set_threads(nt)
+            t0 = time.perf_counter()
+            extra = fn()
+            res["ms" if nt == threads else "single_thread_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+            res.update(extra or {})
+        O.set_threads(1)
+        return res
+
+    rp, ci, v, b1 = P.config1(128)
+    a1 = O.Csr.from_arrays(rp, ci, v, 128 * 128)
+    out["pressure_correction_128x128_jacobi_pcg"] = timed(
+        lambda: {"iterations": O.solve(a1, b1, kind="cg", prec="jacobi", tol_rel=1e-10, max_it=20000)[1].iterations})
+
+    (mrp, mci, mv, bm), (prp, pci, pv, bp) = P.config3(512)
+    am = O.Csr.from_arrays(mrp, mci, mv, 512 * 512)
+    ap_ = O.Csr.from_arrays(prp, pci, pv, bp.size)
+    cav = {}
+    cav["momentum_bicgstab_jacobi"] = timed(
+        lambda: {"iterations": O.solve(am, bm, kind="bicgstab", prec="jacobi", tol_rel=1e-10,
+                                       max_it=20000)[1].iterations})
+    cav["momentum_gmres30_jacobi"] = timed(
+        lambda: {"iterations": O.solve(am, bm, kind="gmres", prec="jacobi", tol_rel=1e-10, max_it=20000,
+                                       restart=30)[1].iterations})
+    amg_p = O.Amg(ap_, coarsening="three_phase", theta=0.25, interp="smoothed", smoother="jacobi", omega=0.8,
+                  coarse_solve="direct", min_coarse=1000)
+    cav["pressure_amg_pcg"] = timed(
+        lambda: {"iterations": O.solve(ap_, bp, kind="cg", prec="amg", amg=amg_p, tol_rel=1e-10,
+                                       max_it=1000)[1].iterations})
+    out["lid_driven_cavity_512x512_re1000"] = cav
+
+    nets = NW.synth_networks()
+    batch = NW.channel_batch(nets)
+    nrp, nci, nv, nb = batch.export()
+    no = nets.node_offset
+    sample = np.sort(np.random.default_rng(4999).choice(nets.count, size=512, replace=False))
+    systems = []
+    for s_ in sample:
+        r0, r1 = int(no[s_]), int(no[s_ + 1])
+        k0, k1 = int(nrp[r0]), int(nrp[r1])
+        systems.append((O.Csr.from_arrays(nrp[r0:r1 + 1] - k0, nci[k0:k1] - r0, nv[k0:k1], r1 - r0),
+                        np.ascontiguousarray(nb[r0:r1])))
+    O.set_threads(1)
+
+    def cascade(sys_):
+        return O.network_cascade(sys_[0], sys_[1], tol=1e-10, max_it=1000)[1]
+
+    net = {"sample": f"{sample.size} of the {nets.count} networks (seeded uniform draw, "
+                     f"{int(sum(no[s_ + 1] - no[s_] for s_ in sample))} nodes), equilibrated tolerance 1e-10, "
+                     f"1000 CG iterations before the LU fallback (the reference's default cap)"}
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as pool:  # one network per task; ctypes releases the GIL
+        paths = list(pool.map(cascade, systems))
+    ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for sys_ in systems:
+        cascade(sys_)
+    ms1 = (time.perf_counter() - t0) * 1e3
+    net.update({"ms": round(ms, 2), "single_thread_ms": round(ms1, 2),
+                "systems_per_s": round(sample.size * 1e3 / ms, 1),
+                "single_thread_systems_per_s": round(sample.size * 1e3 / ms1, 1),
+                "paths": {name: int(sum(1 for p_ in paths if p_ == code))
+                          for code, name in ((0, "dense"), (1, "krylov"), (2, "dense_fallback"))}})
+    out["networks_4096"] = net
+
+    from cfdrs_amd import adjoint as ADJ
+
+    nx = ny = 256
+    g = P.grf(nx, ny)
+    k = np.exp(g)
+    a5, rhs5 = O.assemble_varcoef(nx, ny, 1, float(nx * nx), k, ADJ.DIR_MASK, np.array(ADJ.DIR_VAL))
+    amg5 = O.Amg(a5, coarsening="three_phase", theta=0.25, interp="smoothed", smoother="jacobi", omega=0.8,
+                 coarse_solve="direct", min_coarse=1000)
+    east = np.arange(ny) * nx + nx - 1
+
+    def adjoint():
+        p_, _ = O.solve(a5, rhs5, kind="cg", prec="amg", amg=amg5, tol_rel=1e-10, max_it=1000)
+        dJdp = np.zeros(nx * ny)
+        dJdp[east] = k[east]
+        lam, _ = O.solve(a5, dJdp, kind="cg", prec="amg", amg=amg5, tol_rel=1e-10, max_it=1000)
+        dJdk = np.zeros(nx * ny)
+        dJdk[east] = p_[east]
+        O.coef_grad_2d(nx, ny, float(nx * nx), ADJ.DIR_MASK, np.array(ADJ.DIR_VAL[:4]), k, p_, lam, dJdk)
+
+    out["adjoint_flux_gradient_256x256"] = timed(adjoint)
+    out["adjoint_flux_gradient_256x256"]["sample"] = "forward and adjoint AMG-PCG solves to 1e-10 and the gradient"
+    log(f"CPU oracle, secondary workloads: {out}")
     return out
 
 
@@ -615,7 +769,62 @@ def cpu_baseline(rp, ci, v, b_host, n, args, cfg):
     return {"value": round(timings[threads] * 1e3, 2), "unit": "ms", "cores": threads, "kind": "port",
             "sample": f"full FP64 AMG-PCG solve to 1e-10 of the same {n}-unknown system ({rep.iterations} "
                       f"iterations, relative residual {rel:.2e}); AMG setup {setup:.1f} s (one thread) excluded",
-            "single_thread_ms": round(timings[1] * 1e3, 2), "iterations": rep.iterations}
+            "single_thread_ms": round(timings[1] * 1e3, 2), "iterations": rep.iterations,
+            "host": host_cpu()}, x
+
+
+def host_cpu() -> dict:
+    """CPU model, physical cores and logical CPUs of this host (/proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, phys, logical = None, set(), 0
+    try:
+        pid = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, val = line.partition(":")
+                k, val = k.strip(), val.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = val
+                elif k == "physical id":
+                    pid = val
+                elif k == "core id":
+                    core = val
+                elif not k and pid is not None and core is not None:
+                    phys.add((pid, core))
+                    pid = core = None
+        if pid is not None and core is not None:
+            phys.add((pid, core))
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(phys) or None, "logical_cpus": logical or None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def pcg_iteration_bytes(amg, n: int, nnz: int, s: int, cfg) -> int:
+    """Algorithmic bytes of one AMG-PCG iteration: B_spmv + B_prec + 8 n s,
+    B_prec summing over the levels (nu1 + nu2) Jacobi sweeps (B_spmv,l +
+    3 n_l s), the residual (B_spmv,l + n_l s), R_l and P_l products and
+    2 n_l s, plus the coarsest dense solve (s_c n_c^2 + 2 n_c s, FP64 inverse)."""
+    st = amg.statistics
+    sizes, nnzs = st.level_sizes, st.level_nnz
+
+    def spmv(rows, cols, z):
+        return z * (s + 4) + rows * (s + 4) + cols * s + 4
+
+    nu = cfg.pre_smooth_iterations + cfg.post_smooth_iterations
+    prec = 0
+    for l in range(len(sizes) - 1):
+        nl = sizes[l]
+        bl = spmv(nl, nl, nnzs[l])
+        r_nnz = amg.level(l, "R")[1].size
+        p_nnz = amg.level(l, "P")[1].size
+        prec += nu * (bl + 3 * nl * s) + (bl + nl * s) + spmv(sizes[l + 1], nl, r_nnz) \
+            + spmv(nl, sizes[l + 1], p_nnz) + 2 * nl * s
+    nc = sizes[-1]
+    prec += 8 * nc * nc + 2 * nc * s
+    return int(spmv(n, n, nnz) + prec + 8 * n * s)
 
 
 if __name__ == "__main__":

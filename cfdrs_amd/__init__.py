@@ -9,7 +9,8 @@ from ._native import CfdlaError, NativeLibraryMissing
 from .linear_solver import (IterativeSolverConfig, KrylovWorkspace, Precision, SolveError, SolveOutcome,
                             SolveReport, SolverError, SolverKind, Termination, attempt, bicgstab,
                             bicgstab_preconditioned, cg, cg_preconditioned, converged_or_none, gmres,
-                            gmres_preconditioned, gmres_preconditioned_with_workspace, interpret, restart_width)
+                            gmres_preconditioned, gmres_preconditioned_with_workspace, interpret, restart_width,
+                            solve_transpose)
 from .multigrid import (AlgebraicMultigrid, AMGConfig, AMGStatistics, CoarseningStrategy, CoarseSolver,
                         InterpolationStrategy, JacobiPreconditioner, SmootherType, SuccessiveOverRelaxation)
 from .sparse import CsrMatrix, SparseMatrixBuilder
@@ -22,5 +23,5 @@ __all__ = [
     "SolverError", "SolverKind", "SparseMatrixBuilder", "SuccessiveOverRelaxation", "Termination", "attempt",
     "bicgstab",
     "bicgstab_preconditioned", "cg", "cg_preconditioned", "converged_or_none", "gmres", "gmres_preconditioned",
-    "gmres_preconditioned_with_workspace", "interpret", "restart_width",
+    "gmres_preconditioned_with_workspace", "interpret", "restart_width", "solve_transpose",
 ]

@@ -137,6 +137,10 @@ SIGNATURES = {
     "cfdla_batch_get_schedule": (C.c_int, [_VP, C.POINTER(BatchSchedule)]),
     "cfdla_batch_solve": (C.c_int, [_VP, C.POINTER(SolverConfig), _VP, _VP]),
     "cfdla_batch_solve_device": (C.c_int, [_VP, C.POINTER(SolverConfig), _VP, _VP, _VP]),
+    "cfdla_batch_edge_flows_device": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "cfdla_batch_outlet_flux_device": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "cfdla_varcoef_refresh_device": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64, C.c_double, _VP, C.c_int32,
+                                               _VP, _VP]),
     "cfdla_batch_solve_device_noalloc": (C.c_int, [_VP, C.POINTER(SolverConfig), _VP, _VP, _VP]),
     "cfdla_batch_autotune": (C.c_int, [_VP, C.POINTER(SolverConfig), C.c_int32, _VP, _VP, C.POINTER(BatchSchedule)]),
     "cfdla_batch_edge_gradient_device": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),

@@ -810,6 +810,120 @@ __global__ __launch_bounds__(kBlock) void k_network_grad(int64_t ne, const int *
 }
 }  // namespace
 
+namespace {
+// ---- edge flows from node pressures ------------------------------------------
+// The reference's update_from_solution (crates/cfd-1d/src/domain/network/
+// wrapper.rs:349-428): flow_e = (p_from - p_to) / (R_e + k_e |q_prev,e|),
+// after it rejects an edge with R < 0, k < 0, both R and k below epsilon, or
+// an effective resistance that is non-finite or not above epsilon.  The
+// check pass runs first, so a rejected batch has no flow written.
+__global__ __launch_bounds__(kBlock) void k_flow_check(int64_t ne, const double *__restrict__ R,
+                                                       const double *__restrict__ kq, const double *__restrict__ q,
+                                                       int *bad) {
+  const double eps = 2.220446049250313e-16;
+  int local = 0;
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+    const double r = R[e], k = kq ? kq[e] : 0.0;
+    const double reff = kq ? r + k * fabs(q[e]) : r;
+    const bool ok = r >= 0.0 && k >= 0.0 && !(fabs(r) < eps && fabs(k) < eps) && isfinite(reff) && reff > eps;
+    local += ok ? 0 : 1;
+  }
+  if (local) atomicAdd(bad, local);
+}
+
+__global__ __launch_bounds__(kBlock) void k_flows(int64_t ne, const int32_t *__restrict__ ei,
+                                                  const int32_t *__restrict__ ej, const double *__restrict__ p,
+                                                  const double *__restrict__ R, const double *__restrict__ kq,
+                                                  double *q) {
+#pragma clang fp contract(off)
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+    const double reff = kq ? R[e] + kq[e] * fabs(q[e]) : R[e];
+    q[e] = (p[ei[e]] - p[ej[e]]) / reff;
+  }
+}
+
+// ---- outlet objective ----------------------------------------------------------
+// J_s = sum over the edges e = (i, j) of network s of G_e x_e with
+// x_e = m_j (p_i - p_j) + m_i (p_j - p_i): the flow into the nodes whose
+// outlet mask m is set.  One block per network sums its edges in a fixed
+// order (strided lanes, then a shuffle / LDS tree), so J is reproducible;
+// x_e, the explicit dJ/dG_e, is written when asked for.
+__global__ __launch_bounds__(kBlock) void k_outlet_edges(const int64_t *__restrict__ eoff,
+                                                         const int32_t *__restrict__ ei,
+                                                         const int32_t *__restrict__ ej,
+                                                         const uint8_t *__restrict__ m, const double *__restrict__ p,
+                                                         const double *__restrict__ g, double *__restrict__ J,
+                                                         double *__restrict__ dJdg) {
+  __shared__ double wsum[kBlock / 64];
+  const int s = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t e = eoff[s] + threadIdx.x; e < eoff[s + 1]; e += kBlock) {
+    const int i = ei[e], j = ej[e];
+    const double x = (m[j] ? p[i] - p[j] : 0.0) + (m[i] ? p[j] - p[i] : 0.0);
+    if (dJdg) dJdg[e] = x;
+    acc += g[e] * x;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+    J[s] = t;
+  }
+}
+
+// dJ/dp_i = sum over the edges at free node i (in edge order) of
+// G_e (m_other - m_i); 0 on Dirichlet nodes, which the adjoint system keeps
+// as identity rows.
+__global__ __launch_bounds__(kBlock) void k_outlet_dp(int64_t nrows, const int64_t *__restrict__ inc_ptr,
+                                                      const int32_t *__restrict__ inc_edge,
+                                                      const int32_t *__restrict__ ei, const int32_t *__restrict__ ej,
+                                                      const uint8_t *__restrict__ isdir,
+                                                      const uint8_t *__restrict__ m, const double *__restrict__ g,
+                                                      double *__restrict__ dJdp) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kBlock) {
+    double d = 0.0;
+    if (!isdir[i]) {
+      const double mi = m[i] ? 1.0 : 0.0;
+      for (int64_t t = inc_ptr[i]; t < inc_ptr[i + 1]; ++t) {
+        const int e = inc_edge[t];
+        const int o = ei[e] == (int)i ? ej[e] : ei[e];
+        d += g[e] * ((m[o] ? 1.0 : 0.0) - mi);
+      }
+    }
+    dJdp[i] = d;
+  }
+}
+}  // namespace
+
+void network_flows(int64_t ne, const int32_t *ei, const int32_t *ej, const double *p, const double *R,
+                   const double *kq, double *q, int *bad) {
+  if (ne == 0) return;
+  hipLaunchKernelGGL(k_net_clear, dim3(1), dim3(64), 0, stream(), bad);
+  hipLaunchKernelGGL(k_flow_check, dim3(blocks_for(ne, kBlock * 4)), dim3(kBlock), 0, stream(), ne, R, kq, q, bad);
+  int h = 0;
+  CFDLA_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  sync_stream();
+  CFDLA_REQUIRE(h == 0, CFDLA_ERR_CONFIG,
+                std::to_string(h) + " edge(s) with a negative, zero or non-finite (effective) resistance");
+  hipLaunchKernelGGL(k_flows, dim3(blocks_for(ne, kBlock * 4)), dim3(kBlock), 0, stream(), ne, ei, ej, p, R, kq, q);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void network_outlet(int64_t nsys, int64_t nrows, const int64_t *eoff, const int32_t *ei, const int32_t *ej,
+                    const int64_t *inc_ptr, const int32_t *inc_edge, const uint8_t *isdir, const uint8_t *mask,
+                    const double *p, const double *g, double *J, double *dJdp, double *dJdg) {
+  if (nsys == 0) return;
+  hipLaunchKernelGGL(k_outlet_edges, dim3((unsigned)nsys), dim3(kBlock), 0, stream(), eoff, ei, ej, mask, p, g, J,
+                     dJdg);
+  if (dJdp && nrows)
+    hipLaunchKernelGGL(k_outlet_dp, dim3(blocks_for(nrows, kBlock * 4)), dim3(kBlock), 0, stream(), nrows, inc_ptr,
+                       inc_edge, ei, ej, isdir, mask, g, dJdp);
+  CFDLA_HIP(hipGetLastError());
+}
+
 void network_gradient(int64_t ne, const int32_t *ei, const int32_t *ej, const uint8_t *isdir, const double *p,
                       const double *lam, const double *explicit_dg, const double *dg_dparam, double *out) {
   if (ne == 0) return;

@@ -23,6 +23,7 @@ struct cfdla_matrix_s {
   bool host_stale = false;    // device values newer than host.v (device updates)
   DevBuf<int32_t> dperm;      // perm on the device, for device updates
   int64_t simple_nx = 0, simple_ny = 0;  // SIMPLE pressure pattern verified for this grid
+  int64_t varcoef_grid[3] = {0, 0, 0};   // variable-coefficient pattern verified for this grid
 };
 
 struct cfdla_precond_s {
@@ -314,6 +315,47 @@ cfdla_status cfdla_simple_pressure_refresh_device(cfdla_matrix m, int64_t nx, in
   });
 }
 
+cfdla_status cfdla_varcoef_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, int64_t nz, double inv_h2,
+                                         const double *k_dev, int32_t dir_mask, const double *dir_val,
+                                         double *rhs_bc_dev) {
+  return guarded([&] {
+    require_ptr(m, "matrix");
+    require_ptr(k_dev, "k_dev");
+    CFDLA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, CFDLA_ERR_INVALID_ARG, "grid must have at least one cell");
+    CFDLA_REQUIRE(nx * ny * nz < INT32_MAX, CFDLA_ERR_INVALID_ARG, "grid exceeds 32-bit device indexing");
+    if (dir_mask) require_ptr(dir_val, "dir_val");
+    const int64_t n = nx * ny * nz;
+    CFDLA_REQUIRE(m->host.n == n && m->host.m == n, CFDLA_ERR_DIMENSION,
+                  "matrix is not the variable-coefficient operator of this grid");
+    if (m->varcoef_grid[0] != nx || m->varcoef_grid[1] != ny || m->varcoef_grid[2] != nz) {
+      // the stored pattern must be cfdla_problem_varcoef's for this grid
+      const HostCsr &h = m->host;
+      const bool three_d = nz > 1;
+      const int64_t sy = nx, sz = nx * ny;
+      for (int64_t p = 0; p < n; ++p) {
+        const int64_t x = p % nx, y = (p / nx) % ny, z = p / sz;
+        int64_t want[7], c = 0;
+        if (three_d && z > 0) want[c++] = p - sz;
+        if (y > 0) want[c++] = p - sy;
+        if (x > 0) want[c++] = p - 1;
+        want[c++] = p;
+        if (x + 1 < nx) want[c++] = p + 1;
+        if (y + 1 < ny) want[c++] = p + sy;
+        if (three_d && z + 1 < nz) want[c++] = p + sz;
+        bool ok = h.rp[p + 1] - h.rp[p] == c;
+        for (int64_t q = 0; q < c && ok; ++q) ok = h.ci[h.rp[p] + q] == want[q];
+        CFDLA_REQUIRE(ok, CFDLA_ERR_INVALID_ARG, "matrix pattern is not cfdla_problem_varcoef's for this grid");
+      }
+      m->varcoef_grid[0] = nx;
+      m->varcoef_grid[1] = ny;
+      m->varcoef_grid[2] = nz;
+    }
+    DevCsr &d = devmat(m);
+    varcoef_refresh(nx, ny, nz, inv_h2, k_dev, dir_mask, dir_val, d.rp.p, d.v64.p, rhs_bc_dev);
+    device_values_changed(m);
+  });
+}
+
 cfdla_status cfdla_pressure_rhs_device(int64_t nx, int64_t ny, double dx, double dy, double rho, double dt,
                                        const double *u, const double *v, int32_t pin_reference, double *rhs) {
   return guarded([&] {
@@ -364,7 +406,8 @@ cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format) {
   return guarded([&] {
     require_ptr(m, "matrix");
     CFDLA_REQUIRE(format == CFDLA_STORAGE_AUTO || format == CFDLA_STORAGE_CSR_ROWBLOCK ||
-                      format == CFDLA_STORAGE_SELL64 || format == CFDLA_STORAGE_DIA,
+                      format == CFDLA_STORAGE_SELL64 || format == CFDLA_STORAGE_DIA ||
+                      format == CFDLA_STORAGE_CSR_INT32,
                   CFDLA_ERR_INVALID_ARG, "unknown storage format");
     if (m->dev.fmt_request == format) return;
     m->dev.fmt_request = format;
@@ -800,7 +843,8 @@ struct cfdla_batch_s {
   std::vector<int64_t> node_off, edge_off, ei, ej;
   std::vector<uint8_t> isdir;
   std::vector<double> dirval, neumann, g;
-  DevBuf<int32_t> d_ei, d_ej;
+  DevBuf<int32_t> d_ei, d_ej, d_inc_edge, d_bad;
+  DevBuf<int64_t> d_eoff, d_inc_ptr;
   DevBuf<uint8_t> d_isdir;
   bool edges_on_device = false;
   NetworkInput input() const {
@@ -1012,6 +1056,44 @@ cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *conf
   });
 }
 
+namespace {
+// Global edge endpoints, per-network edge offsets, each node's incident edges
+// (in edge order) and the Dirichlet flags on the device, uploaded once.
+void batch_edges_on_device(cfdla_batch_s *b) {
+  if (b->edges_on_device) return;
+  const int64_t E = (int64_t)b->ei.size();
+  const int64_t ns = (int64_t)b->node_off.size() - 1;
+  const int64_t nrows = b->node_off.empty() ? 0 : b->node_off.back();
+  std::vector<int32_t> gi((size_t)E), gj((size_t)E);
+  for (int64_t s = 0; s < ns; ++s)
+    for (int64_t e = b->edge_off[s]; e < b->edge_off[s + 1]; ++e) {
+      gi[e] = (int32_t)(b->node_off[s] + b->ei[e]);
+      gj[e] = (int32_t)(b->node_off[s] + b->ej[e]);
+    }
+  std::vector<int64_t> inc_ptr((size_t)nrows + 1, 0);
+  for (int64_t e = 0; e < E; ++e) {
+    inc_ptr[(size_t)gi[e] + 1]++;
+    inc_ptr[(size_t)gj[e] + 1]++;
+  }
+  for (int64_t i = 0; i < nrows; ++i) inc_ptr[i + 1] += inc_ptr[i];
+  std::vector<int32_t> inc((size_t)std::max<int64_t>(1, 2 * E));
+  std::vector<int64_t> next(inc_ptr.begin(), inc_ptr.end() - 1);
+  for (int64_t e = 0; e < E; ++e) {
+    inc[(size_t)next[gi[e]]++] = (int32_t)e;
+    inc[(size_t)next[gj[e]]++] = (int32_t)e;
+  }
+  b->d_ei.upload(gi.data(), std::max<size_t>(1, gi.size()));
+  b->d_ej.upload(gj.data(), std::max<size_t>(1, gj.size()));
+  b->d_isdir.upload(b->isdir.data(), std::max<size_t>(1, b->isdir.size()));
+  b->d_eoff.upload(b->edge_off.data(), b->edge_off.size());
+  b->d_inc_ptr.upload(inc_ptr.data(), inc_ptr.size());
+  b->d_inc_edge.upload(inc.data(), inc.size());
+  b->d_bad.ensure(1);
+  sync_stream();
+  b->edges_on_device = true;
+}
+}  // namespace
+
 cfdla_status cfdla_batch_edge_gradient_device(cfdla_batch b, const double *p, const double *lambda,
                                               const double *explicit_dg, const double *dg_dparam, double *out) {
   return guarded([&] {
@@ -1022,21 +1104,39 @@ cfdla_status cfdla_batch_edge_gradient_device(cfdla_batch b, const double *p, co
       require_ptr(lambda, "lambda");
       require_ptr(out, "out");
     }
-    if (!b->edges_on_device) {
-      std::vector<int32_t> gi((size_t)E), gj((size_t)E);
-      const int64_t ns = (int64_t)b->node_off.size() - 1;
-      for (int64_t s = 0; s < ns; ++s)
-        for (int64_t e = b->edge_off[s]; e < b->edge_off[s + 1]; ++e) {
-          gi[e] = (int32_t)(b->node_off[s] + b->ei[e]);
-          gj[e] = (int32_t)(b->node_off[s] + b->ej[e]);
-        }
-      b->d_ei.upload(gi.data(), std::max<size_t>(1, gi.size()));
-      b->d_ej.upload(gj.data(), std::max<size_t>(1, gj.size()));
-      b->d_isdir.upload(b->isdir.data(), std::max<size_t>(1, b->isdir.size()));
-      sync_stream();
-      b->edges_on_device = true;
-    }
+    batch_edges_on_device(b);
     network_gradient(E, b->d_ei.p, b->d_ej.p, b->d_isdir.p, p, lambda, explicit_dg, dg_dparam, out);
+  });
+}
+
+cfdla_status cfdla_batch_edge_flows_device(cfdla_batch b, const double *p, const double *resistance,
+                                           const double *quad_coeff, double *flow_inout) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    const int64_t E = (int64_t)b->ei.size();
+    if (E == 0) return;
+    require_ptr(p, "p");
+    require_ptr(resistance, "resistance");
+    require_ptr(flow_inout, "flow");
+    batch_edges_on_device(b);
+    network_flows(E, b->d_ei.p, b->d_ej.p, p, resistance, quad_coeff, flow_inout, b->d_bad.p);
+  });
+}
+
+cfdla_status cfdla_batch_outlet_flux_device(cfdla_batch b, const double *p, const double *conductance,
+                                            const uint8_t *outlet_mask, double *flux, double *dflux_dp,
+                                            double *dflux_dg) {
+  return guarded([&] {
+    require_ptr(b, "batch");
+    const int64_t ns = (int64_t)b->node_off.size() - 1;
+    if (ns <= 0) return;
+    require_ptr(p, "p");
+    require_ptr(conductance, "conductance");
+    require_ptr(outlet_mask, "outlet_mask");
+    require_ptr(flux, "flux");
+    batch_edges_on_device(b);
+    network_outlet(ns, b->node_off.back(), b->d_eoff.p, b->d_ei.p, b->d_ej.p, b->d_inc_ptr.p, b->d_inc_edge.p,
+                   b->d_isdir.p, outlet_mask, p, conductance, flux, dflux_dp, dflux_dg);
   });
 }
 

@@ -293,6 +293,11 @@ void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, c
                     const double *p, const double *du, const double *dv, const double *rho, const int32_t *rp,
                     double *v64, double *rhs, bool want_max);
 double simple_max_read();
+// Rows of the variable-coefficient Poisson operator (cfdla_problem_varcoef's
+// pattern, sorted columns from rp[p]) into v64 from device k, bit for bit as
+// the host generator forms them; rhs (nullable) gets the Dirichlet terms.
+void varcoef_refresh(int64_t nx, int64_t ny, int64_t nz, double inv_h2, const double *k, int dir_mask,
+                     const double *dir_val, const int32_t *rp, double *v64, double *rhs);
 void simple_corrections(int64_t nx, int64_t ny, double dx, double dy, double relax, const double *pp,
                         const double *du, const double *dv, double *p, double *u, double *v);
 // Cell-centred pressure-correction right-hand side on device arrays.
@@ -346,6 +351,10 @@ template <typename T>
 void copy_vec(int n, const T *src, T *dst, Scal *s, bool check_done);
 template <typename T>
 void fill_vec(int n, T value, T *dst);
+// dst = 0, skipped once the solve has stopped (check_done and s->done), so
+// work enqueued after convergence leaves the solver's vectors alone.
+template <typename T>
+void zero_vec(int n, T *dst, Scal *s, bool check_done);
 void f64_to_f32_scaled(int n, const double *src, double scale, float *dst);
 void axpy_f32_to_f64(int n, double a, const float *x, double *y);
 // Residual replacement of the reliable-update CG, skipped when check_done
@@ -380,6 +389,11 @@ void sor_colour(int count, const int *perm, const int *rp, const int *ci, const 
 // The first colour from z = 0, which sees no nonzero neighbour:
 // z_i = piv_i r_i where first[i], 0 elsewhere (all 0 when check_done and
 // s->done != 0).
+// One colour of a multicolour Gauss-Seidel sweep (rows perm[0..count)),
+// x_i += omega ((b_i - sum_{j != i} a_ij x_j) / a_ii - x_i) where |a_ii| > 1e-15.
+template <typename T>
+void gs_colour(int count, const int *perm, const int *rp, const int *ci, const T *v, const T *b, T *x, double omega,
+               Scal *s, bool check_done);
 template <typename T>
 void sor_first(int n, const uint8_t *first, const T *piv, const T *r, T *z, Scal *s, bool check_done);
 

@@ -624,6 +624,32 @@ __global__ __launch_bounds__(kBlock) void k_sor_colour(int count, const int *__r
   }
 }
 
+// One colour of a multicolour Gauss-Seidel sweep on A x = b, the reference's
+// row update (smoothers.rs:42-82): with the diagonal d and the off-diagonal
+// sum of row i (columns in stored order), x_i += omega ((b_i - sum) / d - x_i)
+// unless |d| <= 1e-15.  Rows of one colour are not coupled, so updating them
+// together equals updating them one after another.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_colour(int count, const int *__restrict__ perm,
+                                                      const int *__restrict__ rp, const int *__restrict__ ci,
+                                                      const T *__restrict__ v, const T *__restrict__ b, T *x, T omega,
+                                                      Scal *s, int check_done) {
+  if (check_done && s->done) return;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < count; t += gridDim.x * kBlock) {
+    const int i = perm[t];
+    T sum = T(0), diag = T(0);
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      if (j == i) diag = v[k];
+      else sum += v[k] * x[j];
+    }
+    if (fabs((double)diag) > 1e-15) {
+      const T nv = (b[i] - sum) / diag;
+      x[i] = x[i] + omega * (nv - x[i]);
+    }
+  }
+}
+
 // The first colour of a sweep from z = 0 sees only zero neighbours:
 // z_i = piv_i r_i there and 0 elsewhere, one streaming pass instead of a
 // fill and a sweep over the matrix rows.  (After convergence, z = 0.)
@@ -776,6 +802,21 @@ void copy_vec(int n, const T *src, T *dst, Scal *s, bool check_done) {
 }
 
 template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_chk(int n, T *__restrict__ dst, Scal *s, int check_done) {
+  if (check_done && s->done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = T(0);
+}
+
+template <typename T>
+void zero_vec(int n, T *dst, Scal *s, bool check_done) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_zero_chk<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, dst, s,
+                     check_done ? 1 : 0);
+}
+template void zero_vec<double>(int, double *, Scal *, bool);
+template void zero_vec<float>(int, float *, Scal *, bool);
+
+template <typename T>
 void fill_vec(int n, T value, T *dst) {
   hipLaunchKernelGGL((k_fill<T>), dim3(blocks_for(n, kBlock * 4)), dim3(kBlock), 0, stream(), n, value, dst);
   CFDLA_HIP(hipGetLastError());
@@ -858,6 +899,18 @@ void sor_colour(int count, const int *perm, const int *rp, const int *ci, const 
                      rp, ci, v, piv, r, z, s, check_done ? 1 : 0);
   CFDLA_HIP(hipGetLastError());
 }
+template <typename T>
+void gs_colour(int count, const int *perm, const int *rp, const int *ci, const T *v, const T *b, T *x, double omega,
+               Scal *s, bool check_done) {
+  if (count == 0) return;
+  hipLaunchKernelGGL((k_gs_colour<T>), dim3(blocks_for(count, kBlock * 4)), dim3(kBlock), 0, stream(), count, perm,
+                     rp, ci, v, b, x, (T)omega, s, check_done ? 1 : 0);
+  CFDLA_HIP(hipGetLastError());
+}
+template void gs_colour<double>(int, const int *, const int *, const int *, const double *, const double *, double *,
+                                double, Scal *, bool);
+template void gs_colour<float>(int, const int *, const int *, const int *, const float *, const float *, float *,
+                               double, Scal *, bool);
 template <typename T>
 void sor_first(int n, const uint8_t *first, const T *piv, const T *r, T *z, Scal *s, bool check_done) {
   if (n == 0) return;

@@ -56,6 +56,18 @@ void rect_conductance_device(int64_t n, const double *w, const double *h, const 
 void network_gradient(int64_t ne, const int32_t *ei, const int32_t *ej, const uint8_t *isdir, const double *p,
                       const double *lam, const double *explicit_dg, const double *dg_dparam, double *out);
 
+// Edge flows (p_i - p_j) / (R + k |q_prev|) into q (k nullable: R only);
+// throws CFDLA_ERR_CONFIG, writing nothing, when an edge breaks the
+// reference's invariants.  bad: one device int of scratch.
+void network_flows(int64_t ne, const int32_t *ei, const int32_t *ej, const double *p, const double *R,
+                   const double *kq, double *q, int *bad);
+// Outlet flux per network, J[s], and optionally dJ/dp (per node, 0 on
+// Dirichlet nodes) and the explicit dJ/dG (per edge).  eoff: nsys + 1 edge
+// offsets; inc_ptr / inc_edge: each node's incident edges in edge order.
+void network_outlet(int64_t nsys, int64_t nrows, const int64_t *eoff, const int32_t *ei, const int32_t *ej,
+                    const int64_t *inc_ptr, const int32_t *inc_edge, const uint8_t *isdir, const uint8_t *mask,
+                    const double *p, const double *g, double *J, double *dJdp, double *dJdg);
+
 struct BatchSchedule {
   int32_t wave_max_rows = 256;  // systems with n <= this are solved by one wavefront
   int32_t blocks_per_cu = 2;    // persistent grid = blocks_per_cu * CUs

@@ -360,6 +360,60 @@ __global__ __launch_bounds__(kBlock) void k_gj_extract(int n, const double *m, d
   }
 }
 
+
+// ---- variable-coefficient Poisson rows --------------------------------------
+// One lane per cell: the row of cfdla_problem_varcoef (face coefficient
+// 2 kP kN / (kP + kN) * inv_h2; diagonal summed over the faces -z, -y, -x,
+// +x, +y, +z, then the Dirichlet faces in the same order; values written in
+// column order) into v64 from rp[p], and the Dirichlet right-hand side.  FP
+// contraction is off, as in the host generator, so the values are the
+// host's bit for bit.
+__global__ __launch_bounds__(kBlock) void k_varcoef_rows(int64_t nx, int64_t ny, int64_t nz, double inv_h2,
+                                                         const double *__restrict__ k, int dir_mask, double dv0,
+                                                         double dv1, double dv2, double dv3, double dv4,
+                                                         double dv5, const int32_t *__restrict__ rp,
+                                                         double *__restrict__ v64, double *__restrict__ rhs) {
+#pragma clang fp contract(off)
+  const int64_t n = nx * ny * nz, sy = nx, sz = nx * ny;
+  const bool three_d = nz > 1;
+  const double dv[6] = {dv0, dv1, dv2, dv3, dv4, dv5};
+  for (int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
+    const int64_t x = p % nx, y = (p / nx) % ny, z = p / sz;
+    const double kp = k[p];
+    const bool has[6] = {three_d && z > 0, y > 0, x > 0, x + 1 < nx, y + 1 < ny, three_d && z + 1 < nz};
+    const int64_t nb[6] = {p - sz, p - sy, p - 1, p + 1, p + sy, p + sz};
+    double c[6];
+    double diag = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      c[f] = 0.0;
+      if (has[f]) {
+        const double kq = k[nb[f]];
+        c[f] = (2.0 * kp * kq / (kp + kq)) * inv_h2;
+        diag += c[f];
+      }
+    }
+    const bool on[6] = {three_d && z == 0, y == 0, x == 0, x == nx - 1, y == ny - 1, three_d && z == nz - 1};
+    const int bit[6] = {4, 2, 0, 1, 3, 5};
+    double bc = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+      if (on[f] && ((dir_mask >> bit[f]) & 1)) {
+        const double cd = kp * inv_h2;
+        diag += cd;
+        bc += cd * dv[bit[f]];
+      }
+    if (rhs) rhs[p] = bc;
+    int64_t o = rp[p];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+      if (has[f]) v64[o++] = -c[f];
+    v64[o++] = diag;
+#pragma unroll
+    for (int f = 3; f < 6; ++f)
+      if (has[f]) v64[o++] = -c[f];
+  }
+}
 }  // namespace
 
 void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *v64) {
@@ -397,6 +451,15 @@ void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, c
   if (!want_max) return;
   g_max.ensure(1);
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, stream(), grid, ws.partials.p, g_max.p);
+}
+
+void varcoef_refresh(int64_t nx, int64_t ny, int64_t nz, double inv_h2, const double *k, int dir_mask,
+                     const double *dir_val, const int32_t *rp, double *v64, double *rhs) {
+  const double z6[6] = {0, 0, 0, 0, 0, 0};
+  const double *d = dir_val ? dir_val : z6;
+  hipLaunchKernelGGL(k_varcoef_rows, dim3(grid_for(nx * ny * nz)), dim3(kBlock), 0, stream(), nx, ny, nz, inv_h2, k,
+                     dir_mask, d[0], d[1], d[2], d[3], d[4], d[5], rp, v64, rhs);
+  CFDLA_HIP(hipGetLastError());
 }
 
 double simple_max_read() {

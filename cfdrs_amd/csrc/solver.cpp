@@ -233,9 +233,14 @@ bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t
 namespace {
 // Diagonal layout (see DiaDesc) for square matrices of at least 4096 rows
 // whose nonzeros lie on few diagonals filled to at least 90 %.  Returns the
-// stored diagonals diagonal-major in val (nd x n).
+// stored diagonals diagonal-major in val (nd x n).  A matrix that stores an
+// explicit zero is left to the indexed formats: the diagonal kernels skip
+// zero values (the slots of absent entries), while a CSR product multiplies
+// a stored zero, which turns a non-finite x into NaN.
 bool build_dia(const HostCsr &h, DiaDesc &d, std::vector<double> &val, bool allow_sym) {
   if (h.n != h.m || h.n < 4096 || h.nnz() == 0) return false;
+  for (const double a : h.v)
+    if (a == 0.0) return false;
   const int64_t n = h.n;
   std::vector<int64_t> offs;
   for (int64_t i = 0; i < n; ++i)
@@ -315,7 +320,8 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
-  c16_f64 = build_cols16(h, blocks, c16v, cb);
+  const bool allow16 = fmt_request != CFDLA_STORAGE_CSR_INT32;
+  c16_f64 = allow16 && build_cols16(h, blocks, c16v, cb);
   if (c16_f64) {
     c16_64.upload(c16v.data(), c16v.size());
     cbase64.upload(cb.data(), cb.size());
@@ -329,7 +335,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks);
     nrb32 = (int)blocks.size() / 2 - 1;
     rb32.upload(blocks.data(), blocks.size());
-    c16_f32 = build_cols16(h, blocks, c16v, cb);
+    c16_f32 = allow16 && build_cols16(h, blocks, c16v, cb);
     if (c16_f32) {
       c16_32.upload(c16v.data(), c16v.size());
       cbase32.upload(cb.data(), cb.size());
@@ -483,11 +489,10 @@ JacobiPrecond::JacobiPrecond(const HostCsr &a) {
 void JacobiPrecond::apply(const double *r, double *z, Scal *s, bool chk) { jacobi_first<double>(n, d64.p, 1.0, r, z, s, chk); }
 void JacobiPrecond::apply(const float *r, float *z, Scal *s, bool chk) { jacobi_first<float>(n, d32.p, 1.0, r, z, s, chk); }
 
-ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
-  CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "SOR requires a square matrix");
-  CFDLA_REQUIRE(std::isfinite(omega) && omega > 0.0 && omega < 2.0, CFDLA_ERR_CONFIG,
-                "SOR relaxation factor must lie in (0, 2)");
-  n = (int)a.n;
+int greedy_colouring(const HostCsr &a, std::vector<int32_t> &colour, std::vector<int32_t> &colour_off,
+                     std::vector<int32_t> &order) {
+  const int n = (int)a.n;
+  int ncolours = 0;
   // symmetrized adjacency (couplings in either direction)
   std::vector<int64_t> deg((size_t)n + 1, 0);
   for (int64_t i = 0; i < a.n; ++i)
@@ -528,7 +533,7 @@ ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
       }
     }
   }
-  std::vector<int32_t> colour((size_t)n, -1);
+  colour.assign((size_t)n, -1);
   std::vector<int32_t> seen;  // seen[c] == i: colour c is taken by a neighbour of i
   for (const int i : bfs) {
     for (int64_t k = deg[(size_t)i]; k < deg[(size_t)i + 1]; ++k) {
@@ -546,9 +551,19 @@ ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
   colour_off.assign((size_t)ncolours + 1, 0);
   for (int i = 0; i < n; ++i) colour_off[(size_t)colour[(size_t)i] + 1]++;
   for (int c = 0; c < ncolours; ++c) colour_off[(size_t)c + 1] += colour_off[(size_t)c];
-  std::vector<int32_t> order((size_t)n);
+  order.assign((size_t)n, 0);
   std::vector<int32_t> pos(colour_off.begin(), colour_off.end() - 1);
   for (int i = 0; i < n; ++i) order[(size_t)pos[(size_t)colour[(size_t)i]]++] = i;
+  return ncolours;
+}
+
+ColoredSorPrecond::ColoredSorPrecond(const HostCsr &a, double omega) {
+  CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "SOR requires a square matrix");
+  CFDLA_REQUIRE(std::isfinite(omega) && omega > 0.0 && omega < 2.0, CFDLA_ERR_CONFIG,
+                "SOR relaxation factor must lie in (0, 2)");
+  n = (int)a.n;
+  std::vector<int32_t> colour, order;
+  ncolours = greedy_colouring(a, colour, colour_off, order);
   // pivots omega / a_ii; a row without a (nonzero) diagonal takes a unit
   // pivot, as SuccessiveOverRelaxation::from_csr_with_identity_rows does
   std::vector<double> piv((size_t)n, omega);
@@ -614,8 +629,9 @@ AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
   CFDLA_REQUIRE(c.coarsening >= 0 && c.coarsening <= 3, CFDLA_ERR_CONFIG, "AMG coarsening must be 0, 1, 2 or 3");
   CFDLA_REQUIRE(c.interpolation >= 0 && c.interpolation <= 2, CFDLA_ERR_CONFIG,
                 "AMG interpolation must be 0, 1 or 2");
-  CFDLA_REQUIRE(c.smoother == 2, CFDLA_ERR_CONFIG,
-                "the device V-cycle smooths with Jacobi only (smoother = 2); Gauss-Seidel sweeps are sequential");
+  CFDLA_REQUIRE(c.smoother >= 0 && c.smoother <= 4, CFDLA_ERR_CONFIG, "AMG smoother must be 0, 1, 2, 3 or 4");
+  CFDLA_REQUIRE(std::isfinite(c.relaxation_factor) && c.relaxation_factor > 0.0, CFDLA_ERR_CONFIG,
+                "AMG relaxation factor must be positive");
   CFDLA_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, CFDLA_ERR_CONFIG, "AMG coarse_solver must be 0 or 1");
   auto t0 = std::chrono::steady_clock::now();
   GalerkinFn galerkin;
@@ -676,6 +692,11 @@ void AmgPrecond::upload() {
     if (l + 1 < L) {
       d.P.upload(h.P, true, false);
       d.R.upload(h.R, true, false);
+    }
+    if (cfg.smoother != 2) {
+      std::vector<int32_t> colour, order;
+      greedy_colouring(h.A, colour, d.gs_off, order);
+      d.gs_perm.upload(order.data(), std::max<size_t>(1, order.size()));
     }
     size_t n = (size_t)h.A.n;
     if (l > 0) {
@@ -776,7 +797,53 @@ T *AmgPrecond::first_sweep_target(int l) {
 }
 
 template <typename T>
+void AmgPrecond::gs_sweeps(DevLevel &L, const T *b, T *x, int sweeps, Scal *s, bool chk) {
+  const int nc = (int)L.gs_off.size() - 1;
+  const double w = cfg.relaxation_factor;
+  auto colour = [&](int c) {
+    gs_colour<T>(L.gs_off[(size_t)c + 1] - L.gs_off[(size_t)c], L.gs_perm.p + L.gs_off[(size_t)c], L.A.rp.p, L.A.ci.p,
+                 L.A.vals<T>(), b, x, w, s, chk);
+  };
+  for (int k = 0; k < sweeps; ++k)
+    for (int c = 0; c < nc; ++c) colour(c);
+  if (cfg.smoother == 1)  // symmetric: the backward sweeps follow (smoothers.rs:97-137)
+    for (int k = 0; k < sweeps; ++k)
+      for (int c = nc - 1; c >= 0; --c) colour(c);
+}
+
+template <typename T>
+void AmgPrecond::vcycle_gs(int l, const T *b, T *x, Scal *s, bool chk, int dot_op) {
+  DevLevel &L = lv[(size_t)l];
+  RedWs &ws = ctx().ws;
+  const int n = L.A.n;
+  const bool coarsest = (size_t)l + 1 == lv.size();
+  if (coarsest && cfg.coarse_solver == 1) {
+    dense_gemv<T>(n, coarse_inv.p, b, x, s, chk);
+  } else {
+    // amg.rs:423-503 from the zero guess (amg.rs:553): pre-smoothing, and on
+    // the coarsest level 10 more sweeps (amg.rs:501)
+    zero_vec<T>(n, x, s, chk);
+    gs_sweeps<T>(L, b, x, cfg.pre_smooth_iterations, s, chk);
+    if (coarsest) {
+      gs_sweeps<T>(L, b, x, 10, s, chk);
+    } else {
+      DevLevel &C = lv[(size_t)l + 1];
+      csr_apply<T>(L.A, EPI_RESID, x, b, nullptr, 0.0, L.r<T>(), s, chk, OP_NONE, nullptr, ws);
+      csr_apply<T>(L.R, EPI_Y, L.r<T>(), nullptr, nullptr, 0.0, C.b<T>(), s, chk, OP_NONE, nullptr, ws);
+      vcycle_gs<T>(l + 1, C.b<T>(), C.x<T>(), s, chk, OP_NONE);
+      csr_apply<T>(L.P, EPI_ADD, C.x<T>(), nullptr, nullptr, 0.0, x, s, chk, OP_NONE, nullptr, ws);
+      gs_sweeps<T>(L, b, x, cfg.post_smooth_iterations, s, chk);
+    }
+  }
+  if (dot_op != OP_NONE) dot2<T>(n, b, x, nullptr, nullptr, s, chk, dot_op, ws);
+}
+
+template <typename T>
 void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op, bool first_ready) {
+  if (cfg.smoother != 2) {
+    vcycle_gs<T>(l, b, x, s, chk, dot_op);
+    return;
+  }
   DevLevel &L = lv[(size_t)l];
   RedWs &ws = ctx().ws;
   const int n = L.A.n;
@@ -809,7 +876,7 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op, 
   } else if (down >= 1) {
     if (!first_ready) jacobi_first<T>(n, dinv, w, b, buf[cur], s, chk);
   } else {
-    fill_vec<T>(n, T(0), buf[cur]);
+    zero_vec<T>(n, buf[cur], s, chk);
   }
   for (; k < down; ++k) {
     csr_apply<T>(L.A, EPI_JACOBI, buf[cur], b, dinv, w, buf[cur ^ 1], s, chk, OP_NONE, nullptr, ws);
@@ -844,6 +911,8 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op, 
 }
 
 template void AmgPrecond::vcycle<double>(int, const double *, double *, Scal *, bool, int, bool);
+template void AmgPrecond::vcycle_gs<double>(int, const double *, double *, Scal *, bool, int);
+template void AmgPrecond::vcycle_gs<float>(int, const float *, float *, Scal *, bool, int);
 template void AmgPrecond::vcycle<float>(int, const float *, float *, Scal *, bool, int, bool);
 
 // ---------------------------------------------------------------------------
@@ -858,6 +927,19 @@ template <> float *vec<float>(int slot, size_t n) { return ctx().fv_at(slot, n);
 template <typename T> const T *jdinv(const Precond *M);
 template <> const double *jdinv<double>(const Precond *M) { return M->jacobi_dinv64(); }
 template <> const float *jdinv<float>(const Precond *M) { return M->jacobi_dinv32(); }
+
+// From the lagged poll: does the chunk enqueued last probably reach the
+// threshold?  Near convergence the residual falls about geometrically, so
+// rr_next ~ rr (rr / rr_prev).  The host then waits for that chunk's state
+// instead of enqueuing another chunk whose kernels would only exit early.
+struct ConvergenceGate {
+  double prev = -1.0;
+  bool expect(double rr, double thr2) {
+    const bool hit = prev > 0.0 && rr > 0.0 && rr < prev && rr * (rr / prev) <= thr2;
+    prev = rr;
+    return hit;
+  }
+};
 
 int auto_chunk(const Precond *M, int n) {
   if (M->kind() == 2) return 1;
@@ -898,6 +980,7 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
   if (chunk <= 0) chunk = auto_chunk(M, n);
   int64_t enq = 0;
   Poller poll(s);
+  ConvergenceGate gate;
   bool finished = false;
   while (!finished) {
     const Scal *h = nullptr;
@@ -920,7 +1003,11 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
       h = poll.latest();
     }
     if (!h) continue;
-    if (h->done == 0 && enq < max_it) continue;
+    if (h->done == 0 && enq < max_it) {
+      if (!gate.expect(h->rr, thr * thr)) continue;
+      h = poll.latest();
+      if (h->done == 0) continue;
+    }
     sync_stream();
     Scal now = c.read(s);
     o.iters = now.iters;
@@ -1018,6 +1105,7 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
   int32_t replacements = 0;  // enqueued, including those the device skips after stopping
   bool want = false;         // a reduction by delta was seen: replace in the next step
   Poller poll(s);
+  ConvergenceGate gate;
   for (;;) {
     const Scal *h = nullptr;
     if (enq < max_it) {
@@ -1053,17 +1141,24 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
       const double ref = h->nupd > 0 ? h->rr_true : o.rn0 * o.rn0;
       const double de = std::min(0.5, delta * std::pow(10.0, (double)h->nrestart));
       if (delta > 0.0 && h->nupd >= replacements && h->rr <= de * de * ref) want = true;
-      continue;
+      if (!gate.expect(h->rr, thr * thr)) continue;
+      h = poll.latest();
+      if (h->done == 0) continue;
     }
     if (h->done == 1 && enq < max_it) {
-      // the FP32 residual passed the threshold: confirm with the FP64 one
-      // and continue the iteration from it if it is not there yet
+      // the FP32 residual passed the threshold: confirm with the FP64 one,
+      // read it before enqueuing more work, and continue the iteration from
+      // it if it is not there yet
       ++replacements;
       replace(false);
-      direction();
-      poll = Poller(s);
-      poll.post();  // the replaced state, so latest() is defined once the budget is spent
-      continue;
+      const Scal conf = c.read(s);
+      if (conf.done != 4) {
+        direction();
+        poll = Poller(s);
+        gate = ConvergenceGate();
+        poll.post();  // the replaced state, so latest() is defined once the budget is spent
+        continue;
+      }
     }
     sync_stream();
     Scal now = c.read(s);

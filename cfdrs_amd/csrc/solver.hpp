@@ -69,8 +69,19 @@ struct ColoredSorPrecond : Precond {
   void apply(const float *r, float *z, Scal *s, bool check_done) override;
 };
 
+// Greedy colouring in breadth-first order on the symmetrized pattern of a:
+// colour per row, colour_off (ncolours + 1) and the rows grouped by colour
+// (ascending within a colour); returns ncolours.  No two coupled rows share
+// a colour; a bipartite pattern (a 5-point stencil) gets two colours.
+int greedy_colouring(const HostCsr &a, std::vector<int32_t> &colour, std::vector<int32_t> &colour_off,
+                     std::vector<int32_t> &order);
+
 struct DevLevel {
   DevCsr A, P, R;
+  // Gauss-Seidel smoothers: colour classes of A's rows (gs_perm grouped by
+  // colour, gs_off offsets); empty for the Jacobi smoother.
+  std::vector<int32_t> gs_off;
+  DevBuf<int32_t> gs_perm;
   DevBuf<double> b64, x64, t64, r64;
   DevBuf<float> b32, x32, t32, r32;
   template <typename T> T *b();
@@ -143,6 +154,11 @@ struct AmgPrecond : Precond {
   // Jacobi iterate from zero into first_sweep_target(l).
   template <typename T> void vcycle(int l, const T *b, T *x, Scal *s, bool check_done, int dot_op,
                                     bool first_ready = false);
+  // The V-cycle with a Gauss-Seidel family smoother (multicolour sweeps in
+  // place on x; amg.rs:382-391 maps every smoother but Jacobi and symmetric
+  // Gauss-Seidel to forward Gauss-Seidel).
+  template <typename T> void vcycle_gs(int l, const T *b, T *x, Scal *s, bool check_done, int dot_op);
+  template <typename T> void gs_sweeps(DevLevel &L, const T *b, T *x, int sweeps, Scal *s, bool check_done);
   // Buffer that level l's V-cycle writes its first Jacobi iterate to when it
   // starts with a plain sweep from zero; nullptr when it starts otherwise.
   template <typename T> T *first_sweep_target(int l);

@@ -87,6 +87,13 @@ def rect_conductance_device(width, height, length, viscosity=WATER_VISCOSITY):
     return g, dg
 
 
+def _require_cuda_f64(t, n: int, name: str) -> None:
+    import torch
+
+    if not (_is_torch(t) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n):
+        raise N.CfdlaError(N.ERR_INVALID_ARG, f"{name} must be a contiguous float64 CUDA tensor of {n}")
+
+
 def _report(r: N.Report) -> SolveReport:
     return SolveReport(r.iterations, r.initial_residual_norm, r.final_residual_norm, r.threshold,
                        Termination(r.termination), r.outer_iterations)
@@ -200,6 +207,48 @@ class NetworkBatch:
                None if dg_dparam is None else dg_dparam.data_ptr(), out.data_ptr())
         return out
 
+    def edge_flows(self, p, resistance, quad_coeff=None, flow=None):
+        """Edge flows from device pressures p, as update_from_solution forms
+        them (wrapper.rs:349-428): (p_i - p_j) / (R + k |q_prev|) with the
+        previous flows taken from ``flow`` (required when quad_coeff is given)
+        and the new ones written there.  Float64 CUDA tensors; returns flow."""
+        import torch
+
+        _require_cuda_f64(p, self.n_rows, "p")
+        _require_cuda_f64(resistance, self.edge_i.size, "resistance")
+        if quad_coeff is not None:
+            _require_cuda_f64(quad_coeff, self.edge_i.size, "quad_coeff")
+            if flow is None:
+                raise N.CfdlaError(N.ERR_INVALID_ARG, "a quadratic coefficient needs the previous flows")
+        if flow is None:
+            flow = torch.zeros(self.edge_i.size, dtype=torch.float64, device=p.device)
+        _require_cuda_f64(flow, self.edge_i.size, "flow")
+        bind_torch_stream()
+        N.call("cfdla_batch_edge_flows_device", self.handle, p.data_ptr(), resistance.data_ptr(),
+               None if quad_coeff is None else quad_coeff.data_ptr(), flow.data_ptr())
+        return flow
+
+    def outlet_flux(self, p, conductance, outlet_mask, derivatives: bool = False):
+        """Per-network flux into the nodes flagged in outlet_mask (uint8 CUDA
+        tensor per node) from device pressures and conductances; with
+        derivatives, also dJ/dp (per node, 0 on Dirichlet nodes) and the
+        explicit dJ/dG (per edge)."""
+        import torch
+
+        _require_cuda_f64(p, self.n_rows, "p")
+        _require_cuda_f64(conductance, self.edge_i.size, "conductance")
+        if not (_is_torch(outlet_mask) and outlet_mask.is_cuda and outlet_mask.dtype == torch.uint8
+                and outlet_mask.is_contiguous() and outlet_mask.numel() == self.n_rows):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "outlet_mask must be a contiguous uint8 CUDA tensor of n_rows")
+        J = torch.empty(self.n_sys, dtype=torch.float64, device=p.device)
+        dp = torch.empty_like(p) if derivatives else None
+        dg = torch.empty_like(conductance) if derivatives else None
+        bind_torch_stream()
+        N.call("cfdla_batch_outlet_flux_device", self.handle, p.data_ptr(), conductance.data_ptr(),
+               outlet_mask.data_ptr(), J.data_ptr(), None if dp is None else dp.data_ptr(),
+               None if dg is None else dg.data_ptr())
+        return (J, dp, dg) if derivatives else J
+
     # ---- autotuning -------------------------------------------------------------------
     DEFAULT_CANDIDATES = [(w, b) for w in (0, 64, 128, 256, 512) for b in (1, 2, 4)]
 
@@ -274,3 +323,84 @@ def outlet_flux(nets: SynthNetworks, g, p):
     touch = at_i | at_j
     contrib = np.where(touch, g * (p[other] - p[o + base]), 0.0)
     return np.bincount(sys_of_edge, weights=contrib, minlength=nets.count)
+
+
+class WidthDesign:
+    """Outlet flux of every network as a function of the channel widths, on
+    the GPU: widths -> rectangular-duct conductances (rectangular.rs:95-157)
+    -> device re-assembly -> batched Jacobi-PCG -> per-network outlet flux.
+    ``flux(width)`` is a differentiable torch function whose backward pass is
+    one batched adjoint solve (A is symmetric on the free nodes; Dirichlet
+    rows are identity rows, so the adjoint vanishes there) and the edge
+    gradient kernel, times dG/dw.  No reference counterpart (the reference
+    has no adjoint code)."""
+
+    def __init__(self, batch: NetworkBatch, height, length, outlet_mask, viscosity=WATER_VISCOSITY,
+                 config: IterativeSolverConfig | None = None):
+        import torch
+
+        self.batch = batch
+        self.height = height
+        self.length = length
+        self.viscosity = viscosity
+        self.outlet_mask = outlet_mask
+        self.config = config or IterativeSolverConfig.new(1e-12).with_max_iterations(50000)
+        dev = outlet_mask.device
+        self.sys_of_node = torch.from_numpy(np.repeat(np.arange(batch.n_sys), np.diff(batch.node_offset))).to(dev)
+        self.sys_of_edge = torch.from_numpy(np.repeat(np.arange(batch.n_sys), np.diff(batch.edge_offset))).to(dev)
+        self._g = None  # the conductances the batch currently holds
+
+    def _set(self, g) -> None:
+        if self._g is None or self._g.data_ptr() != g.data_ptr():
+            self.batch.update_conductance_device(g)
+            self._g = g
+
+    def _solve(self, x, rhs=None) -> None:
+        _, reps = self.batch.solve(x, self.config, rhs=rhs)
+        bad = [r for r in reps if r.termination not in (Termination.Converged, Termination.InitialResidual)]
+        if bad:
+            raise RuntimeError(f"{len(bad)} network solve(s) did not converge: {bad[0]}")
+
+    def forward(self, width):
+        """(J per network, p, G, dG/dw) for a float64 CUDA tensor of widths."""
+        import torch
+
+        g, dg = rect_conductance_device(width.detach(), self.height, self.length, self.viscosity)
+        self._set(g)
+        p = torch.zeros(self.batch.n_rows, dtype=torch.float64, device=width.device)
+        self._solve(p)
+        return self.batch.outlet_flux(p, g, self.outlet_mask), p, g, dg
+
+    def gradient(self, weights, p, g, dg):
+        """d(sum_s weights_s J_s)/dw: one batched adjoint solve and the edge
+        gradient kernel."""
+        import torch
+
+        self._set(g)
+        _, dJdp, dJdg = self.batch.outlet_flux(p, g, self.outlet_mask, derivatives=True)
+        rhs = (weights[self.sys_of_node] * dJdp).contiguous()
+        lam = torch.zeros_like(p)
+        self._solve(lam, rhs)
+        explicit = (weights[self.sys_of_edge] * dJdg).contiguous()
+        return self.batch.edge_gradient(p, lam, explicit, dg)
+
+    def flux(self, width):
+        """Differentiable per-network outlet flux J(width)."""
+        import torch
+
+        design = self
+
+        class OutletFluxOfWidths(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, w):
+                _require_cuda_f64(w, design.batch.edge_i.size, "width")
+                J, p, g, dg = design.forward(w)
+                ctx.save_for_backward(p, g, dg)
+                return J
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                p, g, dg = ctx.saved_tensors
+                return design.gradient(grad_out.contiguous(), p, g, dg)
+
+        return OutletFluxOfWidths.apply(width)

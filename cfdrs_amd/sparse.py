@@ -167,12 +167,33 @@ class CsrMatrix:
         return {"format": {0: "csr_rowblock", 1: "sell64", 2: "dia"}[fmt.value], "stored_entries": ent.value,
                 "units": units.value}
 
-    STORAGE_FORMATS = {"auto": -1, "csr_rowblock": 0, "sell64": 1, "dia": 2}
+    def varcoef_refresh_device(self, nx: int, ny: int, nz: int, inv_h2: float, k, dir_mask: int, dir_val,
+                               rhs=None) -> None:
+        """Rewrite the values (cfdla_problem_varcoef's pattern for this grid)
+        from a float64 CUDA tensor of cell coefficients k, on the device and
+        bit for bit as the host generator; rhs (float64 CUDA tensor or None)
+        receives the Dirichlet terms."""
+        import torch
+
+        if not (_is_torch(k) and k.is_cuda and k.dtype == torch.float64 and k.is_contiguous()
+                and k.numel() == nx * ny * nz):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "k must be a contiguous float64 CUDA tensor of nx*ny*nz")
+        if rhs is not None and not (_is_torch(rhs) and rhs.is_cuda and rhs.dtype == torch.float64
+                                    and rhs.is_contiguous() and rhs.numel() == nx * ny * nz):
+            raise N.CfdlaError(N.ERR_INVALID_ARG, "rhs must be a contiguous float64 CUDA tensor of nx*ny*nz")
+        dv = np.ascontiguousarray(np.broadcast_to(np.asarray(dir_val, np.float64), (6,)))
+        bind_torch_stream()
+        N.call("cfdla_varcoef_refresh_device", self.handle, nx, ny, nz, inv_h2, k.data_ptr(), dir_mask, N.ptr(dv),
+               None if rhs is None else rhs.data_ptr())
+        self._values = None  # the device copy is newer
+
+    STORAGE_FORMATS = {"auto": -1, "csr_rowblock": 0, "sell64": 1, "dia": 2, "csr_int32": 3}
 
     def set_storage(self, fmt: str) -> dict:
         """Restrict the device format ("auto", "csr_rowblock", "sell64",
-        "dia"); a format the matrix cannot take falls back to row-block CSR.
-        Returns storage().  Meant for timing one product kernel."""
+        "dia", or "csr_int32": row-block CSR with 32-bit columns only); a
+        format the matrix cannot take falls back to row-block CSR.  Returns
+        storage().  Meant for timing one product kernel."""
         if fmt not in self.STORAGE_FORMATS:
             raise ValueError(f"unknown storage format {fmt!r}")
         N.call("cfdla_matrix_set_storage", self.handle, self.STORAGE_FORMATS[fmt])

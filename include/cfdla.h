@@ -212,6 +212,9 @@ cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stor
  * one product kernel (for example the row-block CSR product on a stencil
  * matrix that would otherwise use diagonal storage). */
 #define CFDLA_STORAGE_AUTO (-1)
+/* row-block CSR that keeps 32-bit columns even where 16-bit block offsets
+ * would fit (the CSR layout the reference stores, int32 indices) */
+#define CFDLA_STORAGE_CSR_INT32 3
 cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format);
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
 
@@ -388,6 +391,27 @@ cfdla_status cfdla_batch_autotune(cfdla_batch b, const cfdla_solver_config *conf
 cfdla_status cfdla_batch_edge_gradient_device(cfdla_batch b, const double *p, const double *lambda,
                                               const double *explicit_dg, const double *dg_dparam,
                                               double *out);
+/* Edge flows from device node pressures p (packed, n_rows), as the
+ * reference's update_from_solution forms them
+ * (crates/cfd-1d/src/domain/network/wrapper.rs:349-428):
+ * flow_e = (p_i - p_j) / (R_e + k_e |flow_e|), with the previous flows read
+ * from flow_inout and the new ones written there; quad_coeff (k) may be NULL
+ * (linear channels: flow = (p_i - p_j) / R_e).  An edge with R < 0, k < 0,
+ * R and k both below machine epsilon, or a non-finite or non-positive
+ * effective resistance rejects the call (CFDLA_ERR_CONFIG) before any flow
+ * is written.  All arrays on the device, n_edges long. */
+cfdla_status cfdla_batch_edge_flows_device(cfdla_batch b, const double *p, const double *resistance,
+                                           const double *quad_coeff, double *flow_inout);
+/* Outlet flux of every network, flux[s] = sum over its edges e = (i, j) of
+ * G_e (m_j (p_i - p_j) + m_i (p_j - p_i)): the flow into the nodes whose
+ * outlet_mask (uint8 per node) is set, summed in a fixed order.  Optional
+ * outputs for adjoint gradients: dflux_dp (per node, 0 on Dirichlet nodes)
+ * and dflux_dg (per edge, the explicit derivative in G_e).  Device arrays;
+ * conductance holds the G_e the pressures were solved with.  No counterpart
+ * in the reference (its design objectives are host loops over the flows). */
+cfdla_status cfdla_batch_outlet_flux_device(cfdla_batch b, const double *p, const double *conductance,
+                                            const uint8_t *outlet_mask, double *flux, double *dflux_dp,
+                                            double *dflux_dg);
 cfdla_status cfdla_batch_destroy(cfdla_batch b);
 
 /* ---- problem generators (host, deterministic) -------------------------------- */
@@ -494,6 +518,18 @@ cfdla_status cfdla_problem_varcoef(int64_t nx, int64_t ny, int64_t nz, double in
                                    const double *k, int32_t dir_mask, const double *dir_val,
                                    int64_t *nnz, int64_t *row_ptr, int64_t *col_indices,
                                    double *values, double *rhs_bc);
+/* Rewrites the values of m, which must hold cfdla_problem_varcoef's pattern
+ * for this grid (checked on the first call), from device coefficients k_dev
+ * (nx*ny*nz doubles), on the device and bit for bit as
+ * cfdla_problem_varcoef forms them; rhs_bc_dev (device, nullable) receives
+ * the Dirichlet contributions.  Derived device copies (FP32, diagonal
+ * storage, smoother diagonal) follow; a preconditioner built on m is
+ * brought up to date with cfdla_amg_update_values_device.  No counterpart in
+ * the reference, which assembles on the host: it keeps design loops over
+ * the coefficients (adjoint gradients) on the GPU. */
+cfdla_status cfdla_varcoef_refresh_device(cfdla_matrix m, int64_t nx, int64_t ny, int64_t nz, double inv_h2,
+                                         const double *k_dev, int32_t dir_mask, const double *dir_val,
+                                         double *rhs_bc_dev);
 
 #ifdef __cplusplus
 }

@@ -360,3 +360,58 @@ def coef_grad_2d(nx, ny, inv_h2, dir_mask, dir_val, k, p, lam, dJdk=None):
        np.ascontiguousarray(p, np.float64), np.ascontiguousarray(lam, np.float64),
        None if dj is None else dj.ctypes.data, out)
     return out
+
+
+def rect_conductance_complex_step(w, h, L, mu, step: float = 1e-30):
+    """G = 1/R for rectangular channels (crates/cfd-1d/src/physics/resistance/
+    models/rectangular.rs:95-157,209-245: R = Po(eps) mu L / (2 A Dh^2),
+    Dh = 4 A / (2 (w + h)), Po = 96 / ((1 + eps)^2 (1 - 192 eps / pi^5
+    tanh(pi / (2 eps)))), eps = min(w, h) / max(w, h)) and dG/dw by the
+    complex-step derivative (Squire & Trapp, SIAM Rev. 40 (1998) 110):
+    dG/dw = Im G(w + i s) / s, exact to rounding for this analytic formula
+    and independent of any hand-derived chain rule.  Arrays of equal length."""
+    w = np.asarray(w, np.float64)
+    h = np.broadcast_to(np.asarray(h, np.float64), w.shape)
+    L = np.broadcast_to(np.asarray(L, np.float64), w.shape)
+
+    def resistance(wc):
+        area = wc * h
+        dh = 4.0 * area / (2.0 * (wc + h))
+        narrow = wc.real <= h
+        eps = np.where(narrow, wc / h, h / wc)
+        po = 96.0 / ((1.0 + eps) ** 2 * (1.0 - (192.0 * eps / np.pi ** 5) * np.tanh(np.pi / (2.0 * eps))))
+        return po * mu * L / (2.0 * area * dh * dh)
+
+    g = 1.0 / resistance(w.astype(np.complex128))
+    s = step * w
+    dg = (1.0 / resistance(w + 1j * s)).imag / s
+    return g.real.copy(), dg
+
+
+def network_outlet_gradient(n, ei, ej, width, height, length, mu, is_dir, dir_val, outlet):
+    """Outlet flux of one network and its gradient in the channel widths.
+
+    J = sum over edges e = (i, j) of G_e [m_j (p_i - p_j) + m_i (p_j - p_i)]
+    (the flow into the nodes with outlet mask m = 1, with flows G_e (p_i - p_j)
+    as the reference's update_from_solution forms them for linear channels,
+    crates/cfd-1d/src/domain/network/wrapper.rs:349-428), p from the oracle's
+    assembly (matrix_assembly.rs:163-247) and a dense FP64 solve.  Adjoint:
+    A^T lambda = dJ/dp (zero on Dirichlet rows), then
+    dJ/dG_e = dJ/dG_e|explicit - (lambda_i - lambda_j)(p_i - p_j) and
+    dJ/dw_e = dJ/dG_e dG_e/dw_e (complex step).  Returns (J, dJ/dw, p)."""
+    ei = np.asarray(ei, np.int64)
+    ej = np.asarray(ej, np.int64)
+    g, dg = rect_conductance_complex_step(width, height, length, mu)
+    a, rhs = assemble_network(n, ei, ej, g, is_dir, dir_val)
+    A = a.dense()
+    p = np.linalg.solve(A, rhs)
+    m = np.asarray(outlet, np.float64)
+    explicit = m[ej] * (p[ei] - p[ej]) + m[ei] * (p[ej] - p[ei])
+    J = float(np.sum(g * explicit))
+    dJdp = np.zeros(n)
+    np.add.at(dJdp, ei, g * (m[ej] - m[ei]))
+    np.add.at(dJdp, ej, g * (m[ei] - m[ej]))
+    dJdp[np.asarray(is_dir, bool)] = 0.0
+    lam = np.linalg.solve(A.T, dJdp)
+    dJdG = explicit - (lam[ei] - lam[ej]) * (p[ei] - p[ej])
+    return J, dJdG * dg, p

@@ -4,7 +4,8 @@ coefficient-gradient kernel, checked against the CPU oracle's adjoint
 gradient (relative 1e-6, both solves at >= 1e-10) and against central finite
 differences of GPU forward solves on 32 random directions (relative 1e-5
 plus a floor of 1e-8 |grad| |d| for nearly cancelling directions),
-and through torch.autograd."""
+and through torch.autograd.  The device assembly of the operator is checked
+bit for bit against the host generator."""
 import numpy as np
 import pytest
 
@@ -17,12 +18,59 @@ pytestmark = pytest.mark.gpu
 NX = NY = 256
 
 
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
 @pytest.fixture(scope="module")
 def setup():
     g = P.grf(NX, NY, 16.0, seed=P.SEED_C5)
     prob = ADJ.FluxProblem(NX, NY, tol=1e-12)
     J, grad, p, lam = ADJ.flux_and_gradient(prob, g)
     return g, prob, J, grad.cpu().numpy(), p.cpu().numpy()
+
+
+@pytest.mark.parametrize("dims,mask", [((37, 29, 1), 0b0011), ((20, 17, 13), 0b010001), ((64, 64, 1), 0b1111)])
+def test_varcoef_device_assembly_bit_exact(dims, mask):
+    import torch
+
+    import cfdrs_amd as ca
+
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    ih2 = float(nx * nx)
+    dv = np.array([1.0, -2.0, 0.5, 3.0, -1.5, 2.5])
+    rng = np.random.default_rng(n)
+    rp, ci, v, _ = P.varcoef(nx, ny, nz, ih2, np.exp(rng.uniform(-2, 2, n)), mask, dv)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    for seed in (1, 2):  # two refreshes in a row
+        k = np.exp(np.random.default_rng(seed).uniform(-2.3, 2.3, n))
+        rp2, ci2, v2, rhs2 = P.varcoef(nx, ny, nz, ih2, k, mask, dv)
+        rhs = torch.full((n,), 9.0, dtype=torch.float64, device="cuda")
+        a.varcoef_refresh_device(nx, ny, nz, ih2, torch.from_numpy(k).cuda(), mask, dv, rhs)
+        assert np.array_equal(_bits(a.values), _bits(v2))
+        assert np.array_equal(_bits(rhs.cpu().numpy()), _bits(rhs2))
+        x = rng.uniform(-1, 1, n)
+        y = a.spmv(torch.tensor(x, device="cuda")).cpu().numpy()
+        yo = O.Csr.from_arrays(rp2, ci2, v2, n).spmv(x)
+        assert np.allclose(y, yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+
+
+def test_varcoef_device_assembly_rejects_foreign_pattern():
+    import torch
+
+    import cfdrs_amd as ca
+    from cfdrs_amd import _native as N
+
+    nx, ny = 12, 10
+    n = nx * ny
+    rp, ci, v, _ = P.varcoef(nx, ny, 1, 1.0, np.ones(n), 0, np.zeros(6))
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    k = torch.ones(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(N.CfdlaError) as e:  # same size, other grid shape
+        a.varcoef_refresh_device(ny, nx, 1, 1.0, k, 0, np.zeros(6))
+    assert e.value.status == N.ERR_INVALID_ARG
+    assert np.array_equal(a.values, v)
 
 
 def test_gradient_matches_oracle_adjoint(setup):
@@ -38,13 +86,18 @@ def test_gradient_matches_oracle_adjoint(setup):
 
 
 def test_gradient_matches_central_differences(setup):
+    import torch
+
     g, prob, J, grad, _ = setup
     rng = np.random.default_rng(5000)
     h = 1e-4
+    gd = torch.from_numpy(g).cuda()
     for d_i in range(32):
         d = rng.normal(size=NX * NY)
-        jp = ADJ.flux_and_gradient(prob, g + h * d)[0]
-        jm = ADJ.flux_and_gradient(prob, g - h * d)[0]
+        dd = torch.from_numpy(d).cuda()
+        # forward solves only: device assembly, hierarchy recompute, one solve
+        jp = float(prob.forward(gd + h * dd)[0])
+        jm = float(prob.forward(gd - h * dd)[0])
         fd = (jp - jm) / (2 * h)
         # relative 1e-5, with a floor of 1e-8 |grad| |d| (the Cauchy-Schwarz
         # scale of grad . d) for directions where the derivative nearly
@@ -59,6 +112,11 @@ def test_torch_autograd_function(setup):
     g, prob, J, grad, _ = setup
     gt = torch.tensor(g, dtype=torch.float64, device="cuda", requires_grad=True)
     out = ADJ.outlet_flux(gt, prob)
+    assert out.is_cuda and out.dim() == 0
+    # another forward on other coefficients before the backward: the
+    # backward re-assembles the operator of its own forward pass
+    other = ADJ.outlet_flux(torch.tensor(g + 0.1, dtype=torch.float64, device="cuda", requires_grad=True), prob)
+    assert abs(other.item() - J) > 1e-6 * abs(J)
     (3.0 * out).backward()
     assert abs(out.item() - J) <= 1e-12 * abs(J)
-    assert np.allclose(gt.grad.cpu().numpy(), 3.0 * grad, rtol=1e-12, atol=0)
+    assert np.allclose(gt.grad.cpu().numpy(), 3.0 * grad, rtol=1e-10, atol=1e-10 * np.abs(grad).max())

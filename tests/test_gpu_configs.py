@@ -1,0 +1,184 @@
+"""GPU parity on the workloads as configured: the lid-driven cavity systems
+(momentum BiCGSTAB / GMRES(30) with Jacobi, pinned pressure-correction
+AMG-PCG; crates/cfd-2d/src/physics/momentum/solve.rs:137-167,
+crates/cfd-2d/src/pressure_velocity/correction.rs:273-356), the 3D
+variable-coefficient AMG-PCG at 64^3 against an oracle solve (in the manner
+of crates/cfd-math/tests/amg_integration_test.rs:97-143: relative residual
+1e-10, then the error against the reference solution), and the transpose
+solve of a nonsymmetric matrix.
+
+Tolerances: every solve reaches an explicit FP64 relative residual <= 1e-10
+recomputed with the oracle's SpMV; solution errors are bounded against an
+oracle solve at 1e-13 as stated per test (condition-number scaled).
+"""
+import numpy as np
+import pytest
+
+import cfdrs_amd as ca
+from cfdrs_amd import _native as N
+from cfdrs_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_res(orc, x, b):
+    return np.linalg.norm(b - orc.spmv(x)) / np.linalg.norm(b)
+
+
+def rel_err(x, xo):
+    return np.linalg.norm(x - xo) / np.linalg.norm(xo)
+
+
+PRECISIONS = [ca.Precision.FP64, ca.Precision.FP32_REFINED]
+
+
+def _cavity(n):
+    (mrp, mci, mv, bm), (prp, pci, pv, bp) = P.config3(n)
+    nm, npr = bm.size, bp.size
+    return ((ca.CsrMatrix.from_parts(mv, mci, mrp, nm, nm), O.Csr.from_arrays(mrp, mci, mv, nm), bm),
+            (ca.CsrMatrix.from_parts(pv, pci, prp, npr, npr), O.Csr.from_arrays(prp, pci, pv, npr), bp))
+
+
+@pytest.fixture(scope="module")
+def cavity128():
+    return _cavity(128)
+
+
+@pytest.mark.parametrize("kind", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cavity_momentum_128(cavity128, kind, precision):
+    (a, orc, b), _ = cavity128
+    xo, rep_o = O.solve(orc, b, kind="gmres", prec="jacobi", tol_rel=1e-13, max_it=20000, restart=60)
+    assert rel_res(orc, xo, b) <= 1e-13
+    x = np.zeros(b.size)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+    jac = ca.JacobiPreconditioner(a)
+    if kind == "bicgstab":
+        rep = ca.bicgstab_preconditioned(a, b, jac, x, cfg)
+    else:
+        rep = ca.gmres_preconditioned(a, b, jac, x, cfg, 30)
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    # error against the 1e-13 oracle solve: kappa(A) times the 1e-10
+    # residual bounds it; measured 5e-9 .. 1.3e-8 (kappa ~ 1e2)
+    assert rel_err(x, xo) <= 1e-7
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cavity_pressure_amg_pcg_128(cavity128, precision):
+    _, (a, orc, b) = cavity128
+    cfg_amg = ca.AMGConfig.smoothed_aggregation()
+    amg_o = O.Amg(orc, coarsening="three_phase", theta=cfg_amg.strength_threshold, interp="smoothed",
+                  smoother="jacobi", omega=cfg_amg.relaxation_factor, coarse_solve="direct",
+                  sa_omega=cfg_amg.smoothing_omega, min_coarse=cfg_amg.min_coarse_size)
+    xo, rep_o = O.solve(orc, b, kind="cg", prec="amg", amg=amg_o, tol_rel=1e-13, max_it=1000)
+    assert rel_res(orc, xo, b) <= 1e-13
+    amg = ca.AlgebraicMultigrid(a, cfg_amg)
+    assert amg.statistics.level_sizes == [amg_o.level(l).shape[0] for l in range(amg_o.num_levels)]
+    x = np.zeros(b.size)
+    rep = ca.cg_preconditioned(a, b, amg, x, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500)
+                               .with_precision(precision))
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    # pinned pressure Laplacian, kappa ~ 1e4 at 126^2 cells
+    assert rel_err(x, xo) <= 1e-6
+    # the oracle's own 1e-10 solve takes the same number of iterations within two
+    _, rep_o10 = O.solve(orc, b, kind="cg", prec="amg", amg=amg_o, tol_rel=1e-10, max_it=1000)
+    assert abs(rep.iterations - rep_o10.iterations) <= 2
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cavity_full_size_512(precision):
+    (am, orc_m, bm), (ap_, orc_p, bp) = _cavity(512)
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+    jac = ca.JacobiPreconditioner(am)
+    for kind in ("bicgstab", "gmres"):
+        x = np.zeros(bm.size)
+        rep = (ca.bicgstab_preconditioned(am, bm, jac, x, cfg) if kind == "bicgstab"
+               else ca.gmres_preconditioned(am, bm, jac, x, cfg, 30))
+        assert rep.termination == ca.Termination.Converged, (kind, rep)
+        assert rel_res(orc_m, x, bm) <= 1e-10, kind
+    amg = ca.AlgebraicMultigrid(ap_, ca.AMGConfig.smoothed_aggregation())
+    x = np.zeros(bp.size)
+    rep = ca.cg_preconditioned(ap_, bp, amg, x, cfg.with_max_iterations(500))
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc_p, x, bp) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_amg_pcg_config2_64_cubed_against_oracle(precision):
+    nn = 64
+    n = nn ** 3
+    rp, ci, v, b = P.config2(nn)
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    cfg_amg = ca.AMGConfig.smoothed_aggregation()
+    amg_o = O.Amg(orc, coarsening="three_phase", theta=cfg_amg.strength_threshold, interp="smoothed",
+                  smoother="jacobi", omega=cfg_amg.relaxation_factor, coarse_solve="direct",
+                  sa_omega=cfg_amg.smoothing_omega, min_coarse=cfg_amg.min_coarse_size)
+    O.set_threads(8)
+    try:
+        xo, _ = O.solve(orc, b, kind="cg", prec="amg", amg=amg_o, tol_rel=1e-13, max_it=1000)
+    finally:
+        O.set_threads(1)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    amg = ca.AlgebraicMultigrid(a, cfg_amg)
+    # integer structure of the hierarchy equals the oracle's
+    assert amg.statistics.level_sizes == [amg_o.level(l).shape[0] for l in range(amg_o.num_levels)]
+    assert amg.statistics.level_nnz == [amg_o.level(l).arrays()[1].size for l in range(amg_o.num_levels)]
+    x = np.zeros(n)
+    rep = ca.cg_preconditioned(a, b, amg, x, ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500)
+                               .with_precision(precision))
+    assert rep.termination == ca.Termination.Converged, rep
+    assert rel_res(orc, x, b) <= 1e-10
+    # Poisson with two decades of coefficients on 64^3: kappa(A) ~ 1e5, so a
+    # 1e-10 residual bounds the relative error by ~1e-5; measured ~1e-9
+    assert rel_err(x, xo) <= 1e-6
+    assert np.max(np.abs(x - xo)) <= 1e-6 * np.max(np.abs(xo))
+
+
+def test_solve_transpose_device_nonsymmetric():
+    import torch
+
+    # upwind momentum matrix at 48^2: nonsymmetric, so the library solves
+    # with its cached host-built transpose and a Jacobi preconditioner
+    nn = 48
+    rng = np.random.default_rng(5)
+    u, w = rng.uniform(-1, 1, nn * nn), rng.uniform(-1, 1, nn * nn)
+    orc = O.assemble_momentum_u(nn, nn, 1 / (nn - 1), 1 / (nn - 1), 1e-2, 1e-3, 1.0, u, w)
+    rp, ci, v = orc.arrays()
+    n = rp.size - 1
+    d = orc.dense()
+    assert not np.array_equal(d, d.T)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    g = P.uniform(77, n)
+    lam_ref = np.linalg.solve(d.T, g)
+    dev = torch.device("cuda:0")
+    for kind in (N.KIND_BICGSTAB, N.KIND_GMRES):
+        lam = torch.zeros(n, dtype=torch.float64, device=dev)
+        rep = ca.solve_transpose(a, torch.from_numpy(g).to(dev), None, lam,
+                                 ca.IterativeSolverConfig.new(1e-12).with_max_iterations(5000), kind)
+        assert rep.termination == ca.Termination.Converged, (kind, rep)
+        lh = lam.cpu().numpy()
+        assert np.linalg.norm(g - d.T @ lh) / np.linalg.norm(g) <= 1e-12
+        assert rel_err(lh, lam_ref) <= 1e-10
+    # the transpose is rebuilt after the values change
+    a.update_values(2.0 * v)
+    lam = torch.zeros(n, dtype=torch.float64, device=dev)
+    ca.solve_transpose(a, torch.from_numpy(g).to(dev), None, lam,
+                       ca.IterativeSolverConfig.new(1e-12).with_max_iterations(5000), N.KIND_BICGSTAB)
+    assert rel_err(lam.cpu().numpy(), 0.5 * lam_ref) <= 1e-10
+
+
+def test_solve_transpose_device_rejects_mismatched_preconditioner():
+    import torch
+
+    a_small = ca.CsrMatrix.from_parts(*[np.asarray(t) for t in (np.ones(3), [0, 1, 2], [0, 1, 2, 3])], 3, 3)
+    big = ca.CsrMatrix.from_parts(np.ones(5), np.arange(5), np.arange(6), 5, 5)
+    jac = ca.JacobiPreconditioner(big)
+    g = torch.ones(3, dtype=torch.float64, device="cuda:0")
+    lam = torch.zeros(3, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(ca.SolveError) as e:
+        ca.solve_transpose(a_small, g, jac, lam, ca.IterativeSolverConfig.new(1e-10), N.KIND_CG)
+    assert e.value.status == N.ERR_DIMENSION
+    assert torch.count_nonzero(lam) == 0

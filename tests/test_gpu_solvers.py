@@ -179,7 +179,11 @@ def test_spmv_dia_nonsymmetric_integer_bit_exact():
     orc = _momentum()
     rp, ci, v = orc.arrays()
     n = rp.size - 1
-    v = np.round(v * 64.0)  # integer-valued, same (nonsymmetric) pattern
+    # integer-valued, same (nonsymmetric) pattern, no stored zero (a stored
+    # zero keeps a matrix out of diagonal storage)
+    w = np.round(v * 64.0)
+    v = np.where(w == 0.0, np.sign(v), w)
+    assert np.all(v != 0.0)
     x = np.round(P.uniform(5, n, -30.0, 30.0))
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
     assert a.storage()["format"] == "dia"
@@ -217,6 +221,29 @@ def test_spmv_dia_partial_diagonals_and_absent_entries():
     yo = orc.spmv(x)
     assert np.isfinite(y[nn]) and y[nn] == yo[nn]
     assert np.array_equal(np.isfinite(y), np.isfinite(yo))
+
+
+@pytest.mark.parametrize("fmt", ["auto", "csr_rowblock", "sell64"])
+def test_spmv_explicit_zero_with_non_finite_x(fmt):
+    # a stored zero times inf is NaN in CSR (and in the reference's CSR
+    # product); a stencil matrix that stores one is kept out of diagonal
+    # storage, so every device format gives the CSR result
+    nn = 64
+    n = nn * nn
+    rp, ci, v, _ = P.varcoef(nn, nn, 1, 1.0, np.ones(n), 0b0001, np.zeros(6))
+    v = v.copy()
+    v[rp[nn + 5] + 1] = 0.0  # an explicit zero in row nn + 5
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    st = a.set_storage(fmt)
+    assert st["format"] != "dia"
+    x = np.round(P.uniform(7, n, -30.0, 30.0))
+    x[ci[rp[nn + 5] + 1]] = np.inf
+    orc = O.Csr.from_arrays(rp, ci, v, n)
+    y, yo = a.spmv(x), orc.spmv(x)
+    assert np.isnan(yo[nn + 5]) and np.isnan(y[nn + 5])
+    assert np.array_equal(np.isnan(y), np.isnan(yo))
+    fin = np.isfinite(yo)
+    assert np.array_equal(y[fin], yo[fin])
 
 
 def test_spmv_fp32_device_tensor():

@@ -170,12 +170,22 @@ def test_amg_recompute_keeps_transfer_operators():
         assert np.array_equal(a[2], o[2])
 
 
-def test_amg_rejects_sequential_smoothers():
+def test_amg_default_config_builds_and_bad_smoothers_are_rejected():
     rp, ci, v, _ = P.config1(8)
     m = ca.CsrMatrix.from_parts(v, ci, rp, 64, 64)
+    amg = ca.AlgebraicMultigrid(m, ca.AMGConfig())  # the reference default: RS, classical, Gauss-Seidel
+    assert amg.statistics.num_levels >= 1
+    cfg = ca.AMGConfig()
+    cfg.relaxation_factor = 0.0
     with pytest.raises(N.CfdlaError) as e:
-        ca.AlgebraicMultigrid(m, ca.AMGConfig())  # default Gauss-Seidel
+        ca.AlgebraicMultigrid(m, cfg)
     assert e.value.status == N.ERR_CONFIG
+    native = ca.AMGConfig().native()
+    native.smoother = 7
+    import ctypes as C
+
+    h = C.c_void_p()
+    assert N.lib().cfdla_precond_create_amg(m.handle, C.byref(native), C.byref(h)) == N.ERR_CONFIG
 
 
 def test_sparse_matrix_builder_sums_duplicates_in_order():

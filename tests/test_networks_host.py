@@ -116,3 +116,82 @@ def test_three_node_known_answer(known):
             d[i, ci[q]] += v[q]
     assert d[1, 1] == k["diag_b"] and d[0, 0] == 1.0 and d[2, 2] == 1.0 and d[1, 0] == 0.0
     assert rhs[1] == k["rhs_with_neumann_b"]
+
+
+def _network(nets, s):
+    """Local arrays of network s with the channel-batch boundary conditions:
+    node 0 at the inlet pressure, the last node grounded and the outlet."""
+    n = int(nets.node_offset[s + 1] - nets.node_offset[s])
+    e0, e1 = nets.edge_offset[s], nets.edge_offset[s + 1]
+    is_dir = np.zeros(n, np.uint8)
+    dval = np.zeros(n)
+    is_dir[[0, n - 1]] = 1
+    dval[0] = NW.INLET_PRESSURE
+    outlet = np.zeros(n)
+    outlet[n - 1] = 1.0
+    return n, nets.edge_i[e0:e1], nets.edge_j[e0:e1], nets.width[e0:e1].copy(), nets.length[e0:e1], is_dir, dval, outlet
+
+
+def test_oracle_conductance_complex_step_matches_resistance_and_differences():
+    rng = np.random.default_rng(9)
+    w = rng.uniform(51e-6, 500e-6, 64)
+    L = rng.uniform(1e-3, 10e-3, 64)
+    g, dg = O.rect_conductance_complex_step(w, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY)
+    r = np.array([O.rect_resistance(a, NW.CHANNEL_HEIGHT, b, NW.WATER_VISCOSITY) for a, b in zip(w, L)])
+    assert np.max(np.abs(g * r - 1.0)) <= 4e-16 * 4
+    h = 1e-6 * w
+    gp, _ = O.rect_conductance_complex_step(w + h, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY)
+    gm, _ = O.rect_conductance_complex_step(w - h, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY)
+    assert np.max(np.abs((gp - gm) / (2 * h) / dg - 1.0)) <= 1e-7
+    # the host library's closed-form width derivative agrees with the complex step
+    g_lib, dg_lib = NW.rect_conductance(w, NW.CHANNEL_HEIGHT, L)
+    assert np.max(np.abs(g_lib / g - 1.0)) <= 1e-14
+    assert np.max(np.abs(dg_lib / dg - 1.0)) <= 1e-12
+
+
+def test_oracle_network_gradient_matches_central_differences():
+    # pins the oracle's adjoint network gradient (assembly, solve, adjoint,
+    # edge formula and complex-step dG/dw) by central differences of its own
+    # outlet flux along random relative width perturbations
+    nets = NW.synth_networks(count=3, seed_base=4200, nmin=60, nmax=160)
+    rng = np.random.default_rng(42)
+    for s in range(nets.count):
+        n, ei, ej, w, L, is_dir, dval, outlet = _network(nets, s)
+        J, grad, p = O.network_outlet_gradient(n, ei, ej, w, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY, is_dir,
+                                               dval, outlet)
+        # J is the total flow into the outlet: with one inlet and one outlet
+        # it equals the flow out of the inlet (mass conservation)
+        at_in = (ei == 0) | (ej == 0)
+        g, _ = O.rect_conductance_complex_step(w, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY)
+        other = np.where(ei == 0, ej, ei)
+        q_in = np.sum(g[at_in] * (p[0] - p[other[at_in]]))
+        assert abs(J - q_in) <= 1e-9 * abs(J)
+        assert J > 0
+        t = 1e-6
+        for _ in range(6):
+            d = rng.normal(size=w.size) * w
+            jp = O.network_outlet_gradient(n, ei, ej, w + t * d, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY, is_dir,
+                                           dval, outlet)[0]
+            jm = O.network_outlet_gradient(n, ei, ej, w - t * d, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY, is_dir,
+                                           dval, outlet)[0]
+            fd = (jp - jm) / (2 * t)
+            assert abs(fd - grad @ d) <= 1e-6 * abs(fd) + 1e-12 * np.abs(grad * d).sum(), (s, fd, grad @ d)
+
+
+def test_oracle_network_gradient_plausible_mistakes_fail():
+    # the same check catches a flipped adjoint sign and a dropped explicit term
+    nets = NW.synth_networks(count=1, seed_base=4210, nmin=60, nmax=90)
+    n, ei, ej, w, L, is_dir, dval, outlet = _network(nets, 0)
+    args = (NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY, is_dir, dval, outlet)
+    J, grad, p = O.network_outlet_gradient(n, ei, ej, w, *args)
+    g, dg = O.rect_conductance_complex_step(w, NW.CHANNEL_HEIGHT, L, NW.WATER_VISCOSITY)
+    m = outlet
+    explicit = m[ej] * (p[ei] - p[ej]) + m[ei] * (p[ej] - p[ei])
+    d = np.random.default_rng(3).normal(size=w.size) * w
+    t = 1e-6
+    fd = (O.network_outlet_gradient(n, ei, ej, w + t * d, *args)[0] -
+          O.network_outlet_gradient(n, ei, ej, w - t * d, *args)[0]) / (2 * t)
+    no_explicit = grad - explicit * dg
+    assert abs(no_explicit @ d - fd) > 1e-3 * abs(fd)
+    flipped = 2 * explicit * dg - grad  # explicit + (lambda_i - lambda_j)(p_i - p_j), times dG/dw
+    assert abs(flipped @ d - fd) > 1e-3 * abs(fd)
