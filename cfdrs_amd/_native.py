@@ -18,7 +18,7 @@ OK, ERR_INVALID_ARG, ERR_DIMENSION, ERR_CONFIG, ERR_NUMERICAL, ERR_DEVICE, ERR_O
 TERM_CONVERGED, TERM_INITIAL_RESIDUAL, TERM_MAX_ITERATIONS, TERM_BREAKDOWN, TERM_NONPOSITIVE_CURVATURE = range(5)
 # solver kinds / precision / dtype
 KIND_CG, KIND_BICGSTAB, KIND_GMRES = range(3)
-FP64, FP32_REFINED = range(2)
+FP64, FP32_REFINED, PRECISION_AUTO = range(3)
 DT_F64, DT_F32 = range(2)
 MAX_LEVELS = 32
 
@@ -46,7 +46,7 @@ class SolverConfig(C.Structure):
 class Report(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("outer_iterations", C.c_int64),
                 ("initial_residual_norm", C.c_double), ("final_residual_norm", C.c_double),
-                ("threshold", C.c_double), ("termination", C.c_int32), ("reserved", C.c_int32)]
+                ("threshold", C.c_double), ("termination", C.c_int32), ("precision", C.c_int32)]
 
 
 class AmgConfig(C.Structure):

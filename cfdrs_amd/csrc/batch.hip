@@ -392,7 +392,7 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
     rep.rn = sqrt(rr);
     rep.thr = thr;
     rep.term = term;
-    rep.pad = 0;
+    rep.pad = A.refined;  // the report's precision: 0 FP64, 1 FP32-refined
     A.rep[sys] = rep;
   }
 }
@@ -526,7 +526,9 @@ void *g_scratch[kScratchDevices] = {};
 
 void check_batch_config(const cfdla_solver_config &cfg) {
   CFDLA_REQUIRE(cfg.kind == CFDLA_CG, CFDLA_ERR_CONFIG, "batched networks are solved with CG (SPD Laplacians)");
-  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
+  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED ||
+                    cfg.precision == CFDLA_PRECISION_AUTO,
+                CFDLA_ERR_CONFIG,
                 "unknown precision");
   CFDLA_REQUIRE(cfg.max_iter >= 1 && cfg.max_iter < INT32_MAX, CFDLA_ERR_CONFIG, "max_iter out of range");
 }
@@ -566,6 +568,8 @@ void BatchSolver::launch(double *x, const cfdla_solver_config &cfg, const double
   a.inner_tol = cfg.inner_tol > 0.0 ? cfg.inner_tol : 1e-5;
   a.max_it = (int)cfg.max_iter;
   a.max_outer = cfg.max_outer > 0 ? cfg.max_outer : 30;
+  // automatic precision: FP64 (the team solves are L2- and latency-bound, and
+  // their FP32 recurrences need more iterations)
   a.refined = cfg.precision == CFDLA_FP32_REFINED ? 1 : 0;
   a.reliable = cfg.reliable_interval >= 0 ? 1 : 0;  // a replacement interval does not apply per team: delta only
   a.delta = cfg.reliable_delta > 0.0 ? cfg.reliable_delta : 1e-1;

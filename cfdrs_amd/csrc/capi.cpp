@@ -152,6 +152,7 @@ cfdla_report to_report(const SolveOut &o) {
   r.final_residual_norm = o.rn;
   r.threshold = o.thr;
   r.termination = o.term;
+  r.precision = o.precision;
   return r;
 }
 

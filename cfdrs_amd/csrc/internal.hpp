@@ -79,6 +79,12 @@ struct DevBuf {
     ensure(count);
     if (count) CFDLA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
   }
+  // upload plus `pad` zero elements after the data
+  void upload_padded(const T *h, size_t count, size_t pad) {
+    ensure(count + pad);
+    if (count) CFDLA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+    if (pad) CFDLA_HIP(hipMemsetAsync(p + count, 0, pad * sizeof(T), stream()));
+  }
   void download(T *h, size_t count) const {
     if (count) CFDLA_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
   }
@@ -115,6 +121,10 @@ struct HostCsr {
 // blocks of 4096 (the same LDS bytes) ran the 128^3 refined solve 4 % slower
 // (5.10 against 4.88 ms), as their rows were summed by fewer threads.
 constexpr int16_t kSellPad = INT16_MIN;
+// Zero entries stored past nnz in the row-block streams (columns, 16-bit
+// columns, values): the product loads them in aligned quads, and a quad that
+// starts before nnz ends at most 3 entries past it.
+constexpr size_t kRbPad = 4;
 template <typename T>
 constexpr int rb_cap() {
   return 2048;

@@ -198,10 +198,49 @@ int xcd_grid(int units, int units_per_block, int cap) {
 namespace {
 
 // ---- row-block CSR product ----------------------------------------------------
+// Four consecutive entries of the matrix streams (columns, values), loaded as
+// single 8- or 16-byte loads, nontemporal (NT) for operators too large to
+// stay in the caches between products: those streams are read once per
+// product, so they need not displace the gathered vector.
+// The arrays carry kRbPad zero entries past nnz, so a quad that starts
+// inside the array never reads past its allocation.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename V>
+__device__ __forceinline__ V load_vec(const V *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void load_quad(const double *p, double (&v)[4]) {
+  const v2d a = load_vec<NT>(reinterpret_cast<const v2d *>(p));
+  const v2d b = load_vec<NT>(reinterpret_cast<const v2d *>(p + 2));
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+template <bool NT>
+__device__ __forceinline__ void load_quad(const float *p, float (&v)[4]) {
+  const v4f a = load_vec<NT>(reinterpret_cast<const v4f *>(p));
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <bool NT>
+__device__ __forceinline__ void load_quad_cols(const int *p, int (&c)[4]) {
+  const v4i a = load_vec<NT>(reinterpret_cast<const v4i *>(p));
+  c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+}
+template <bool NT>
+__device__ __forceinline__ void load_quad_cols(const uint16_t *p, int base, int (&c)[4]) {
+  const v4u16 a = load_vec<NT>(reinterpret_cast<const v4u16 *>(p));
+  c[0] = base + (int)a.x; c[1] = base + (int)a.y; c[2] = base + (int)a.z; c[3] = base + (int)a.w;
+}
+
 // The host packs rows in order into blocks of at most rb_rows rows and rb_cap
-// nonzeros.  A workgroup streams its block's nonzeros with coalesced,
-// independent loads (up to rb_cap / kBlock = 8 column/value pairs per thread,
-// all issued before the x gathers), parks the products in LDS, then sums each
+// nonzeros.  A workgroup streams its block's nonzeros in quads from the
+// 4-aligned entry at or below its first one (up to rb_cap / (4 kBlock) = 2
+// quads per thread, the at most 3 entries past them by three lanes; all
+// issued before the x gathers), parks the products in LDS, then sums each
 // row from LDS with 1..64 threads (the largest power of two that fits the
 // block's row count) and a shuffle tree.  Row lengths of 4..700 nonzeros (AMG
 // transfer operators and Galerkin products) all keep every lane busy with
@@ -218,7 +257,7 @@ namespace {
 // sweeps (377 against 437 us).
 //
 // C16: columns are 16-bit offsets from the block's base column (cb[q]).
-template <typename T, int EPI, bool DOT, bool C16>
+template <typename T, int EPI, bool DOT, bool C16, bool NT>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const uint16_t *__restrict__ c16,
                                                    const int *__restrict__ cb, const T *__restrict__ av,
@@ -227,9 +266,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
                                                    Scal *s, int check_done, double *partials, T *y2) {
   if (check_done && s->done) return;
   constexpr int CAP = rb_cap<T>();
-  constexpr int U = CAP / kBlock;
+  constexpr int Q = CAP / (4 * kBlock);  // quads per thread
+  static_assert(Q * 4 * kBlock == CAP, "the row-block cap is a whole number of quads per thread");
   constexpr int RPT = rb_rows<T>() / kBlock;  // rows per thread when a block has more rows than threads
-  __shared__ T prod[CAP];
+  __shared__ T prod[CAP + 4];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -279,9 +319,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
     } else {
       // Reduction layout: with nr <= kBlock rows, tpr threads (a power of two
       // up to 64) per row; otherwise up to RPT rows per thread.  Row bounds and
-      // epilogue operands are loaded first and all loads are branch-free
-      // (clamped indices; surplus lanes re-read the last entry or row), so
-      // they, the 2U entry loads and then the U gathers are in flight together.
+      // epilogue operands are loaded first, then every quad of the block and
+      // the tail entries, then all gathers, so they are in flight together.
+      // Products sit in LDS at their offset from ea, the quad-aligned start.
+      const int ea = e0 & ~3, e1 = e0 + ne;
       const bool wide = nr <= kBlock;
       int tpr = 1;
       if (wide)
@@ -293,27 +334,58 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
       for (int j = 0; j < RPT; ++j) {
         const int rl = wide ? min(tid / tpr, nr - 1) : min(tid + j * kBlock, nr - 1);
         if (j == 0 || !wide) {
-          ks[j] = rp[r0 + rl] - e0;
-          kr[j] = rp[r0 + rl + 1] - e0;
+          ks[j] = rp[r0 + rl] - ea;
+          kr[j] = rp[r0 + rl + 1] - ea;
           e[j] = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, omega, y, dotw);
         }
       }
       if (ne > 0) {
-        int c[U];
-        T v[U], xv[U];
+        // a column that is valid for any gather: entries outside [e0, e1)
+        // (neighbouring blocks, the zero padding) gather it and are dropped
+        const int safe = C16 ? b0 : 0;
+        int c[Q][4];
+        T v[Q][4], xv[Q][4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int k = min(tid + u * kBlock, ne - 1);
-          c[u] = col(e0 + k);
-          v[u] = av[e0 + k];
+        for (int u = 0; u < Q; ++u) {
+          const int k = ea + 4 * (tid + u * kBlock);
+          if (k < e1) {
+            if constexpr (C16) load_quad_cols<NT>(c16 + k, b0, c[u]);
+            else load_quad_cols<NT>(ci + k, c[u]);
+            load_quad<NT>(av + k, v[u]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              c[u][j] = safe;
+              v[u][j] = T(0);
+            }
+          }
         }
+        const int kt = ea + 4 * Q * kBlock + tid;  // entries past the quads (e1 - ea <= CAP + 3)
+        const bool tail = tid < 3 && kt < e1;
+        const int ct = tail ? col(kt) : safe;
+        const T vt = tail ? av[kt] : T(0);
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = gather<T, EPI>(c[u], x, b, dinv, omega);
+        for (int u = 0; u < Q; ++u) {
+          const int k = ea + 4 * (tid + u * kBlock);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int k = tid + u * kBlock;
-          if (k < ne) prod[k] = v[u] * xv[u];
+          for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j >= e0 && k + j < e1;
+            xv[u][j] = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
+          }
         }
+        const T xt = gather<T, EPI>(ct, x, b, dinv, omega);
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+          const int k = ea + 4 * (tid + u * kBlock);
+          if (k < e1) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const bool ok = k + j >= e0 && k + j < e1;
+              prod[k + j - ea] = ok ? v[u][j] * xv[u][j] : T(0);
+            }
+          }
+        }
+        if (tail) prod[kt - ea] = vt * xt;
       }
       __syncthreads();
       if (wide) {
@@ -346,20 +418,39 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   }
 }
 
+// Which products load their streams nontemporally, measured on the 128^3
+// solves (FP64 / FP32-refined ms, two runs each): never 6.54 / 4.54; always
+// 6.67 / 4.33; FP64 operators of at least 32 MiB 6.53 / 4.42, of at least
+// 64 MiB 6.54 / 4.55.  So FP32 products always stream nontemporally and FP64
+// ones from 32 MiB up: the small FP64 operators (level-1 transfers, level 2)
+// gain from keeping their lines for the next sweep, the FP32 ones do not.
+#ifndef CFDLA_RB_NT_MIN_BYTES
+#define CFDLA_RB_NT_MIN_BYTES (32.0 * 1024.0 * 1024.0)
+#endif
+
+template <typename T, int EPI, bool DOT, bool C16, bool NT>
+void launch_csr_rb_k(const DevCsr &A, int blocks, const T *x, const T *b, const T *dinv, double omega, T *y,
+                     const T *dotw, Scal *s, bool check_done, RedWs &ws, T *y2) {
+  const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
+                     A.rp.p, C16 ? nullptr : A.ci.p, C16 ? A.cols16<T>() : nullptr, C16 ? A.cbase<T>() : nullptr,
+                     A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
+}
+
 template <typename T, int EPI, bool DOT>
 void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                    Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr) {
   const int nrb = A.nrb<T>();
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
-  const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
-  if (A.has_cols16<T>())
-    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, true>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb, A.rp.p,
-                       nullptr, A.cols16<T>(), A.cbase<T>(), A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
-                       check_done ? 1 : 0, ws.partials.p, y2);
-  else
-    hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, false>), dim3(blocks), dim3(kBlock), 0, stream(), nrb, rb, A.rp.p,
-                       A.ci.p, nullptr, nullptr, A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
-                       ws.partials.p, y2);
+  const bool c16 = A.has_cols16<T>();
+  const bool nt = sizeof(T) == 4 || (double)A.nnz * (double)(sizeof(T) + (c16 ? 2 : 4)) >= CFDLA_RB_NT_MIN_BYTES;
+  if (c16) {
+    if (nt) launch_csr_rb_k<T, EPI, DOT, true, true>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+    else launch_csr_rb_k<T, EPI, DOT, true, false>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+  } else {
+    if (nt) launch_csr_rb_k<T, EPI, DOT, false, true>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+    else launch_csr_rb_k<T, EPI, DOT, false, false>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+  }
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 

@@ -76,7 +76,7 @@ struct BatchSchedule {
 struct BatchReport {
   int64_t iterations, outer;
   double rn0, rn, thr;
-  int32_t term, pad;
+  int32_t term, pad;  // pad: the precision the solve ran in (cfdla_report.precision)
 };
 
 struct BatchSolver {

@@ -323,7 +323,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   const bool allow16 = fmt_request != CFDLA_STORAGE_CSR_INT32;
   c16_f64 = allow16 && build_cols16(h, blocks, c16v, cb);
   if (c16_f64) {
-    c16_64.upload(c16v.data(), c16v.size());
+    c16_64.upload_padded(c16v.data(), c16v.size(), kRbPad);
     cbase64.upload(cb.data(), cb.size());
     sync_stream();
   } else {
@@ -337,7 +337,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     rb32.upload(blocks.data(), blocks.size());
     c16_f32 = allow16 && build_cols16(h, blocks, c16v, cb);
     if (c16_f32) {
-      c16_32.upload(c16v.data(), c16v.size());
+      c16_32.upload_padded(c16v.data(), c16v.size(), kRbPad);
       cbase32.upload(cb.data(), cb.size());
       sync_stream();
     }
@@ -346,13 +346,13 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
     c16_32.release();
     cbase32.release();
   }
-  ci.upload(h.ci.data(), h.ci.size());
-  v64.upload(h.v.data(), h.v.size());
+  ci.upload_padded(h.ci.data(), h.ci.size(), kRbPad);
+  v64.upload_padded(h.v.data(), h.v.size(), kRbPad);
   std::vector<float> f;
   if (want_fp32) {
     f.resize(h.v.size());
     for (size_t k = 0; k < f.size(); ++k) f[k] = (float)h.v[k];
-    v32.upload(f.data(), f.size());
+    v32.upload_padded(f.data(), f.size(), kRbPad);
   }
   std::vector<double> d;
   std::vector<float> df;
@@ -1458,8 +1458,9 @@ SolveOut run_refined(const DevCsr &A, Precond *M, const double *b, double *x, do
 SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, const cfdla_solver_config &cfg) {
   CFDLA_REQUIRE(A.n == A.m, CFDLA_ERR_DIMENSION, "solve requires a square matrix");
   CFDLA_REQUIRE(cfg.kind >= 0 && cfg.kind <= 2, CFDLA_ERR_CONFIG, "unknown solver kind");
-  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED, CFDLA_ERR_CONFIG,
-                "unknown precision");
+  CFDLA_REQUIRE(cfg.precision == CFDLA_FP64 || cfg.precision == CFDLA_FP32_REFINED ||
+                    cfg.precision == CFDLA_PRECISION_AUTO,
+                CFDLA_ERR_CONFIG, "unknown precision");
   CFDLA_REQUIRE(cfg.max_iter >= 1, CFDLA_ERR_CONFIG, "max_iter must be >= 1");
   CFDLA_REQUIRE(cfg.reliable_interval >= -1, CFDLA_ERR_CONFIG, "reliable_interval must be >= -1");
   CFDLA_REQUIRE(std::isfinite(cfg.tol_abs) && std::isfinite(cfg.tol_rel) && cfg.tol_abs >= 0.0 && cfg.tol_rel >= 0.0,
@@ -1483,7 +1484,15 @@ SolveOut solve_device(const DevCsr &A, Precond *M, const double *b, double *x, c
   double bn = std::sqrt(c.read(s).red[0]);
   CFDLA_REQUIRE(std::isfinite(bn), CFDLA_ERR_NUMERICAL, "right-hand side is not finite");
   double thr = std::max(cfg.tol_abs, cfg.tol_rel * bn);
-  SolveOut o = cfg.precision == CFDLA_FP32_REFINED ? run_refined(A, M, b, x, thr, cfg) : run_fp64(A, M, b, x, thr, cfg);
+  int precision = cfg.precision;
+  if (precision == CFDLA_PRECISION_AUTO) {
+    // FP32 halves the value bytes; it pays once an iteration streams more
+    // than the 256 MiB Infinity Cache holds (matrix and preconditioner).
+    const double bytes = 12.0 * (double)A.nnz + 4.0 * (double)A.n + M->stream_bytes();
+    precision = bytes > 256.0 * 1024.0 * 1024.0 ? CFDLA_FP32_REFINED : CFDLA_FP64;
+  }
+  SolveOut o = precision == CFDLA_FP32_REFINED ? run_refined(A, M, b, x, thr, cfg) : run_fp64(A, M, b, x, thr, cfg);
+  o.precision = precision;
   sync_stream();
   return o;
 }

@@ -35,6 +35,9 @@ struct Precond {
   virtual const double *jacobi_dinv64() const { return nullptr; }
   virtual const float *jacobi_dinv32() const { return nullptr; }
   virtual void ensure_fp32() {}
+  // FP64 operator bytes one application streams (matrices as CSR: 12 bytes
+  // per nonzero plus row pointers), for the automatic precision choice.
+  virtual double stream_bytes() const { return 0.0; }
 };
 
 struct JacobiPrecond : Precond {
@@ -45,6 +48,7 @@ struct JacobiPrecond : Precond {
   int kind() const override { return 1; }
   void apply(const double *r, double *z, Scal *s, bool check_done) override;
   void apply(const float *r, float *z, Scal *s, bool check_done) override;
+  double stream_bytes() const override { return 8.0 * n; }
   const double *jacobi_dinv64() const override { return d64.p; }
   const float *jacobi_dinv32() const override { return d32.p; }
 };
@@ -133,6 +137,12 @@ struct AmgPrecond : Precond {
   }
   void update_values(const double *values);
   int kind() const override { return 2; }
+  double stream_bytes() const override {
+    double b = 0.0;
+    for (const HostLevel &h : host.levels)
+      b += 12.0 * (double)(h.A.nnz() + h.P.nnz() + h.R.nnz()) + 4.0 * (double)(h.A.n + h.P.n + h.R.n);
+    return b;
+  }
   void apply(const double *r, double *z, Scal *s, bool check_done) override {
     ensure_device();
     vcycle<double>(0, r, z, s, check_done, OP_NONE);
@@ -168,6 +178,7 @@ struct SolveOut {
   int64_t iters = 0, outer = 0;
   double rn0 = 0, rn = 0, thr = 0;
   int term = CFDLA_TERM_MAX_ITERATIONS;
+  int precision = CFDLA_FP64;  // the arithmetic the solve ran in
 };
 
 // GMRES(m) drivers (gmres.hip).

@@ -37,6 +37,10 @@ class Termination(enum.IntEnum):
 class Precision(enum.IntEnum):
     FP64 = N.FP64
     FP32_REFINED = N.FP32_REFINED
+    # FP32-refined when an iteration streams more than 256 MiB of FP64
+    # operator bytes (matrix and preconditioner), FP64 otherwise; batched
+    # network solves take FP64.  SolveReport.precision says which ran.
+    AUTO = N.PRECISION_AUTO
 
 
 @dataclass
@@ -49,6 +53,7 @@ class SolveReport:
     threshold: float
     termination: Termination
     outer_iterations: int = 0
+    precision: Precision = Precision.FP64  # the arithmetic the solve ran in
 
 
 class SolveError(RuntimeError):
@@ -166,7 +171,7 @@ def _solve(matrix: CsrMatrix, rhs, preconditioner, solution, config: IterativeSo
     except N.CfdlaError as e:
         raise SolveError(e) from None
     return SolveReport(rep.iterations, rep.initial_residual_norm, rep.final_residual_norm, rep.threshold,
-                       Termination(rep.termination), rep.outer_iterations)
+                       Termination(rep.termination), rep.outer_iterations, Precision(rep.precision))
 
 
 def cg_preconditioned(matrix, right_hand_side, preconditioner, solution, config) -> SolveReport:

@@ -21,7 +21,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .linear_solver import IterativeSolverConfig, SolveReport, Termination
+from .linear_solver import IterativeSolverConfig, Precision, SolveReport, Termination
 from .sparse import _is_torch, bind_torch_stream
 
 WATER_VISCOSITY = 1.002e-3  # Pa s
@@ -96,7 +96,7 @@ def _require_cuda_f64(t, n: int, name: str) -> None:
 
 def _report(r: N.Report) -> SolveReport:
     return SolveReport(r.iterations, r.initial_residual_norm, r.final_residual_norm, r.threshold,
-                       Termination(r.termination), r.outer_iterations)
+                       Termination(r.termination), r.outer_iterations, Precision(r.precision))
 
 
 class NetworkBatch:

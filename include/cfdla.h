@@ -59,8 +59,15 @@ typedef enum { CFDLA_CG = 0, CFDLA_BICGSTAB = 1, CFDLA_GMRES = 2 } cfdla_solver_
 
 /* Arithmetic of the solve.  FP32_REFINED: FP32 SpMV, smoother and inner
  * Krylov iterations inside FP64 defect correction (outer residual, update
- * and coarsest AMG solve in FP64). */
-typedef enum { CFDLA_FP64 = 0, CFDLA_FP32_REFINED = 1 } cfdla_precision;
+ * and coarsest AMG solve in FP64).  PRECISION_AUTO picks one per solve: the
+ * refined form when an iteration streams more than 256 MiB of FP64
+ * operator bytes (the matrix plus the preconditioner's matrices: beyond the
+ * Infinity Cache, where halving the value bytes pays), FP64 otherwise
+ * (cache-resident systems, whose kernels are latency-bound, gain nothing
+ * from FP32 and pay for the residual replacements); batched network solves
+ * take FP64.  The report's precision field says which ran.  No counterpart
+ * in the reference. */
+typedef enum { CFDLA_FP64 = 0, CFDLA_FP32_REFINED = 1, CFDLA_PRECISION_AUTO = 2 } cfdla_precision;
 
 /* Value type of device vectors passed to *_device entry points. */
 typedef enum { CFDLA_F64 = 0, CFDLA_F32 = 1 } cfdla_dtype;
@@ -103,7 +110,7 @@ typedef struct {
   double final_residual_norm;
   double threshold;
   int32_t termination;  /* cfdla_termination */
-  int32_t reserved;
+  int32_t precision;    /* cfdla_precision the solve ran in (FP64 or FP32_REFINED) */
 } cfdla_report;
 
 /* AMGConfig (crates/cfd-math/src/linear_solver/preconditioners/multigrid/mod.rs:219-260).

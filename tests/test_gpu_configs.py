@@ -182,3 +182,36 @@ def test_solve_transpose_device_rejects_mismatched_preconditioner():
         ca.solve_transpose(a_small, g, jac, lam, ca.IterativeSolverConfig.new(1e-10), N.KIND_CG)
     assert e.value.status == N.ERR_DIMENSION
     assert torch.count_nonzero(lam) == 0
+
+
+def test_automatic_precision_follows_the_streamed_bytes():
+    # cache-resident systems run in FP64; one whose iteration streams more
+    # than 256 MiB of FP64 operator bytes runs FP32-refined
+    rp, ci, v, b = P.config1(64)
+    n = b.size
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    x = np.zeros(n)
+    rep = ca.cg_preconditioned(a, b, ca.JacobiPreconditioner(a), x,
+                               ca.IterativeSolverConfig.new(1e-10).with_max_iterations(5000)
+                               .with_precision(ca.Precision.AUTO))
+    assert rep.termination == ca.Termination.Converged and rep.precision == ca.Precision.FP64
+    # 7-point stencil on 160^3 cells with a dominant diagonal (10 against six
+    # -1 couplings): 28.5M nonzeros, 342 MB of FP64 CSR
+    import torch
+
+    nn = 160
+    k = np.ones(nn ** 3)
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, 1.0, k, 0, np.zeros(6))
+    diag = np.repeat(np.arange(nn ** 3), np.diff(rp)) == ci
+    v = v.copy()
+    v[diag] = 10.0
+    n = nn ** 3
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    bd = torch.from_numpy(P.uniform(77, n)).cuda()
+    xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(500).with_precision(ca.Precision.AUTO)
+    rep = ca.cg_preconditioned(a, bd, ca.JacobiPreconditioner(a), xd, cfg)
+    assert rep.termination == ca.Termination.Converged and rep.precision == ca.Precision.FP32_REFINED
+    r = torch.empty_like(xd)
+    a.spmv_into(xd, r)
+    assert float(torch.linalg.norm(bd - r) / torch.linalg.norm(bd)) <= 1e-10

@@ -223,6 +223,41 @@ def test_spmv_dia_partial_diagonals_and_absent_entries():
     assert np.array_equal(np.isfinite(y), np.isfinite(yo))
 
 
+@pytest.mark.parametrize("fmt", ["csr_rowblock", "csr_int32"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_spmv_row_blocks_at_the_cap_bit_exact(fmt, dtype):
+    # Row blocks hold at most 2048 nonzeros and are read in 4-aligned quads
+    # from the aligned entry at or below their first one; the <= 3 entries
+    # past the quads go to three lanes.  Rows of 2045..2048 entries after
+    # short rows of 1..3 entries start blocks at every alignment and fill
+    # them to the cap; 2049 and 5000 take the long-row path.  Integer data:
+    # every partial sum is exact, so FP64 and FP32 must match bit for bit.
+    import torch
+
+    lens = []
+    for pre in (1, 2, 3, 0):
+        for big in (2045, 2046, 2047, 2048):
+            lens += [pre, big] if pre else [big]
+    lens += [2049, 7, 5000, 3, 1]
+    rng = np.random.default_rng(17)
+    m = 9000
+    rp, ci, v = [0], [], []
+    for L in lens:
+        cols = np.sort(rng.choice(m, size=L, replace=False))
+        ci.extend(cols.tolist())
+        v.extend(rng.integers(-3, 4, size=L).astype(float).tolist())
+        rp.append(len(ci))
+    rp, ci, v = np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
+    n = rp.size - 1
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
+    a.set_storage(fmt)
+    x = np.random.default_rng(2).integers(-40, 41, size=m).astype(float)
+    yo = O.Csr.from_arrays(rp, ci, v, n, m).spmv(x)
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    y = a.spmv(torch.tensor(x, dtype=dt, device="cuda")).double().cpu().numpy()
+    assert np.array_equal(y, yo)
+
+
 @pytest.mark.parametrize("fmt", ["auto", "csr_rowblock", "sell64"])
 def test_spmv_explicit_zero_with_non_finite_x(fmt):
     # a stored zero times inf is NaN in CSR (and in the reference's CSR
