@@ -166,6 +166,8 @@ def main() -> None:
 
     err = float(torch.linalg.norm(results["FP32_REFINED"]["x"] - results["FP64"]["x"]) /
                 torch.linalg.norm(results["FP64"]["x"]))
+    auto_picks = solve(ca.Precision.AUTO).precision.name  # the default policy's choice here
+    log(f"Precision.AUTO picks {auto_picks}")
     # solve-level bytes: the per-iteration model of the metric, B_spmv + B_prec
     # + 8 n s, times the iterations, over the measured time
     solve_bw = {}
@@ -245,6 +247,7 @@ def main() -> None:
         "refined_iterations": results["FP32_REFINED"]["iterations"],
         "refined_outer_steps": results["FP32_REFINED"]["outer"],
         "speedup_refined_vs_fp64": round(t64 / tref, 4),
+        "precision_auto_picks": auto_picks,
         "rel_residual_fp64": results["FP64"]["rel_residual"],
         "rel_residual_refined": results["FP32_REFINED"]["rel_residual"],
         "refined_vs_fp64_solution_rel_diff": err,
@@ -436,12 +439,22 @@ def secondary_configs(args) -> dict:
         return {"ms": round(ms, 3), "iterations": rep.iterations, "outer": rep.outer_iterations,
                 "termination": rep.termination.name, "solves_per_s": round(1e3 / ms, 1)}
 
+    def auto_pick(a, b_host, prec, kind, restart=30):
+        # the precision Precision.AUTO chooses for this system (one solve)
+        b = torch.from_numpy(b_host).to(dev)
+        x = torch.zeros_like(b)
+        cfg = ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(ca.Precision.AUTO)
+        fn = {"cg": ca.cg_preconditioned, "bicgstab": ca.bicgstab_preconditioned}.get(kind)
+        rep = fn(a, b, prec, x, cfg) if fn else ca.gmres_preconditioned(a, b, prec, x, cfg, restart)
+        return rep.precision.name
+
     # 2D pressure correction, 128^2, Jacobi-PCG
     rp, ci, v, b1 = P.config1(128)
     a1 = ca.CsrMatrix.from_parts(v, ci, rp, 128 * 128, 128 * 128)
     j1 = ca.JacobiPreconditioner(a1)
     out["pressure_correction_128x128_jacobi_pcg"] = {
         prec.name: krylov(a1, b1, j1, "cg", prec) for prec in (ca.Precision.FP64, ca.Precision.FP32_REFINED)}
+    out["pressure_correction_128x128_jacobi_pcg"]["AUTO_picks"] = auto_pick(a1, b1, j1, "cg")
     log(f"2D pressure correction: {out['pressure_correction_128x128_jacobi_pcg']}")
 
     # lid-driven cavity 512^2: momentum (nonsymmetric) and pinned pressure correction
@@ -458,6 +471,10 @@ def secondary_configs(args) -> dict:
         cav[f"momentum_gmres30_jacobi_{prec.name}"] = krylov(am, bm, jm, "gmres", prec)
         cav[f"momentum_gmres30_sor_{prec.name}"] = krylov(am, bm, sm, "gmres", prec)
         cav[f"pressure_amg_pcg_{prec.name}"] = krylov(ap_, bp, amg_p, "cg", prec)
+    cav["AUTO_picks"] = {"momentum_bicgstab_jacobi": auto_pick(am, bm, jm, "bicgstab"),
+                         "momentum_gmres30_jacobi": auto_pick(am, bm, jm, "gmres"),
+                         "momentum_gmres30_sor": auto_pick(am, bm, sm, "gmres"),
+                         "pressure_amg_pcg": auto_pick(ap_, bp, amg_p, "cg")}
     out["lid_driven_cavity_512x512_re1000"] = cav
     log(f"cavity: {cav}")
 

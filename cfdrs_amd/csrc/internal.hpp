@@ -105,10 +105,18 @@ struct HostCsr {
 //
 // Square matrices with short rows whose column offsets |col - row| fit in
 // int16 also get a sliced-ELL copy: slices of 64 consecutive rows (one
-// wavefront, one lane per row), entries stored column-major within a slice
-// so every load of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32)
-// value segment plus a 128 B offset segment, padded to the longest row of the
-// slice with kSellPad offsets.  (Sorting rows by length inside 1024-row
+// wavefront, one lane per row), padded to the longest row of the slice with
+// kSellPad offsets.  Within a slice a lane's slots are stored in groups of
+// sell_group consecutive entries, group-major: slot q of lane l sits at
+// slice start + (q / G) 64 G + l G + q % G.  With G = 4 every lane loads four
+// values (16 B FP32, 2 x 16 B FP64) and four offsets (8 B) per group, and a
+// wavefront load is one contiguous 1 KiB / 2 KiB value segment; G = 1
+// (plain column-major slices) serves matrices whose rows are so short that
+// rounding the slice width up to 4 would pad them past the limit.  On the
+// first AMG level of the 128^3 problem (37 entries per row on average) groups
+// of 4 took the FP64 Jacobi sweep from 29.2 to 28.1 us and left FP32 at
+// 19.0 us; splitting each slice's groups over 4 wavefronts (kernels.hip)
+// gave the rest.  (Sorting rows by length inside 1024-row
 // windows cut the padding of the first AMG level of the 128^3 problem from
 // 28 % to 2.7 % but made its sweeps slower, 29.2 -> 37.8 us FP64: the lanes
 // of a wavefront no longer hold neighbouring rows, so its x gathers and
@@ -168,9 +176,10 @@ struct DevCsr {
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;
   bool sell = false;
+  int sell_group = 1;  // slots per lane stored together: 1 or 4
   int nslices = 0;
   int64_t sell_entries = 0;
-  DevBuf<int32_t> sell_ptr;   // nslices + 1 entry offsets (multiples of 64)
+  DevBuf<int32_t> sell_ptr;   // nslices + 1 entry offsets (multiples of 64 sell_group)
   DevBuf<int16_t> sell_delta; // col - row, kSellPad for padding
   DevBuf<double> sell_v64;
   DevBuf<float> sell_v32;

@@ -455,12 +455,37 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
 }
 
 // ---- sliced-ELL product ------------------------------------------------------
-// One wavefront per 64-row slice, one lane per row, no cross-lane reduction.
-// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
-// so the slices are split into 8 contiguous ranges and XCD q walks range q:
-// the x gathers of neighbouring rows then hit the L2 of the XCD that owns
-// them.  The grid is a multiple of 8 blocks.
-template <typename T, int EPI, bool DOT, int CH>
+// One lane per row of a 64-row slice; KS wavefronts share a slice, each
+// summing a contiguous 1/KS of its slot groups, and the partial sums meet in
+// LDS.  A lane's slots come in groups of GS (1 or 4) consecutive entries, one
+// load of the values and one of the offsets per group; GPS groups per
+// unrolled step are all in flight before their x gathers.  (With one
+// wavefront per slice the first AMG level of the 128^3 problem, 47 slots per
+// row, ran a chain of 6 dependent load-then-gather steps per wavefront at
+// about 4 wavefronts per SIMD: latency-bound, 19 us per FP32 sweep for
+// 65 MB.)  Workgroups are dealt to the 8 XCDs round-robin (block b runs on
+// XCD b % 8), so the slices are split into 8 contiguous ranges and XCD q
+// walks range q: the x gathers of neighbouring rows then hit the L2 of the
+// XCD that owns them.  The grid is a multiple of 8 blocks.
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+
+// The streams are loaded with the default cache policy: the level matrices
+// are re-read by the next sweep of the same level, and nontemporal loads,
+// which keep them out of the caches, ran the 128^3 solves slower (FP64 7.00
+// against 6.51 ms, FP32 4.52 against 4.33 ms).
+template <int GS, typename T>
+__device__ __forceinline__ void sell_group_load(const short *d, const T *v, int (&dd)[GS], T (&vv)[GS]) {
+  if constexpr (GS == 4) {
+    const v4i16 a = *reinterpret_cast<const v4i16 *>(d);
+    dd[0] = a.x; dd[1] = a.y; dd[2] = a.z; dd[3] = a.w;
+    load_quad<false>(v, vv);
+  } else {
+    dd[0] = d[0];
+    vv[0] = v[0];
+  }
+}
+
+template <typename T, int EPI, bool DOT, int GS, int GPS, int KS>
 __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *__restrict__ sp,
                                                  const short *__restrict__ dl, const T *__restrict__ av,
                                                  const T *__restrict__ x, const T *__restrict__ b,
@@ -468,42 +493,65 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *
                                                  int check_done, double *partials) {
   if (check_done && s->done) return;
   constexpr int WPB = kBlock / 64;
+  constexpr int SPB = WPB / KS;     // slices per block and step
+  constexpr int GSTRIDE = 64 * GS;  // slots between a lane's consecutive groups
+  static_assert(WPB % KS == 0, "KS must divide the wavefronts of a block");
+  __shared__ T part[KS > 1 ? (KS - 1) * SPB * 64 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sw = wave / KS, kp = wave % KS;  // slice of the step, part of its slots
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
   const int chunk = (nslices + 7) >> 3;
   const int s0 = xcd * chunk;
   const int s1 = min(nslices, s0 + chunk);
   double dacc = 0.0;
-  for (int sl = s0 + local * WPB + wave; sl < s1; sl += per * WPB) {
+  // block-uniform trip count: every wavefront reaches the barriers
+  for (int sb = s0 + local * SPB; sb < s1; sb += per * SPB) {
+    const int sl = sb + sw;
     const int row = sl * 64 + lane;
-    if (row >= n) continue;
-    const int base = sp[sl];
-    const int w = (sp[sl + 1] - base) >> 6;
-    const short *dd = dl + base + lane;
-    const T *vv = av + base + lane;
-    const EpiIn<T> e = epi_load<T, EPI, DOT>(row, x, b, dinv, omega, y, dotw);
-    // Branch-free: indices clamped to the slice, padding gathers x[row] and
-    // is masked out of the sum, so all loads of a chunk are in flight at once.
+    const bool live = sl < s1 && row < n;
     T sum = T(0);
-    for (int k0 = 0; k0 < w; k0 += CH) {
-      int d[CH];
-      bool ok[CH];
-      T v[CH], xv[CH];
+    EpiIn<T> e{T(0), T(0), T(0), T(0), T(0)};
+    if (live) {
+      const int base = sp[sl];
+      const int ng = (sp[sl + 1] - base) / GSTRIDE;
+      const int ga = ng * kp / KS, gb = ng * (kp + 1) / KS;
+      const short *dd = dl + base + lane * GS;
+      const T *vv = av + base + lane * GS;
+      if (kp == 0) e = epi_load<T, EPI, DOT>(row, x, b, dinv, omega, y, dotw);
+      // Branch-free: group indices clamped to the part, padding gathers
+      // x[row] and is masked out of the sum, so all loads of a step are in
+      // flight at once.  Entries are summed in row order within a part.
+      for (int g0 = ga; g0 < gb; g0 += GPS) {
+        int d[GPS][GS];
+        T v[GPS][GS], xv[GPS][GS];
+        bool ok[GPS][GS];
 #pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        const int k = min(k0 + u, w - 1);
-        d[u] = dd[k * 64];
-        v[u] = vv[k * 64];
+        for (int u = 0; u < GPS; ++u) {
+          const int g = min(g0 + u, gb - 1);
+          sell_group_load<GS>(dd + g * GSTRIDE, vv + g * GSTRIDE, d[u], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < GPS; ++u)
+#pragma unroll
+          for (int j = 0; j < GS; ++j) {
+            ok[u][j] = g0 + u < gb && d[u][j] != kSellPad;
+            xv[u][j] = gather<T, EPI>(row + (ok[u][j] ? d[u][j] : 0), x, b, dinv, omega);
+          }
+#pragma unroll
+        for (int u = 0; u < GPS; ++u)
+#pragma unroll
+          for (int j = 0; j < GS; ++j) sum += ok[u][j] ? v[u][j] * xv[u][j] : T(0);
       }
-#pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        ok[u] = k0 + u < w && d[u] != kSellPad;
-        xv[u] = gather<T, EPI>(row + (ok[u] ? d[u] : 0), x, b, dinv, omega);
-      }
-#pragma unroll
-      for (int u = 0; u < CH; ++u) sum += ok[u] ? v[u] * xv[u] : T(0);
     }
-    epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc);
+    if constexpr (KS > 1) {
+      if (kp > 0) part[((kp - 1) * SPB + sw) * 64 + lane] = sum;
+      __syncthreads();
+      if (kp == 0)
+#pragma unroll
+        for (int q = 1; q < KS; ++q) sum += part[((q - 1) * SPB + sw) * 64 + lane];
+      __syncthreads();  // the next step rewrites part
+    }
+    if (kp == 0 && live) epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc);
   }
   if constexpr (DOT) {
     double v[1] = {dacc};
@@ -514,14 +562,29 @@ __global__ __launch_bounds__(kBlock) void k_sell(int n, int nslices, const int *
 template <typename T, int EPI, bool DOT>
 void launch_sell(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
                  Scal *s, bool check_done, int op, RedWs &ws) {
-  const int blocks = xcd_grid(A.nslices, kBlock / 64);
-  // 8 entries per unrolled step: on the first AMG level of the 128^3 problem
-  // (47 slots per row on average) 16 per step measured slower in both
-  // precisions (solve 6.69 against 6.60 ms FP64, 4.88 against 4.83 ms FP32)
-  hipLaunchKernelGGL((k_sell<T, EPI, DOT, 8>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices, A.sell_ptr.p,
-                     A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0,
-                     ws.partials.p);
-  if (DOT) finish(1, s, op, blocks, ws, check_done);
+  const int cd = check_done ? 1 : 0;
+  if (A.sell_group == 4) {
+    // 4 wavefronts per slice, one group in flight per step.  On the first
+    // AMG level of the 128^3 problem (solves FP64 / FP32-refined, ms): one
+    // wavefront per slice 6.52 / 4.38 (groups of 1: 6.52 / 4.34); two per
+    // slice 6.40 / 4.26; four 6.35 / 4.23 (two groups per step) and
+    // 6.34 / 4.22 (one).
+    constexpr int KS = 4;
+    const int blocks = xcd_grid(A.nslices, (kBlock / 64) / KS, kMaxBlocks);
+    hipLaunchKernelGGL((k_sell<T, EPI, DOT, 4, 1, KS>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices,
+                       A.sell_ptr.p, A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, cd,
+                       ws.partials.p);
+    if (DOT) finish(1, s, op, blocks, ws, check_done);
+  } else {
+    // 8 entries per unrolled step: on the first AMG level of the 128^3
+    // problem 16 per step measured slower in both precisions (solve 6.69
+    // against 6.60 ms FP64, 4.88 against 4.83 ms FP32)
+    const int blocks = xcd_grid(A.nslices, kBlock / 64);
+    hipLaunchKernelGGL((k_sell<T, EPI, DOT, 1, 8, 1>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.nslices,
+                       A.sell_ptr.p, A.sell_delta.p, A.sell_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, cd,
+                       ws.partials.p);
+    if (DOT) finish(1, s, op, blocks, ws, check_done);
+  }
 }
 
 template <typename T, int EPI, bool DOT>

@@ -189,39 +189,62 @@ void smoother_dinv(const HostCsr &h, std::vector<double> &d) {
 }  // namespace
 
 namespace {
-// Sliced-ELL layout for square matrices of at least 4096 rows when every
-// offset fits in int16, rows average at most 64 nonzeros and the padding
-// stays under 35 % (7-point stencils: ~0.2 %; the first smoothed-aggregation
-// Galerkin level of a 3D Poisson problem: ~28 %).
-bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t> &delta, std::vector<double> &val) {
-  if (h.n != h.m || h.n < 4096) return false;
-  if ((double)h.nnz() > 64.0 * (double)h.n) return false;
+// Slice start offsets of the sliced-ELL layout with groups of gs slots: slice
+// s holds 64 lanes x (its longest row rounded up to gs) slots.  Returns the
+// total slot count, or -1 past int32.
+int64_t sell_slices(const HostCsr &h, int gs, std::vector<int32_t> &ptr) {
   const int64_t nslices = (h.n + 63) / 64;
   ptr.assign((size_t)nslices + 1, 0);
   int64_t total = 0;
   for (int64_t s = 0; s < nslices; ++s) {
     int64_t w = 0;
-    for (int64_t r = s * 64; r < std::min<int64_t>(h.n, s * 64 + 64); ++r) {
-      w = std::max<int64_t>(w, h.rp[r + 1] - h.rp[r]);
-      for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) {
-        int64_t d = (int64_t)h.ci[k] - r;
-        if (d < -32767 || d > 32767) return false;
-      }
-    }
-    total += w * 64;
-    if (total >= INT32_MAX) return false;
-    ptr[s + 1] = (int32_t)total;
+    for (int64_t r = s * 64; r < std::min<int64_t>(h.n, s * 64 + 64); ++r) w = std::max<int64_t>(w, h.rp[r + 1] - h.rp[r]);
+    total += (w + gs - 1) / gs * gs * 64;
+    if (total >= INT32_MAX) return -1;
+    ptr[(size_t)s + 1] = (int32_t)total;
   }
-  if ((double)total > 1.35 * (double)h.nnz() + 64.0 * 16.0) return false;
+  return total;
+}
+
+// Position of entry q of the row in lane l of a slice starting at `start`.
+inline int64_t sell_pos(int64_t start, int64_t q, int64_t l, int gs) {
+  return start + q / gs * 64 * gs + l * gs + q % gs;
+}
+
+// Sliced-ELL layout for square matrices of at least 4096 rows when every
+// offset fits in int16, rows average at most 64 nonzeros and the padding
+// stays under 35 % (7-point stencils: ~0.2 %; the first smoothed-aggregation
+// Galerkin level of a 3D Poisson problem: ~28 %).  Groups of 4 slots when
+// they stay under that limit, else groups of 1; gs returns the choice.
+bool build_sell(const HostCsr &h, std::vector<int32_t> &ptr, std::vector<int16_t> &delta, std::vector<double> &val,
+                int &gs) {
+  if (h.n != h.m || h.n < 4096) return false;
+  if ((double)h.nnz() > 64.0 * (double)h.n) return false;
+  for (int64_t r = 0; r < h.n; ++r)
+    for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) {
+      const int64_t d = (int64_t)h.ci[k] - r;
+      if (d < -32767 || d > 32767) return false;
+    }
+  const double limit = 1.35 * (double)h.nnz() + 64.0 * 16.0;
+  int64_t total = -1;
+  for (int g : {4, 1}) {
+    total = sell_slices(h, g, ptr);
+    if (total >= 0 && (double)total <= limit) {
+      gs = g;
+      break;
+    }
+    total = -1;
+  }
+  if (total < 0) return false;
   delta.assign((size_t)total, kSellPad);
   val.assign((size_t)total, 0.0);
-  for (int64_t s = 0; s < nslices; ++s)
+  for (int64_t s = 0; s + 1 < (int64_t)ptr.size(); ++s)
     for (int64_t l = 0; l < 64; ++l) {
       const int64_t r = s * 64 + l;
       if (r >= h.n) break;
-      int64_t k0 = h.rp[r];
+      const int64_t k0 = h.rp[r];
       for (int64_t k = k0; k < h.rp[r + 1]; ++k) {
-        size_t pos = (size_t)(ptr[s] + (k - k0) * 64 + l);
+        const size_t pos = (size_t)sell_pos(ptr[(size_t)s], k - k0, l, gs);
         delta[pos] = (int16_t)((int64_t)h.ci[k] - r);
         val[pos] = h.v[k];
       }
@@ -384,7 +407,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   std::vector<int32_t> sp;
   std::vector<int16_t> sd;
   std::vector<double> sv;
-  sell = allow_sell && want_sell && !dia && build_sell(h, sp, sd, sv);
+  sell = allow_sell && want_sell && !dia && build_sell(h, sp, sd, sv, sell_group);
   if (!sell) {
     sell_ptr.release();
     sell_delta.release();
@@ -428,18 +451,12 @@ void DevCsr::build_maps(const HostCsr &h) {
       }
     }
   if (sell) {
-    // the layout of build_sell: slice s starts after 64 * (longest row) slots
-    // of every slice before it; entry q of row r sits at start + q * 64 + lane
-    int64_t start = 0;
-    for (int64_t s0 = 0; s0 * 64 < h.n; ++s0) {
-      const int64_t r1 = std::min<int64_t>(h.n, s0 * 64 + 64);
-      int64_t w = 0;
-      for (int64_t r = s0 * 64; r < r1; ++r) {
-        w = std::max<int64_t>(w, h.rp[r + 1] - h.rp[r]);
-        for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) pos[(size_t)k] = (int32_t)(start + (k - h.rp[r]) * 64 + (r - s0 * 64));
-      }
-      start += w * 64;
-    }
+    // the layout of build_sell
+    std::vector<int32_t> sp;
+    sell_slices(h, sell_group, sp);
+    for (int64_t r = 0; r < h.n; ++r)
+      for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k)
+        pos[(size_t)k] = (int32_t)sell_pos(sp[(size_t)(r / 64)], k - h.rp[r], r % 64, sell_group);
   }
   fmt_pos.upload(pos.data(), pos.size());
   if (!mirror.empty()) fmt_mirror.upload(mirror.data(), mirror.size());

@@ -135,17 +135,29 @@ def _banded_csr(n, lo, hi, band, seed, empty_every=0):
     return np.array(rp, np.int64), np.array(ci, np.int64), np.array(v)
 
 
-@pytest.mark.parametrize("n", [4096, 5003])
-def test_spmv_sell_integer_bit_exact(n):
-    # sliced ELL with lanes ordered by row length and bounds-checked loads over
-    # the padding: uneven rows (and, at 5003, empty rows and a short last
-    # slice), bit-exact in FP64 and FP32 on integer data; a non-finite x at a
-    # column a row does not hold must not reach that row through the padding
+def _sell_slots(rp, group):
+    # slots of the sliced-ELL layout: per 64-row slice, 64 lanes x the longest
+    # row rounded up to the group size
+    w = np.diff(rp)
+    w = np.concatenate([w, np.zeros((-w.size) % 64, w.dtype)]).reshape(-1, 64).max(axis=1)
+    return int(np.sum(-(-w // group) * group) * 64)
+
+
+@pytest.mark.parametrize("n,lo,hi,group", [(4096, 20, 30, 4), (5003, 20, 30, 4), (4096, 5, 5, 1), (5003, 5, 5, 1)])
+def test_spmv_sell_integer_bit_exact(n, lo, hi, group):
+    # sliced ELL with bounds-checked loads over the padding: uneven rows (and,
+    # at 5003, empty rows and a short last slice), bit-exact in FP64 and FP32
+    # on integer data; a non-finite x at a column a row does not hold must
+    # not reach that row through the padding.  Rows of 20..30 entries take
+    # groups of 4 slots per lane; rows of exactly 5 would be padded to 8 by
+    # groups of 4, so they take groups of 1.
     import torch
 
-    rp, ci, v = _banded_csr(n, 20, 30, 3000, seed=n, empty_every=97 if n % 2 else 0)
+    rp, ci, v = _banded_csr(n, lo, hi, 3000, seed=n, empty_every=97 if n % 2 else 0)
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
-    assert a.storage()["format"] == "sell64"
+    st = a.storage()
+    assert st["format"] == "sell64"
+    assert st["stored_entries"] == _sell_slots(rp, group)
     orc = O.Csr.from_arrays(rp, ci, v, n)
     x = np.round(P.uniform(9, n, -20.0, 20.0))
     yo = orc.spmv(x)
