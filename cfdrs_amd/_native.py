@@ -56,11 +56,11 @@ class AmgConfig(C.Structure):
                 ("relaxation_factor", C.c_double), ("strength_threshold", C.c_double),
                 ("smoothing_omega", C.c_double), ("max_aggregate_size", C.c_int32),
                 ("max_interpolation_points", C.c_int32), ("galerkin_on_device", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("separate_coarse_restriction", C.c_int32)]
 
 
 class AmgStatistics(C.Structure):
-    _fields_ = [("num_levels", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("num_levels", C.c_int32), ("fused_level", C.c_int32),
                 ("level_rows", C.c_int64 * MAX_LEVELS), ("level_nnz", C.c_int64 * MAX_LEVELS),
                 ("operator_complexity", C.c_double), ("grid_complexity", C.c_double),
                 ("setup_time", C.c_double)]

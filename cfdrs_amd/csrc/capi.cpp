@@ -570,6 +570,7 @@ cfdla_status cfdla_amg_stats(cfdla_precond p, cfdla_amg_statistics *out) {
     amg->sync_host();
     const auto &lv = amg->host.levels;
     out->num_levels = (int32_t)lv.size();
+    out->fused_level = amg->on_device ? amg->fused.level : -1;
     auto count = [](const HostCsr &a) {
       int64_t c = 0;
       for (double x : a.v) c += std::fabs(x) > 1e-15;

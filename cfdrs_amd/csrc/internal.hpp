@@ -304,6 +304,9 @@ void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *
 bool symmetric_on_device(const DevCsr &d);
 // FP32 copy, diagonal / sliced-ELL arrays and smoother diagonal from v64.
 void scatter_formats(DevCsr &d);
+// S (into sv) and E (into ev) values of a level's fused operator from A's
+// values and smoother diagonal (fused_jacobi.hpp); both follow A's slots.
+void fused_values(const DevCsr &A, double w, double *sv, double *ev);
 // SIMPLE pressure-correction interior rows into v64 (5 sorted slots from
 // rp[idx]) and rhs; with want_max, max |imbalance| is reduced on the device
 // for simple_max_read (which synchronizes).  The row kernel's block maxima
@@ -383,7 +386,7 @@ void f64_to_f32(int n, const double *src, float *dst, Scal *s, bool check_done);
 
 // x = Ainv * b (dense row-major FP64 inverse), b/x of type T.
 template <typename T>
-void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done);
+void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done, int parts = 1);
 
 // BiCGSTAB vector kernels
 template <typename T>

@@ -244,8 +244,8 @@ __device__ __forceinline__ void load_quad_cols(const uint16_t *p, int base, int 
 // row from LDS with 1..64 threads (the largest power of two that fits the
 // block's row count) and a shuffle tree.  Row lengths of 4..700 nonzeros (AMG
 // transfer operators and Galerkin products) all keep every lane busy with
-// 8 loads in flight.  A row longer than rb_cap is a block of its own and is
-// summed block-strided straight from memory.
+// 8 loads in flight.  A row longer than rb_cap is a block of its own, summed
+// in steps of rb_cap entries straight from memory.
 //
 // Blocks are dealt to the 8 XCDs round-robin; XCD q walks the q-th
 // contiguous range of row blocks, so neighbouring rows share its L2.
@@ -301,10 +301,41 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
     }
     const int r0 = d0.x, e0 = d0.y, nr = d1.x - d0.x, ne = d1.y - d0.y;
     if (ne > CAP) {
-      // a single long row, summed block-strided from memory
+      // a single long row, CAP entries per step: every thread loads Q quads
+      // (all in flight before their gathers) and keeps a running sum
       const EpiIn<T> e = epi_load<T, EPI, DOT>(r0, x, b, dinv, omega, y, dotw);
+      const int ea = e0 & ~3, e1 = e0 + ne;
+      const int safe = C16 ? b0 : 0;
       T acc = T(0);
-      for (int k = tid; k < ne; k += kBlock) acc += av[e0 + k] * gather<T, EPI>(col(e0 + k), x, b, dinv, omega);
+      for (int base = ea; base < e1; base += CAP) {
+        int c[Q][4];
+        T v[Q][4];
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+          const int k = base + 4 * (tid + u * kBlock);
+          if (k < e1) {
+            if constexpr (C16) load_quad_cols<NT>(c16 + k, b0, c[u]);
+            else load_quad_cols<NT>(ci + k, c[u]);
+            load_quad<NT>(av + k, v[u]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              c[u][j] = safe;
+              v[u][j] = T(0);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+          const int k = base + 4 * (tid + u * kBlock);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j >= e0 && k + j < e1;
+            const T xv = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
+            acc += ok ? v[u][j] * xv : T(0);
+          }
+        }
+      }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
       if ((tid & 63) == 0) wsum[tid >> 6] = acc;
@@ -389,10 +420,21 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
       }
       __syncthreads();
       if (wide) {
+        // four running sums per thread, so four LDS reads are in flight
+        // (one sum waited for every read: on blocks of a few long rows,
+        // 32 dependent reads per thread)
         const bool own = tid / tpr < nr;
         const int ke = own ? kr[0] : ks[0];
-        T sum = T(0);
-        for (int k = ks[0] + lane; k < ke; k += tpr) sum += prod[k];
+        T s4[4] = {T(0), T(0), T(0), T(0)};
+        int k = ks[0] + lane;
+        for (; k + 3 * tpr < ke; k += 4 * tpr) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) s4[u] += prod[k + u * tpr];
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+          if (k + u * tpr < ke) s4[u] += prod[k + u * tpr];
+        T sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
         if (own && lane == 0) epi_store<T, EPI, DOT>(e[0], r0 + tid / tpr, sum, omega, y, dotw, dacc, y2);
       } else {
@@ -694,16 +736,40 @@ __global__ __launch_bounds__(kBlock) void k_f64_to_f32(int n, const double *__re
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dst[i] = (float)src[i];
 }
 
+// x = A^-1 b with the dense inverse, one wavefront per row.  b[j] is the
+// sum of `parts` consecutive entries (parts > 1: the partial sums of a
+// restriction whose long rows were split into pieces).  Each block stages
+// the sums in LDS while its rows' first 16 x 64 inverse entries are already
+// in flight; the row sum runs in the order j = lane, lane + 64, ...
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double *__restrict__ a,
-                                                       const T *__restrict__ b, T *__restrict__ x, Scal *s,
-                                                       int check_done) {
+                                                       const T *__restrict__ b, int parts, T *__restrict__ x,
+                                                       Scal *s, int check_done) {
   if (check_done && s->done) return;
+  extern __shared__ double bs[];  // n sums
+  constexpr int PF = 16;
   const int lane = threadIdx.x & 63;
   const int row = (int)((blockIdx.x * (unsigned)kBlock + threadIdx.x) >> 6);
+  double av[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int j = lane + 64 * u;
+    av[u] = row < n && j < n ? a[(size_t)row * n + j] : 0.0;
+  }
+  for (int j = threadIdx.x; j < n; j += kBlock) {
+    double bj = (double)b[(size_t)j * parts];
+    for (int k = 1; k < parts; ++k) bj += (double)b[(size_t)j * parts + k];
+    bs[j] = bj;
+  }
+  __syncthreads();
   if (row >= n) return;  // whole wave leaves together
   double acc = 0.0;
-  for (int j = lane; j < n; j += 64) acc += a[(size_t)row * n + j] * (double)b[j];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int j = lane + 64 * u;
+    if (j < n) acc += av[u] * bs[j];
+  }
+  for (int j = lane + 64 * PF; j < n; j += 64) acc += a[(size_t)row * n + j] * bs[j];
   acc = wave_sum(acc);
   if (lane == 0) x[row] = (T)acc;
 }
@@ -1000,11 +1066,11 @@ void f64_to_f32(int n, const double *src, float *dst, Scal *s, bool check_done) 
 }
 
 template <typename T>
-void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done) {
+void dense_gemv(int n, const double *ainv, const T *b, T *x, Scal *s, bool check_done, int parts) {
   int blocks = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_dense_gemv<T>), dim3(blocks), dim3(kBlock), 0, stream(), n, ainv, b, x, s,
-                     check_done ? 1 : 0);
+  hipLaunchKernelGGL((k_dense_gemv<T>), dim3(blocks), dim3(kBlock), (size_t)n * sizeof(double), stream(), n, ainv,
+                     b, parts, x, s, check_done ? 1 : 0);
   CFDLA_HIP(hipGetLastError());
 }
 
@@ -1118,7 +1184,7 @@ void coef_grad_poisson2d(int64_t nx, int64_t ny, double inv_h2, int dir_mask, co
   template void cg_direction<T>(int, const T *, T *, Scal *);                                                  \
   template void copy_vec<T>(int, const T *, T *, Scal *, bool);                                                \
   template void fill_vec<T>(int, T, T *);                                                                      \
-  template void dense_gemv<T>(int, const double *, const T *, T *, Scal *, bool);                             \
+  template void dense_gemv<T>(int, const double *, const T *, T *, Scal *, bool, int);                             \
   template void bicg_p_update<T>(int, const T *, const T *, T *, Scal *);                                      \
   template void bicg_s_update<T>(int, const T *, const T *, T *, Scal *, RedWs &);                             \
   template void bicg_x_r_update<T>(int, const T *, const T *, const T *, const T *, T *, T *, Scal *, RedWs &,  \

@@ -20,6 +20,7 @@
 #include <climits>
 
 #include "internal.hpp"
+#include "fused_jacobi.hpp"
 #include "simple_stencil.hpp"
 #include "solver.hpp"
 
@@ -427,6 +428,32 @@ bool symmetric_on_device(const DevCsr &d) {
   hipLaunchKernelGGL(k_mirror_check, dim3(grid_for(d.nnz)), dim3(kBlock), 0, stream(), d.nnz, d.v64.p,
                      d.fmt_mirror.p, bad);
   return flag_read() == 0;
+}
+
+namespace {
+// S and E values of a level's fused operator from A's values and the
+// smoother's inverse diagonal, one lane per row (fused_jacobi.hpp).
+__global__ __launch_bounds__(kBlock) void k_fused_values(int n, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ ci,
+                                                         const double *__restrict__ av,
+                                                         const double *__restrict__ dinv, double w, double *sv,
+                                                         double *ev) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double di = dinv[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      const double a = av[k], dj = dinv[j];
+      sv[k] = fused_s(i == j, w, a, di, dj);
+      ev[k] = fused_e(i == j, w, a, dj);
+    }
+  }
+}
+}  // namespace
+
+void fused_values(const DevCsr &A, double w, double *sv, double *ev) {
+  if (A.n > 0)
+    hipLaunchKernelGGL(k_fused_values, dim3(grid_for(A.n)), dim3(kBlock), 0, stream(), A.n, A.rp.p, A.ci.p, A.v64.p,
+                       A.sdinv64.p, w, sv, ev);
 }
 
 void scatter_formats(DevCsr &d) {

@@ -8,6 +8,7 @@
 // waits for the host, and kernels of steps enqueued after convergence exit
 // immediately.
 #include "solver.hpp"
+#include "fused_jacobi.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -636,6 +637,8 @@ template <> double *DevLevel::t<double>() { return t64.p; }
 template <> float *DevLevel::t<float>() { return t32.p; }
 template <> double *DevLevel::r<double>() { return r64.p; }
 template <> float *DevLevel::r<float>() { return r32.p; }
+template <> double *DevLevel::fx<double>() { return fx64.p; }
+template <> float *DevLevel::fx<float>() { return fx32.p; }
 
 AmgPrecond::AmgPrecond(const HostCsr &a, const cfdla_amg_config &c) : cfg(c) {
   CFDLA_REQUIRE(a.n == a.m, CFDLA_ERR_DIMENSION, "AMG requires a square matrix");
@@ -721,6 +724,10 @@ void AmgPrecond::upload() {
     }
     d.t64.alloc(n); d.r64.alloc(n); d.t32.alloc(n); d.r32.alloc(n);
   }
+  fused = FusedOp{};
+  if (L >= 3 && cfg.smoother == 2 && cfg.coarse_solver == 1 && cfg.pre_smooth_iterations == 2 &&
+      !cfg.separate_coarse_restriction)
+    build_fused(L - 2);
   coarse_n = (int)host.levels.back().A.n;
   if (cfg.coarse_solver == 1) {
     CFDLA_REQUIRE(coarse_n <= 8192, CFDLA_ERR_CONFIG,
@@ -749,7 +756,88 @@ void refresh_level(DevCsr &d, HostCsr &h, bool want_dinv) {
   d.general_dia = true;
   d.upload(h, true, want_dinv);
 }
+
+void put_plan(DevPlan &d, const ProductPlan &p) {
+  d.nslots = (int)p.ptr.size() - 1;
+  d.ptr.upload(p.ptr.data(), p.ptr.size());
+  d.pa.upload(p.pa.data(), p.pa.size());
+  d.pb.upload(p.pb.data(), p.pb.size());
+  sync_stream();
+}
 }  // namespace
+
+bool AmgPrecond::build_fused(size_t l) {
+  const HostCsr &A = host.levels[l].A, &R = host.levels[l].R;
+  for (int64_t i = 0; i < A.n; ++i) {
+    bool diag = false;
+    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) diag = diag || A.ci[k] == i;
+    if (!diag) return false;
+  }
+  std::vector<double> d;
+  smoother_dinv(A, d);
+  const double w = cfg.relaxation_factor;
+  FusedOp f;
+  f.E = A;
+  HostCsr S = A;
+  for (int64_t i = 0; i < A.n; ++i)
+    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+      const int64_t j = A.ci[k];
+      f.E.v[k] = fused_e(i == j, w, A.v[k], d[j]);
+      S.v[k] = fused_s(i == j, w, A.v[k], d[i], d[j]);
+    }
+  f.T = csr_multiply(R, f.E);
+  f.M = csr_multiply(f.T, f.E);
+  // F stands in for a sweep and a residual over A and the restriction R;
+  // keep the three products when it would stream more than twice their bytes
+  if ((double)(S.nnz() + f.M.nnz()) > 2.0 * (2.0 * (double)A.nnz() + (double)R.nnz())) return false;
+  // M's rows (thousands of entries on the level above a 641-row coarsest
+  // level) are cut into `parts` pieces of at most one row block's capacity,
+  // so they pack into ordinary row blocks instead of running as single-
+  // block long rows (31 against 13 us FP64 at 128^3); the dense coarse solve
+  // adds each row's pieces.  Pieces are consecutive slices of the row, so
+  // F's values are S's followed by M's unchanged.
+  const int64_t piece = rb_cap<double>();
+  int64_t parts = 1;
+  for (int64_t j = 0; j < f.M.n; ++j) parts = std::max(parts, (f.M.rp[j + 1] - f.M.rp[j] + piece - 1) / piece);
+  HostCsr &F = f.F;
+  F.n = A.n + f.M.n * parts;
+  F.m = A.m;
+  F.rp = S.rp;
+  F.rp.reserve((size_t)F.n + 1);
+  for (int64_t j = 0; j < f.M.n; ++j)
+    for (int64_t k = 1; k <= parts; ++k)
+      F.rp.push_back(S.nnz() + std::min(f.M.rp[j + 1], f.M.rp[j] + k * piece));
+  F.ci = S.ci;
+  F.ci.insert(F.ci.end(), f.M.ci.begin(), f.M.ci.end());
+  F.v = S.v;
+  F.v.insert(F.v.end(), f.M.v.begin(), f.M.v.end());
+  DevLevel &dl = lv[l];
+  dl.F.upload(F, true, false, false);
+  dl.fx64.alloc((size_t)F.n);
+  dl.fx32.alloc((size_t)F.n);
+  f.level = (int)l;
+  f.parts = (int)parts;
+  fused = std::move(f);
+  return true;
+}
+
+void AmgPrecond::refresh_fused_device() {
+  const size_t l = (size_t)fused.level;
+  DevLevel &d = lv[l];
+  FusedOp &f = fused;
+  if (!f.plans) {
+    put_plan(f.t, product_plan(host.levels[l].R, f.E, f.T));
+    put_plan(f.m, product_plan(f.T, f.E, f.M));
+    f.e64.alloc((size_t)f.E.nnz());
+    f.t64.alloc((size_t)f.T.nnz());
+    f.plans = true;
+  }
+  // S into F's first rows, E beside; then T = R E and M = T E into F's last rows
+  fused_values(d.A, cfg.relaxation_factor, d.F.v64.p, f.e64.p);
+  product_numeric(f.t.nslots, f.t.ptr.p, f.t.pa.p, f.t.pb.p, d.R.v64.p, f.e64.p, f.t64.p);
+  product_numeric(f.m.nslots, f.m.ptr.p, f.m.pa.p, f.m.pb.p, f.t64.p, f.e64.p, d.F.v64.p + f.E.nnz());
+  refresh_level(d.F, f.F, false);
+}
 
 void AmgPrecond::update_values_device(const double *fine_values) {
   ensure_device();
@@ -765,18 +853,11 @@ void AmgPrecond::update_values_device(const double *fine_values) {
     // host product's)
     gplan.clear();
     gplan.resize(L - 1);
-    auto put = [](DevPlan &d, const ProductPlan &p) {
-      d.nslots = (int)p.ptr.size() - 1;
-      d.ptr.upload(p.ptr.data(), p.ptr.size());
-      d.pa.upload(p.pa.data(), p.pa.size());
-      d.pb.upload(p.pb.data(), p.pb.size());
-      sync_stream();
-    };
     for (size_t l = 0; l + 1 < L; ++l) {
       const HostLevel &h = host.levels[l];
       const HostCsr ra_pat = csr_multiply(h.R, h.A);
-      put(gplan[l].ra, product_plan(h.R, h.A, ra_pat));
-      put(gplan[l].rap, product_plan(ra_pat, h.P, host.levels[l + 1].A));
+      put_plan(gplan[l].ra, product_plan(h.R, h.A, ra_pat));
+      put_plan(gplan[l].rap, product_plan(ra_pat, h.P, host.levels[l + 1].A));
       gplan[l].ra_v.alloc((size_t)ra_pat.nnz());
     }
   }
@@ -787,6 +868,7 @@ void AmgPrecond::update_values_device(const double *fine_values) {
     product_numeric(g.rap.nslots, g.rap.ptr.p, g.rap.pa.p, g.rap.pb.p, g.ra_v.p, f.P.v64.p, c.A.v64.p);
     refresh_level(c.A, host.levels[l + 1].A, true);
   }
+  if (fused.level >= 0) refresh_fused_device();
   if (cfg.coarse_solver == 1) {
     const DevCsr &cA = lv.back().A;
     dense_inverse_device(coarse_n, cA.rp.p, cA.ci.p, cA.v64.p, coarse_inv.p);
@@ -806,6 +888,7 @@ T *AmgPrecond::first_sweep_target(int l) {
   DevLevel &L = lv[(size_t)l];
   const bool coarsest = (size_t)l + 1 == lv.size();
   if (coarsest && cfg.coarse_solver == 1) return nullptr;
+  if (l == fused.level) return nullptr;  // the fused product starts from b alone
   const int down = coarsest ? cfg.pre_smooth_iterations + 10 : cfg.pre_smooth_iterations;
   const int up = coarsest ? 0 : cfg.post_smooth_iterations;
   if (down < 1 || (down >= 2 && L.A.nnz <= 8 * (int64_t)L.A.n)) return nullptr;
@@ -876,6 +959,27 @@ void AmgPrecond::vcycle(int l, const T *b, T *x, Scal *s, bool chk, int dot_op, 
   // the coarsest level 10 more sweeps (amg.rs:501).
   const int down = coarsest ? cfg.pre_smooth_iterations + 10 : cfg.pre_smooth_iterations;
   const int up = coarsest ? 0 : cfg.post_smooth_iterations;
+  if (l == fused.level) {
+    // the two pre-sweeps from zero, the residual and the restriction as one
+    // product: [x; pieces of b_c] = F b (fused_jacobi.hpp), then the dense
+    // coarse solve (adding the pieces), the prolongation and the
+    // post-sweeps, the last one into x
+    DevLevel &C = lv[(size_t)l + 1];
+    T *fx = L.fx<T>();
+    csr_apply<T>(L.F, EPI_Y, b, nullptr, nullptr, 0.0, fx, s, chk, OP_NONE, nullptr, ws);
+    dense_gemv<T>(C.A.n, coarse_inv.p, fx + n, C.x<T>(), s, chk, fused.parts);
+    csr_apply<T>(L.P, EPI_ADD, C.x<T>(), nullptr, nullptr, 0.0, fx, s, chk, OP_NONE, nullptr, ws);
+    const T *src = fx;
+    for (int j = 1; j <= up; ++j) {
+      T *dst = (up - j) % 2 == 0 ? x : L.t<T>();
+      const int op = j == up ? dot_op : OP_NONE;
+      csr_apply<T>(L.A, EPI_JACOBI, src, b, dinv, w, dst, s, chk, op, op != OP_NONE ? b : nullptr, ws);
+      src = dst;
+    }
+    if (up == 0) copy_vec<T>(n, fx, x, s, chk);
+    if (up == 0 && dot_op != OP_NONE) dot2<T>(n, b, x, nullptr, nullptr, s, chk, dot_op, ws);
+    return;
+  }
   T *buf[2] = {x, L.t<T>()};
   // number of buffer flips after the first sweep; the iterate must end in x
   const int flips = (down > 0 ? down - 1 : 0) + up;

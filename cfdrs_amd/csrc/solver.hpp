@@ -88,10 +88,16 @@ struct DevLevel {
   DevBuf<int32_t> gs_perm;
   DevBuf<double> b64, x64, t64, r64;
   DevBuf<float> b32, x32, t32, r32;
+  // The fused pre-smoothing and restriction of this level (fused_jacobi.hpp,
+  // F = [S; M]) and the output of its product, [x_l; b_{l+1}].
+  DevCsr F;
+  DevBuf<double> fx64;
+  DevBuf<float> fx32;
   template <typename T> T *b();
   template <typename T> T *x();
   template <typename T> T *t();
   template <typename T> T *r();
+  template <typename T> T *fx();
 };
 
 // Device copy of a ProductPlan.
@@ -105,6 +111,19 @@ struct DevPlan {
 struct GalerkinPlan {
   DevPlan ra, rap;
   DevBuf<double> ra_v;
+};
+
+// The fused operator of one level (fused_jacobi.hpp): the host F = [S; M],
+// the patterns of E = I - w A D (A's), T = R E and M = T E, and for the
+// device refresh the per-slot plans of the two products with E's and T's
+// device values.
+struct FusedOp {
+  int level = -1;
+  int parts = 1;  // pieces per row of M in F (the coarse solve adds them)
+  HostCsr F, E, T, M;
+  DevPlan t, m;
+  DevBuf<double> e64, t64;
+  bool plans = false;
 };
 
 // C = A B on the device (galerkin.hip), bit-identical to csr_multiply: C's
@@ -124,6 +143,12 @@ struct AmgPrecond : Precond {
   bool on_device = false;
   bool host_stale = false;  // a device refresh left newer level values than host
   std::vector<GalerkinPlan> gplan;  // per level, made on the first device refresh
+  // The level before a dense coarsest one runs its two Jacobi pre-sweeps
+  // from zero, residual and restriction as one product when the smoother is
+  // Jacobi with two pre-sweeps (level -1: none).
+  FusedOp fused;
+  bool build_fused(size_t l);
+  void refresh_fused_device();
   // AlgebraicMultigrid::recompute (amg.rs:179-214) on the device from
   // level-0 values already on the device (CSR order of the fine matrix).
   void update_values_device(const double *fine_values);

@@ -75,6 +75,10 @@ class AMGConfig:
     # form the Galerkin products on the GPU (bit-identical to the host
     # products; no reference counterpart)
     galerkin_on_device: bool = False
+    # keep the pre-sweeps, residual and restriction of the level above a
+    # dense coarsest level as three products instead of one fused product
+    # (the same operator in exact arithmetic; no reference counterpart)
+    separate_coarse_restriction: bool = False
 
     @classmethod
     def default(cls) -> "AMGConfig":
@@ -109,7 +113,8 @@ class AMGConfig:
                            int(self.interpolation_strategy), int(self.smoother_type), self.pre_smooth_iterations,
                            self.post_smooth_iterations, int(self.coarse_solver), self.relaxation_factor,
                            self.strength_threshold, self.smoothing_omega, self.max_aggregate_size,
-                           self.max_interpolation_points, 1 if self.galerkin_on_device else 0, 0)
+                           self.max_interpolation_points, 1 if self.galerkin_on_device else 0,
+                           1 if self.separate_coarse_restriction else 0)
 
 
 @dataclass
@@ -120,6 +125,9 @@ class AMGStatistics:
     operator_complexity: float = 1.0
     grid_complexity: float = 1.0
     setup_time: float = 0.0
+    # the level whose pre-sweeps and restriction run as one product (-1:
+    # none, or the hierarchy is not on the device yet)
+    fused_level: int = -1
 
 
 class _Preconditioner:
@@ -232,7 +240,7 @@ class AlgebraicMultigrid(_Preconditioner):
         N.call("cfdla_amg_stats", self.handle, C.byref(s))
         L = s.num_levels
         return AMGStatistics(L, list(s.level_rows[:L]), list(s.level_nnz[:L]), s.operator_complexity,
-                             s.grid_complexity, s.setup_time)
+                             s.grid_complexity, s.setup_time, s.fused_level)
 
     def level(self, level: int, which: str = "A"):
         """(row_ptr, col_indices, values, nrows, ncols) of A_l, R_l or P_l."""

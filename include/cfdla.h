@@ -139,7 +139,13 @@ typedef struct {
    * (bit-identical to the host products; their plans also serve
    * cfdla_amg_update_values_device).  No reference counterpart. */
   int32_t galerkin_on_device;
-  int32_t reserved;
+  /* 1: keep the two Jacobi pre-sweeps, the residual and the restriction of
+   * the level above a dense coarsest level as three products.  0 (default):
+   * when the smoother is Jacobi with two pre-sweeps and the coarsest level
+   * is solved directly, that level runs them as one product with
+   * F = [2wD - w^2 DAD; R (I - wAD)^2] (D the smoother's inverse diagonal),
+   * the same operator in exact arithmetic.  No reference counterpart. */
+  int32_t separate_coarse_restriction;
 } cfdla_amg_config;
 
 #define CFDLA_MAX_LEVELS 32
@@ -147,7 +153,9 @@ typedef struct {
 /* AMGStatistics (multigrid/mod.rs:366-395). */
 typedef struct {
   int32_t num_levels;
-  int32_t reserved;
+  /* the level whose pre-sweeps and restriction run as one product (see
+   * separate_coarse_restriction); -1 none or not yet on the device */
+  int32_t fused_level;
   int64_t level_rows[CFDLA_MAX_LEVELS];
   int64_t level_nnz[CFDLA_MAX_LEVELS];
   double operator_complexity;

@@ -378,6 +378,42 @@ def test_amg_vcycle_matches_oracle(case, nn):
     assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
 
 
+@pytest.mark.parametrize("nn", [24, 20])
+def test_fused_coarse_restriction_matches_separate_products(nn):
+    # the level above the dense coarsest one runs its two Jacobi pre-sweeps,
+    # residual and restriction as one product with [2wD - w^2 DAD; R (I - wAD)^2]:
+    # the same V-cycle as the three products up to rounding, and the oracle's
+    import torch
+
+    n = nn ** 3
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4, np.zeros(6))
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, n)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = 100
+    sep = ca.AMGConfig(**{**cfg.__dict__, "separate_coarse_restriction": True})
+    fused, plain = ca.AlgebraicMultigrid(a, cfg), ca.AlgebraicMultigrid(a, sep)
+    r = P.uniform(78, n)
+    z, zs = np.empty(n), np.empty(n)
+    fused.apply_to(r, z)
+    plain.apply_to(r, zs)
+    L = fused.statistics.num_levels
+    assert L >= 3
+    assert fused.statistics.fused_level == L - 2 and plain.statistics.fused_level == -1
+    assert np.linalg.norm(z - zs) <= 1e-13 * np.linalg.norm(zs)
+    oamg = O.Amg(O.Csr.from_arrays(rp, ci, v, n), coarsening="three_phase", interp="smoothed", smoother="jacobi",
+                 omega=cfg.relaxation_factor, coarse_solve="direct", sa_omega=cfg.smoothing_omega,
+                 theta=cfg.strength_threshold, min_coarse=cfg.min_coarse_size)
+    zo = oamg.apply(r)
+    assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+    rt = torch.tensor(r, dtype=torch.float32, device="cuda")
+    z32, zs32 = torch.empty_like(rt), torch.empty_like(rt)
+    fused.apply_to(rt, z32)
+    plain.apply_to(rt, zs32)
+    z32, zs32 = z32.double().cpu().numpy(), zs32.double().cpu().numpy()
+    assert np.linalg.norm(z32 - zo) <= 1e-5 * np.linalg.norm(zo)
+    assert np.linalg.norm(z32 - zs32) <= 1e-5 * np.linalg.norm(zs32)
+
+
 def test_pcg_jacobi_config1():
     rp, ci, v, b = P.config1(128)
     n = 128 * 128
