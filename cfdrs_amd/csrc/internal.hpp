@@ -227,7 +227,9 @@ template <> inline const double *DevCsr::dia_vals<double>() const { return dia_v
 template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32.p; }
 
 
-void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb);
+// span > 0: a block also closes before its columns would span more than
+// `span` (so 16-bit column offsets fit every block whose rows each do).
+void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span = 0);
 
 // Device-resident scalars of one solve.  A reducing kernel writes block
 // partials; the single-block finalize kernel launched after it sums them into

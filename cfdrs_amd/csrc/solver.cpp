@@ -139,18 +139,39 @@ struct Poller {
 // ---------------------------------------------------------------------------
 // Device CSR
 // ---------------------------------------------------------------------------
-void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb) {
+void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span) {
   rb.assign({0, 0});
+  // a row that alone spans more than `span` rules 16-bit offsets out for the
+  // whole partition: then blocks are not cut for the span at all
+  for (int64_t r = 0; r < h.n && span > 0; ++r)
+    if (h.rp[r + 1] > h.rp[r]) {
+      const auto mm = std::minmax_element(h.ci.begin() + h.rp[r], h.ci.begin() + h.rp[r + 1]);
+      if ((int64_t)*mm.second - (int64_t)*mm.first > span) span = 0;
+    }
   int64_t rows = 0, nz = 0;
+  int64_t lo = INT64_MAX, hi = INT64_MIN;  // column range of the open block
   for (int64_t r = 0; r < h.n; ++r) {
     const int64_t len = h.rp[r + 1] - h.rp[r];
-    if (rows > 0 && (nz + len > cap || rows == max_rows)) {
+    int64_t rlo = INT64_MAX, rhi = INT64_MIN;
+    if (span > 0)
+      for (int64_t k = h.rp[r]; k < h.rp[r + 1]; ++k) {
+        rlo = std::min<int64_t>(rlo, h.ci[k]);
+        rhi = std::max<int64_t>(rhi, h.ci[k]);
+      }
+    const bool wide = span > 0 && len > 0 && nz > 0 && std::max(hi, rhi) - std::min(lo, rlo) > span;
+    if (rows > 0 && (nz + len > cap || rows == max_rows || wide)) {
       rb.push_back((int32_t)r);
       rb.push_back((int32_t)h.rp[r]);
       rows = nz = 0;
+      lo = INT64_MAX;
+      hi = INT64_MIN;
     }
     rows += 1;
     nz += len;
+    if (len > 0) {
+      lo = std::min(lo, rlo);
+      hi = std::max(hi, rhi);
+    }
   }
   if (rows > 0) {
     rb.push_back((int32_t)h.n);
@@ -341,10 +362,14 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   // sweep; the per-block descriptor loads and barriers dominate.)
   std::vector<uint16_t> c16v;
   std::vector<int32_t> cb;
-  build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks);
+  // with 16-bit columns allowed, a block also closes before its columns
+  // would span more than 65535 (the first restriction of the 128^3
+  // hierarchy spans up to three planes of 16384 per row)
+  const int64_t span = fmt_request != CFDLA_STORAGE_CSR_INT32 ? 65535 : 0;
+  build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks, span);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
-  const bool allow16 = fmt_request != CFDLA_STORAGE_CSR_INT32;
+  const bool allow16 = span > 0;
   c16_f64 = allow16 && build_cols16(h, blocks, c16v, cb);
   if (c16_f64) {
     c16_64.upload_padded(c16v.data(), c16v.size(), kRbPad);
@@ -356,7 +381,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   }
   c16_f32 = false;
   if (want_fp32) {
-    build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks);
+    build_row_blocks(h, rb_cap<float>(), rb_rows<float>(), blocks, span);
     nrb32 = (int)blocks.size() / 2 - 1;
     rb32.upload(blocks.data(), blocks.size());
     c16_f32 = allow16 && build_cols16(h, blocks, c16v, cb);

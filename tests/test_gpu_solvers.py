@@ -50,14 +50,16 @@ def test_spmv_integer_bit_exact(row_len, m):
 
 # Row blocks hold at most 2048 nonzeros and 512 (FP64) / 256 (FP32) rows, so
 # with 5 entries per row the FP64 blocks are rows [0, 409), [409, 818), ...
-# and the FP32 blocks rows [0, 256), [256, 512), ...  A partition stores 16-bit
-# column offsets when every block's columns satisfy max - min <= 65535.
+# and the FP32 blocks rows [0, 256), [256, 512), ...  A block also closes
+# before its columns would span more than 65535, unless some row alone does,
+# and a partition stores 16-bit column offsets when every block's columns
+# satisfy max - min <= 65535.
 #   span65535: every row holds columns {0, 7, 1000, 30000, 65535}: 16-bit in
 #     both partitions, with the largest offset that fits;
-#   span65536: the last column is 65536: 32-bit in both;
+#   span65536: the last column is 65536: no cut helps, 32-bit in both;
 #   fp32_straddles: rows below 409 hold columns 0..4 and rows from 409 on
-#     70000..70004, so every FP64 block spans 4 (16-bit) while the FP32 block
-#     [256, 512) spans 70004 (32-bit FP32 partition only).
+#     70000..70004, so every FP64 block spans 4 while the FP32 block from 256
+#     closes at row 409: 16-bit in both partitions.
 @pytest.mark.parametrize("case", ["span65535", "span65536", "fp32_straddles"])
 def test_spmv_16bit_column_span_boundary(case):
     import torch
