@@ -354,6 +354,125 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
   }
 }
 
+// Gauss-Jordan steps c0 .. c0 + B - 1 on [A | I] in one launch, for
+// n <= kGjMaxRows.  Every block loads the B panel columns into LDS and
+// replays the B steps on them -- pivot row, swap, scale by d = 1 / m_pc,
+// eliminate with f = m_rc -- with the operations of k_gj_step (and of the
+// host), applying each step to its own kGjCols columns held in registers
+// (rows threadIdx.x + 256 q).  The columns left of the panel are dead, and
+// the panel's own columns are dead once its steps are done, so only columns
+// >= c0 + B are carried, src -> dst.  Each element meets the host's
+// operations in the host's order, so the inverse is the host's bit for bit.
+constexpr int kGjPanel = 8;      // steps per launch (fewer when n * B * 8 B exceeds 60 KiB)
+constexpr int kGjCols = 4;       // columns per block
+constexpr int kGjMaxRows = 1024; // rows held by one block (4 per thread)
+constexpr int kGjPanelLds = 7680;  // doubles of the LDS panel
+__global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const double *src, double *dst,
+                                                     int *bad) {
+#pragma clang fp contract(off)
+  extern __shared__ double pnl[];  // n x B, row-major
+  __shared__ double u[kGjCols];    // row c of this block's columns, scaled
+  __shared__ double sw[2][kGjCols];
+  __shared__ int s_p;
+  constexpr int RQ = kGjMaxRows / kBlock;
+  if (*bad) return;  // an earlier step found the matrix singular
+  const int tid = threadIdx.x;
+  const int64_t w = 2 * (int64_t)n;
+  const int64_t j0 = c0 + B + (int64_t)blockIdx.x * kGjCols;
+  for (int e = tid; e < n * B; e += kBlock) {
+    const int r = e / B, k = e - r * B;
+    pnl[e] = src[(int64_t)r * w + c0 + k];
+  }
+  double x[RQ][kGjCols];
+#pragma unroll
+  for (int q = 0; q < RQ; ++q)
+#pragma unroll
+    for (int jj = 0; jj < kGjCols; ++jj) {
+      const int r = tid + q * kBlock;
+      x[q][jj] = r < n && j0 + jj < w ? src[(int64_t)r * w + j0 + jj] : 0.0;
+    }
+  __syncthreads();
+  for (int k = 0; k < B; ++k) {
+    const int c = c0 + k;
+    Piv mine{-1.0, INT_MAX};
+    for (int r = c + tid; r < n; r += kBlock) mine = piv_better(mine, piv_of(pnl[r * B + k], r));
+    const Piv best = piv_block(mine);
+    if (tid == 0) {
+      const double mcc = pnl[c * B + k];
+      s_p = (mcc != mcc || !(best.val > 0.0 && best.val <= 1.7976931348623157e308)) ? -1 : best.row;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p < 0) {
+      if (tid == 0) *bad = 1;
+      return;
+    }
+    const double d = 1.0 / pnl[p * B + k];
+    if (p != c) {
+      // rows c and p trade places: the panel, then this block's columns
+      if (tid < B) {
+        const double a = pnl[c * B + tid], b = pnl[p * B + tid];
+        pnl[c * B + tid] = b;
+        pnl[p * B + tid] = a;
+      }
+#pragma unroll
+      for (int q = 0; q < RQ; ++q)
+#pragma unroll
+        for (int jj = 0; jj < kGjCols; ++jj) {
+          const int r = tid + q * kBlock;
+          if (r == c) sw[0][jj] = x[q][jj];
+          if (r == p) sw[1][jj] = x[q][jj];
+        }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < RQ; ++q)
+#pragma unroll
+        for (int jj = 0; jj < kGjCols; ++jj) {
+          const int r = tid + q * kBlock;
+          if (r == c) x[q][jj] = sw[1][jj];
+          if (r == p) x[q][jj] = sw[0][jj];
+        }
+    }
+    __syncthreads();
+    // row c scaled by d
+    if (tid >= k && tid < B) pnl[c * B + tid] = pnl[c * B + tid] * d;
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+      if (tid + q * kBlock == c)
+#pragma unroll
+        for (int jj = 0; jj < kGjCols; ++jj) {
+          x[q][jj] = x[q][jj] * d;
+          u[jj] = x[q][jj];
+        }
+    __syncthreads();
+    // every other row r with f = m_rc != 0: m_rj - f m_cj
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      const int r = tid + q * kBlock;
+      if (r >= n || r == c) continue;
+      const double f = pnl[r * B + k];
+      if (f == 0.0) continue;
+      for (int kk = k; kk < B; ++kk) {
+        const double t = f * pnl[c * B + kk];
+        pnl[r * B + kk] = pnl[r * B + kk] - t;
+      }
+#pragma unroll
+      for (int jj = 0; jj < kGjCols; ++jj) {
+        const double t = f * u[jj];
+        x[q][jj] = x[q][jj] - t;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < RQ; ++q)
+#pragma unroll
+    for (int jj = 0; jj < kGjCols; ++jj) {
+      const int r = tid + q * kBlock;
+      if (r < n && j0 + jj < w) dst[(int64_t)r * w + j0 + jj] = x[q][jj];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_gj_extract(int n, const double *m, double *inv) {
   for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * kBlock) {
     const int64_t r = e / n, c = e - r * n;
@@ -530,11 +649,27 @@ void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const dou
   hipLaunchKernelGGL(k_dense_fill, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
   hipLaunchKernelGGL(k_dense_scatter, dim3(std::min(n, kMaxBlocks)), dim3(kBlock), 0, stream(), n, rp, ci, v,
                      buf[0].p);
-  hipLaunchKernelGGL(k_gj_init, dim3(nb), dim3(kBlock), 0, stream(), n, buf[0].p, pv[0].p, pr[0].p);
-  for (int c = 0; c < n; ++c)
-    hipLaunchKernelGGL(k_gj_step, dim3(nb), dim3(kBlock), 0, stream(), n, c, buf[c & 1].p, buf[(c + 1) & 1].p,
-                       pv[c & 1].p, pr[c & 1].p, pv[(c + 1) & 1].p, pr[(c + 1) & 1].p, nb, bad);
-  hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n, buf[n & 1].p,
+  int last = 0;  // the buffer holding the result
+  if (n <= kGjMaxRows) {
+    // B steps per launch (641 launches become 81 for a 641-row level)
+    const int B = std::min(kGjPanel, kGjPanelLds / n);
+    int it = 0;
+    for (int c0 = 0; c0 < n; c0 += B, ++it) {
+      const int b = std::min(B, n - c0);
+      const int64_t live = 2 * (int64_t)n - (c0 + b);
+      const int blocks = (int)((live + kGjCols - 1) / kGjCols);
+      hipLaunchKernelGGL(k_gj_panel, dim3(blocks), dim3(kBlock), (size_t)n * b * sizeof(double), stream(), n, c0,
+                         b, buf[it & 1].p, buf[(it + 1) & 1].p, bad);
+    }
+    last = it & 1;
+  } else {
+    hipLaunchKernelGGL(k_gj_init, dim3(nb), dim3(kBlock), 0, stream(), n, buf[0].p, pv[0].p, pr[0].p);
+    for (int c = 0; c < n; ++c)
+      hipLaunchKernelGGL(k_gj_step, dim3(nb), dim3(kBlock), 0, stream(), n, c, buf[c & 1].p, buf[(c + 1) & 1].p,
+                         pv[c & 1].p, pr[c & 1].p, pv[(c + 1) & 1].p, pr[(c + 1) & 1].p, nb, bad);
+    last = n & 1;
+  }
+  hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n, buf[last].p,
                      inv);
   CFDLA_REQUIRE(flag_read() == 0, CFDLA_ERR_NUMERICAL,
                 "AMG coarsest operator is singular; use coarse_solver = reference sweeps");
