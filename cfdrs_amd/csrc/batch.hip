@@ -87,29 +87,80 @@ struct BlockTeam {
   }
 };
 
-// (A v)_row over the team-local vector v in LDS.  (An ELL copy with a fixed,
-// unrolled slot loop took 256 VGPRs, one block per CU, and ran slower:
-// 39.7 against 28.5 ms for the 4,096 networks in FP64.)
-template <typename T, typename V>
-__device__ __forceinline__ T row_product(const BArgs &A, const T *vals, int row, int r0, const V *v) {
-  T s = T(0);
-  for (int k = A.rp[row]; k < A.rp[row + 1]; ++k) s += vals[k] * (T)v[A.ci[k] - r0];
-  return s;
+// The lane's rows of its team's system, read once per system: their entry
+// ranges and equilibration factors (the matrix and the factors do not
+// change during the solve).  A row the lane does not own gets the empty
+// range at its first row's start, a valid index.
+template <class Team>
+struct LaneRows {
+  int kb[kRowsPerLane], ke[kRowsPerLane];
+  int kmax;  // the longest of the lane's rows
+  double eq[kRowsPerLane];
+  __device__ void load(const BArgs &A, int r0, int n, int tl) {
+    kmax = 0;
+    const int safe = A.rp[r0 + (tl < n ? tl : 0)];
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const int i = tl + j * Team::size;
+      kb[j] = ke[j] = safe;
+      eq[j] = 0.0;
+      if (i < n) {
+        kb[j] = A.rp[r0 + i];
+        ke[j] = A.rp[r0 + i + 1];
+        eq[j] = A.eq[r0 + i];
+        kmax = max(kmax, ke[j] - kb[j]);
+      }
+    }
+  }
+};
+
+// q_j = (A v)_row over the team-local vector v in LDS, for all of the lane's
+// rows at once, slot-major: step k loads the k-th entry of every row, so a
+// step's loads are in flight together and a product costs the longest
+// row's length in memory round trips rather than the sum of the lane's row
+// lengths (row by row it was ~38 of the ~49 us per iteration of a 2,000-row
+// system, the matrix being served from the Infinity Cache).  Each row still
+// sums its entries in order.  (An ELL copy with a fixed, unrolled slot loop
+// took 256 VGPRs, one block per CU, and ran slower: 39.7 against 28.5 ms for
+// the 4,096 networks in FP64.)
+template <typename T, typename V, class Team>
+__device__ __forceinline__ void lane_product(const BArgs &A, const T *vals, const LaneRows<Team> &L, int r0,
+                                             const V *v, T (&q)[kRowsPerLane]) {
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) q[j] = T(0);
+  for (int k = 0; k < L.kmax; ++k) {
+    T a[kRowsPerLane];
+    int c[kRowsPerLane];
+    bool ok[kRowsPerLane];
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      ok[j] = L.kb[j] + k < L.ke[j];
+      const int e = ok[j] ? L.kb[j] + k : L.kb[j];
+      a[j] = vals[e];
+      c[j] = A.ci[e] - r0;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowsPerLane; ++j) {
+      const T t = (T)v[ok[j] ? c[j] : 0];
+      if (ok[j]) q[j] += a[j] * t;
+    }
+  }
 }
 
 // r = b - A x over the lane's rows (x in LDS); returns sum (eq_i r_i)^2.
 template <class Team>
-__device__ __forceinline__ double team_resid(const BArgs &A, int r0, int n, int tl, const double *xs,
-                             double (&r)[kRowsPerLane], double *sh) {
+__device__ __forceinline__ double team_resid(const BArgs &A, const LaneRows<Team> &L, int r0, int n, int tl,
+                                             const double *xs, double (&r)[kRowsPerLane], double *sh) {
+  double ax[kRowsPerLane];
+  lane_product<double>(A, A.v64, L, r0, xs, ax);
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < kRowsPerLane; ++j) {
     const int i = tl + j * Team::size;
     r[j] = 0.0;
     if (i < n) {
-      const int row = r0 + i;
-      r[j] = A.b[row] - row_product<double>(A, A.v64, row, r0, xs);
-      const double e = A.eq[row] * r[j];
+      r[j] = A.b[r0 + i] - ax[j];
+      const double e = L.eq[j] * r[j];
       acc += e * e;
     }
   }
@@ -120,16 +171,19 @@ __device__ __forceinline__ double team_resid(const BArgs &A, int r0, int n, int 
 // lane's FP32 correction for FP32), p in LDS, r/z/q in registers.  Returns
 // the termination; iters counts across calls.
 template <class Team, typename T, typename Acc>
-__device__ __forceinline__ int team_cg(const BArgs &A, int r0, int n, int tl, Acc &acc, T *ps, T (&r)[kRowsPerLane],
-                       const T *vals, const T *jd, double thr2, int &iters, double *sh) {
-  T z[kRowsPerLane], q[kRowsPerLane];
+__device__ __forceinline__ int team_cg(const BArgs &A, const LaneRows<Team> &L, int r0, int n, int tl, Acc &acc,
+                                       T *ps, T (&r)[kRowsPerLane], const T *vals, const T *jdv, double thr2,
+                                       int &iters, double *sh) {
+  T z[kRowsPerLane], q[kRowsPerLane], jd[kRowsPerLane];
   double rz = 0.0;
 #pragma unroll
   for (int j = 0; j < kRowsPerLane; ++j) {
     const int i = tl + j * Team::size;
     z[j] = T(0);
+    jd[j] = T(0);
     if (i < n) {
-      z[j] = jd[r0 + i] * r[j];
+      jd[j] = jdv[r0 + i];
+      z[j] = jd[j] * r[j];
       ps[i] = z[j];
       rz += (double)r[j] * (double)z[j];
     }
@@ -138,14 +192,11 @@ __device__ __forceinline__ int team_cg(const BArgs &A, int r0, int n, int tl, Ac
   Team::sync();
   while (iters < A.max_it) {
     double pq = 0.0;
+    lane_product<T>(A, vals, L, r0, ps, q);
 #pragma unroll
     for (int j = 0; j < kRowsPerLane; ++j) {
       const int i = tl + j * Team::size;
-      q[j] = T(0);
-      if (i < n) {
-        q[j] = row_product<T>(A, vals, r0 + i, r0, ps);
-        pq += (double)ps[i] * (double)q[j];
-      }
+      if (i < n) pq += (double)ps[i] * (double)q[j];
     }
     pq = Team::sum(pq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) return CFDLA_TERM_NONPOSITIVE_CURVATURE;
@@ -155,11 +206,10 @@ __device__ __forceinline__ int team_cg(const BArgs &A, int r0, int n, int tl, Ac
     for (int j = 0; j < kRowsPerLane; ++j) {
       const int i = tl + j * Team::size;
       if (i < n) {
-        const int row = r0 + i;
         acc.add(j, i, (T)alpha * ps[i]);
         r[j] = r[j] - (T)alpha * q[j];
-        z[j] = jd[row] * r[j];
-        const double e = A.eq[row] * (double)r[j];
+        z[j] = jd[j] * r[j];
+        const double e = L.eq[j] * (double)r[j];
         rr += e * e;
         rzn += (double)r[j] * (double)z[j];
       }
@@ -200,21 +250,26 @@ struct RegAcc {
 // above twice the recursive residual restarts the direction and widens delta
 // tenfold (up to 0.5): the cycle outran FP32 precision.
 template <class Team>
-__device__ __forceinline__ int team_cg_reliable(const BArgs &A, int r0, int n, int tl, double *xs, float *ps,
-                                double (&r64)[kRowsPerLane], double rr_true, double thr2, int &iters, int &nrep,
-                                double &rr_out, double *sh) {
+__device__ __forceinline__ int team_cg_reliable(const BArgs &A, const LaneRows<Team> &L, int r0, int n, int tl,
+                                                double *xs, float *ps, double (&r64)[kRowsPerLane], double rr_true,
+                                                double thr2, int &iters, int &nrep, double &rr_out, double *sh) {
   const double sc = 1.0 / sqrt(rr_true);
   const double sc2 = sc * sc, thr2s = thr2 * sc2;
   double d2 = A.delta * A.delta;
-  const float *vals = A.v32, *jd = A.jd32;
-  float r[kRowsPerLane], z[kRowsPerLane], q[kRowsPerLane], d[kRowsPerLane];
+  const float *vals = A.v32;
+  float r[kRowsPerLane], z[kRowsPerLane], q[kRowsPerLane], d[kRowsPerLane], jd[kRowsPerLane];
+#pragma unroll
+  for (int j = 0; j < kRowsPerLane; ++j) {
+    const int i = tl + j * Team::size;
+    jd[j] = i < n ? A.jd32[r0 + i] : 0.0f;
+  }
   auto load_residual = [&]() {  // r32 = sc * r64, z = D^-1 r32; returns r.z
     double rz = 0.0;
 #pragma unroll
     for (int j = 0; j < kRowsPerLane; ++j) {
       const int i = tl + j * Team::size;
       r[j] = (float)(r64[j] * sc);
-      z[j] = i < n ? jd[r0 + i] * r[j] : 0.0f;
+      z[j] = i < n ? jd[j] * r[j] : 0.0f;
       rz += (double)r[j] * (double)z[j];
     }
     return Team::sum(rz, sh);
@@ -227,7 +282,7 @@ __device__ __forceinline__ int team_cg_reliable(const BArgs &A, int r0, int n, i
       d[j] = 0.0f;
     }
     Team::sync();
-    rr_out = team_resid<Team>(A, r0, n, tl, xs, r64, sh);
+    rr_out = team_resid<Team>(A, L, r0, n, tl, xs, r64, sh);
     ++nrep;
   };
 #pragma unroll
@@ -243,14 +298,11 @@ __device__ __forceinline__ int team_cg_reliable(const BArgs &A, int r0, int n, i
   bool restart = false;
   while (iters < A.max_it) {
     double pq = 0.0;
+    lane_product<float>(A, vals, L, r0, ps, q);
 #pragma unroll
     for (int j = 0; j < kRowsPerLane; ++j) {
       const int i = tl + j * Team::size;
-      q[j] = 0.0f;
-      if (i < n) {
-        q[j] = row_product<float>(A, vals, r0 + i, r0, ps);
-        pq += (double)ps[i] * (double)q[j];
-      }
+      if (i < n) pq += (double)ps[i] * (double)q[j];
     }
     pq = Team::sum(pq, sh);
     if (!(pq > 0.0) || !isfinite(pq)) {
@@ -263,11 +315,10 @@ __device__ __forceinline__ int team_cg_reliable(const BArgs &A, int r0, int n, i
     for (int j = 0; j < kRowsPerLane; ++j) {
       const int i = tl + j * Team::size;
       if (i < n) {
-        const int row = r0 + i;
         d[j] += (float)alpha * ps[i];
         r[j] = r[j] - (float)alpha * q[j];
-        z[j] = jd[row] * r[j];
-        const double e = A.eq[row] * (double)r[j];
+        z[j] = jd[j] * r[j];
+        const double e = L.eq[j] * (double)r[j];
         rr += e * e;
         rzn += (double)r[j] * (double)z[j];
       }
@@ -317,8 +368,10 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
   const double thr = fmax(A.tol_abs, A.tol_rel * sqrt(bb));
   const double thr2 = thr * thr;
   Team::sync();
+  LaneRows<Team> L;
+  L.load(A, r0, n, tl);
   double r[kRowsPerLane];
-  double rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);
+  double rr = team_resid<Team>(A, L, r0, n, tl, xs, r, sh);
   const double rn0 = sqrt(rr);
   int iters = 0, outer = 0, term;
   if (rr <= thr2) {
@@ -326,9 +379,9 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
   } else if (!A.refined) {
     LdsAcc acc{xs};
     while (true) {
-      term = team_cg<Team, double>(A, r0, n, tl, acc, psd, r, A.v64, A.jd64, thr2, iters, sh);
+      term = team_cg<Team, double>(A, L, r0, n, tl, acc, psd, r, A.v64, A.jd64, thr2, iters, sh);
       Team::sync();
-      rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);  // explicit FP64 residual
+      rr = team_resid<Team>(A, L, r0, n, tl, xs, r, sh);  // explicit FP64 residual
       if (rr <= thr2) {
         term = CFDLA_TERM_CONVERGED;
         break;
@@ -341,7 +394,7 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
       // the explicit residual now in r restarts the recurrence
     }
   } else if (A.reliable) {
-    term = team_cg_reliable<Team>(A, r0, n, tl, xs, reinterpret_cast<float *>(psd), r, rr, thr2, iters, outer, rr,
+    term = team_cg_reliable<Team>(A, L, r0, n, tl, xs, reinterpret_cast<float *>(psd), r, rr, thr2, iters, outer, rr,
                                   sh);
   } else {
     float *ps = reinterpret_cast<float *>(psd);
@@ -357,7 +410,7 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
         r32[j] = (float)(r[j] * sc);
         d.d[j] = 0.0f;
       }
-      const int t = team_cg<Team, float>(A, r0, n, tl, d, ps, r32, A.v32, A.jd32,
+      const int t = team_cg<Team, float>(A, L, r0, n, tl, d, ps, r32, A.v32, A.jd32,
                                           A.inner_tol * A.inner_tol, iters, sh);
 #pragma unroll
       for (int j = 0; j < kRowsPerLane; ++j) {
@@ -365,7 +418,7 @@ __device__ __forceinline__ void solve_system(const BArgs &A, int sys, int tl, do
         if (i < n) xs[i] += rn * (double)d.d[j];
       }
       Team::sync();
-      rr = team_resid<Team>(A, r0, n, tl, xs, r, sh);
+      rr = team_resid<Team>(A, L, r0, n, tl, xs, r, sh);
       const double rn2 = sqrt(rr);
       if (rr <= thr2) {
         term = CFDLA_TERM_CONVERGED;
