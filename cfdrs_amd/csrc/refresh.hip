@@ -354,8 +354,9 @@ __global__ __launch_bounds__(kBlock) void k_gj_step(int n, int c, const double *
   }
 }
 
-// Gauss-Jordan steps c0 .. c0 + B - 1 on [A | I] in one launch, for
-// n <= kGjMaxRows.  Every block loads the B panel columns into LDS and
+// Gauss-Jordan steps c0 .. c0 + B - 1 on [A | I] (column-major, entry
+// (r, j) at j n + r) in one launch, for n <= kGjMaxRows.  Every block loads
+// the B panel columns (contiguous) into LDS and
 // replays the B steps on them -- pivot row, swap, scale by d = 1 / m_pc,
 // eliminate with f = m_rc -- with the operations of k_gj_step (and of the
 // host), applying each step to its own kGjCols columns held in registers
@@ -370,7 +371,7 @@ constexpr int kGjPanelLds = 7680;  // doubles of the LDS panel
 __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const double *src, double *dst,
                                                      int *bad) {
 #pragma clang fp contract(off)
-  extern __shared__ double pnl[];  // n x B, row-major
+  extern __shared__ double pnl[];  // n x B, column-major: entry (r, k) at k n + r
   __shared__ double u[kGjCols];    // row c of this block's columns, scaled
   __shared__ double sw[2][kGjCols];
   __shared__ int s_p;
@@ -379,26 +380,23 @@ __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const
   const int tid = threadIdx.x;
   const int64_t w = 2 * (int64_t)n;
   const int64_t j0 = c0 + B + (int64_t)blockIdx.x * kGjCols;
-  for (int e = tid; e < n * B; e += kBlock) {
-    const int r = e / B, k = e - r * B;
-    pnl[e] = src[(int64_t)r * w + c0 + k];
-  }
+  for (int e = tid; e < n * B; e += kBlock) pnl[e] = src[(int64_t)c0 * n + e];
   double x[RQ][kGjCols];
 #pragma unroll
   for (int q = 0; q < RQ; ++q)
 #pragma unroll
     for (int jj = 0; jj < kGjCols; ++jj) {
       const int r = tid + q * kBlock;
-      x[q][jj] = r < n && j0 + jj < w ? src[(int64_t)r * w + j0 + jj] : 0.0;
+      x[q][jj] = r < n && j0 + jj < w ? src[(j0 + jj) * n + r] : 0.0;
     }
   __syncthreads();
   for (int k = 0; k < B; ++k) {
     const int c = c0 + k;
     Piv mine{-1.0, INT_MAX};
-    for (int r = c + tid; r < n; r += kBlock) mine = piv_better(mine, piv_of(pnl[r * B + k], r));
+    for (int r = c + tid; r < n; r += kBlock) mine = piv_better(mine, piv_of(pnl[k * n + r], r));
     const Piv best = piv_block(mine);
     if (tid == 0) {
-      const double mcc = pnl[c * B + k];
+      const double mcc = pnl[k * n + c];
       s_p = (mcc != mcc || !(best.val > 0.0 && best.val <= 1.7976931348623157e308)) ? -1 : best.row;
     }
     __syncthreads();
@@ -407,13 +405,13 @@ __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const
       if (tid == 0) *bad = 1;
       return;
     }
-    const double d = 1.0 / pnl[p * B + k];
+    const double d = 1.0 / pnl[k * n + p];
     if (p != c) {
       // rows c and p trade places: the panel, then this block's columns
       if (tid < B) {
-        const double a = pnl[c * B + tid], b = pnl[p * B + tid];
-        pnl[c * B + tid] = b;
-        pnl[p * B + tid] = a;
+        const double a = pnl[tid * n + c], b = pnl[tid * n + p];
+        pnl[tid * n + c] = b;
+        pnl[tid * n + p] = a;
       }
 #pragma unroll
       for (int q = 0; q < RQ; ++q)
@@ -435,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const
     }
     __syncthreads();
     // row c scaled by d
-    if (tid >= k && tid < B) pnl[c * B + tid] = pnl[c * B + tid] * d;
+    if (tid >= k && tid < B) pnl[tid * n + c] = pnl[tid * n + c] * d;
 #pragma unroll
     for (int q = 0; q < RQ; ++q)
       if (tid + q * kBlock == c)
@@ -450,11 +448,11 @@ __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const
     for (int q = 0; q < RQ; ++q) {
       const int r = tid + q * kBlock;
       if (r >= n || r == c) continue;
-      const double f = pnl[r * B + k];
+      const double f = pnl[k * n + r];
       if (f == 0.0) continue;
       for (int kk = k; kk < B; ++kk) {
-        const double t = f * pnl[c * B + kk];
-        pnl[r * B + kk] = pnl[r * B + kk] - t;
+        const double t = f * pnl[kk * n + c];
+        pnl[kk * n + r] = pnl[kk * n + r] - t;
       }
 #pragma unroll
       for (int jj = 0; jj < kGjCols; ++jj) {
@@ -469,8 +467,33 @@ __global__ __launch_bounds__(kBlock) void k_gj_panel(int n, int c0, int B, const
 #pragma unroll
     for (int jj = 0; jj < kGjCols; ++jj) {
       const int r = tid + q * kBlock;
-      if (r < n && j0 + jj < w) dst[(int64_t)r * w + j0 + jj] = x[q][jj];
+      if (r < n && j0 + jj < w) dst[(j0 + jj) * n + r] = x[q][jj];
     }
+}
+
+// [A | I] column-major (entry (r, j) at j n + r), for the panel path: the
+// panel and each block's columns are then contiguous.
+__global__ __launch_bounds__(kBlock) void k_dense_fill_cm(int n, double *m) {
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < 2 * (int64_t)n * n;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int64_t j = e / n, r = e - j * n;
+    m[e] = j == n + r ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_scatter_cm(int n, const int32_t *__restrict__ rp,
+                                                             const int32_t *__restrict__ ci,
+                                                             const double *__restrict__ v, double *m) {
+  for (int r = blockIdx.x; r < n; r += gridDim.x)
+    for (int k = rp[r] + (int)threadIdx.x; k < rp[r + 1]; k += kBlock)
+      m[(int64_t)ci[k] * n + r] = m[(int64_t)ci[k] * n + r] + v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gj_extract_cm(int n, const double *m, double *inv) {
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t r = e / n, c = e - r * n;
+    inv[e] = m[(n + c) * (int64_t)n + r];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_gj_extract(int n, const double *m, double *inv) {
@@ -646,11 +669,10 @@ void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const dou
     pr[k].alloc((size_t)nb);
   }
   int *bad = flag_reset();
-  hipLaunchKernelGGL(k_dense_fill, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
-  hipLaunchKernelGGL(k_dense_scatter, dim3(std::min(n, kMaxBlocks)), dim3(kBlock), 0, stream(), n, rp, ci, v,
-                     buf[0].p);
-  int last = 0;  // the buffer holding the result
   if (n <= kGjMaxRows) {
+    hipLaunchKernelGGL(k_dense_fill_cm, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
+    hipLaunchKernelGGL(k_dense_scatter_cm, dim3(std::min(n, kMaxBlocks)), dim3(kBlock), 0, stream(), n, rp, ci, v,
+                       buf[0].p);
     // B steps per launch (641 launches become 81 for a 641-row level)
     const int B = std::min(kGjPanel, kGjPanelLds / n);
     int it = 0;
@@ -661,16 +683,19 @@ void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const dou
       hipLaunchKernelGGL(k_gj_panel, dim3(blocks), dim3(kBlock), (size_t)n * b * sizeof(double), stream(), n, c0,
                          b, buf[it & 1].p, buf[(it + 1) & 1].p, bad);
     }
-    last = it & 1;
+    hipLaunchKernelGGL(k_gj_extract_cm, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n,
+                       buf[it & 1].p, inv);
   } else {
+    hipLaunchKernelGGL(k_dense_fill, dim3(grid_for((int64_t)n * 2 * n)), dim3(kBlock), 0, stream(), n, buf[0].p);
+    hipLaunchKernelGGL(k_dense_scatter, dim3(std::min(n, kMaxBlocks)), dim3(kBlock), 0, stream(), n, rp, ci, v,
+                       buf[0].p);
     hipLaunchKernelGGL(k_gj_init, dim3(nb), dim3(kBlock), 0, stream(), n, buf[0].p, pv[0].p, pr[0].p);
     for (int c = 0; c < n; ++c)
       hipLaunchKernelGGL(k_gj_step, dim3(nb), dim3(kBlock), 0, stream(), n, c, buf[c & 1].p, buf[(c + 1) & 1].p,
                          pv[c & 1].p, pr[c & 1].p, pv[(c + 1) & 1].p, pr[(c + 1) & 1].p, nb, bad);
-    last = n & 1;
+    hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n, buf[n & 1].p,
+                       inv);
   }
-  hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((int64_t)n * n)), dim3(kBlock), 0, stream(), n, buf[last].p,
-                     inv);
   CFDLA_REQUIRE(flag_read() == 0, CFDLA_ERR_NUMERICAL,
                 "AMG coarsest operator is singular; use coarse_solver = reference sweeps");
 }
