@@ -70,6 +70,49 @@ def test_batch_matches_dense(small, precision, reliable, wave_max):
     assert np.allclose(rep_res, res, rtol=1e-6, atol=1e-14)
 
 
+def _dense_plus_synth(nets):
+    # the synthetic networks and one 300-node network whose first 160 nodes
+    # are all connected (25,000+ entries, rows of up to 160)
+    nd = 300
+    ei = [(a, b) for a in range(160) for b in range(a + 1, 160)] + [(a, a + 1) for a in range(159, nd - 1)]
+    e_i = np.concatenate([nets.edge_i, [a for a, _ in ei]])
+    e_j = np.concatenate([nets.edge_j, [b for _, b in ei]])
+    no = np.concatenate([nets.node_offset, [nets.node_offset[-1] + nd]])
+    eo = np.concatenate([nets.edge_offset, [nets.edge_offset[-1] + len(ei)]])
+    g = np.concatenate([NW.rect_conductance(nets.width, NW.CHANNEL_HEIGHT, nets.length)[0],
+                        1e-12 * (1.0 + np.random.default_rng(7).random(len(ei)))])
+    is_dir = np.zeros(no[-1], np.uint8)
+    dval = np.zeros(no[-1])
+    is_dir[no[:-1]] = 1
+    dval[no[:-1]] = NW.INLET_PRESSURE
+    is_dir[no[1:] - 1] = 1
+    return NW.NetworkBatch(no, eo, e_i, e_j, g, is_dir, dval)
+
+
+@pytest.mark.parametrize("precision,reliable", [(ca.Precision.FP64, 0), (ca.Precision.FP32_REFINED, 0),
+                                                (ca.Precision.FP32_REFINED, -1)],
+                         ids=["fp64", "fp32-reliable", "fp32-restarted"])
+def test_schedule_does_not_change_results(small, precision, reliable):
+    # the persistent grid's size only changes which block picks up which
+    # system: every team's arithmetic is the same, so the solutions,
+    # iteration counts and reported norms are bit-identical between one and
+    # two blocks per CU, with a dense system (long rows) in the batch
+    nets, _ = small
+    batch = _dense_plus_synth(nets)
+    cfg = (ca.IterativeSolverConfig.new(1e-10).with_max_iterations(20000).with_precision(precision)
+           .with_reliable_interval(reliable))
+    out = {}
+    for bpc in (1, 2):
+        batch.schedule = {"wave_max_rows": 0, "blocks_per_cu": bpc}
+        out[bpc] = batch.solve(None, cfg)
+    (x1, r1), (x2, r2) = out[1], out[2]
+    assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
+    assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert [r.final_residual_norm for r in r1] == [r.final_residual_norm for r in r2]
+    assert all(r.termination == ca.Termination.Converged for r in r1)
+    assert eq_residuals(batch, x1).max() <= 1e-10
+
+
 def test_batch_reliable_updates_keep_fp64_iteration_counts(small):
     # FP32 recurrence with FP64 residual replacement and adaptive delta: the
     # iteration counts stay near the FP64 ones (restarted refinement roughly
