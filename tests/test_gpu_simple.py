@@ -185,6 +185,34 @@ def test_amg_recompute_device_matches_host(case):
         assert torch.equal(zd32, zh32)
 
 
+@pytest.mark.parametrize("nn,min_coarse", [(20, 1100), (22, 1000)])
+def test_dense_coarse_inverse_device_matches_host(nn, min_coarse):
+    # the device Gauss-Jordan of the coarsest level: eight columns per launch
+    # for levels of at most 1,024 rows (22^3: 10,648 -> 1,445 -> 106 rows),
+    # one column per launch beyond (20^3 with min_coarse 1,100 stops at
+    # 8,000 -> 1,095 rows);
+    # both replay the host's operations, so V-cycles after a device and a
+    # host recompute agree bit for bit
+    torch = _torch()
+    n = nn ** 3
+    rp, ci, v, _ = P.varcoef(nn, nn, nn, float(nn * nn), P.config2_k(nn), 1 << 4)
+    _, _, v2, _ = P.varcoef(nn, nn, nn, float(nn * nn), np.exp(P.uniform(64, n)), 1 << 4)
+    cfg = ca.AMGConfig.smoothed_aggregation()
+    cfg.min_coarse_size = min_coarse
+    a_d, a_h, m_d, m_h = _amg_pair(rp, ci, v, n, cfg)
+    coarse = m_d.statistics.level_sizes[-1]
+    assert (coarse > 1024) == (min_coarse > 1024), coarse
+    a_d.update_values_device(torch.tensor(v2, device="cuda"))
+    m_d.recompute_device(a_d)
+    a_h.update_values(v2)
+    m_h.recompute(a_h)
+    r = P.uniform(73, n)
+    z_d, z_h = np.empty(n), np.empty(n)
+    m_d.apply_to(r, z_d)
+    m_h.apply_to(r, z_h)
+    assert np.array_equal(_bits(z_d), _bits(z_h))
+
+
 def test_corrections_match_oracle():
     torch = _torch()
     nx, ny, dx, dy, relax = 70, 50, 1 / 69, 1 / 49, 0.3
