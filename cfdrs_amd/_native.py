@@ -90,6 +90,8 @@ SIGNATURES = {
     "cfdla_matrix_info": (C.c_int, [_VP, _I64P, _I64P, _I64P]),
     "cfdla_matrix_storage": (C.c_int, [_VP, _I32P, _I64P, _I64P]),
     "cfdla_matrix_set_storage": (C.c_int, [_VP, C.c_int32]),
+    "cfdla_matrix_set_gather": (C.c_int, [_VP, C.c_int32]),
+    "cfdla_matrix_gather": (C.c_int, [_VP, _I32P, _I32P]),
     "cfdla_matrix_destroy": (C.c_int, [_VP]),
     "cfdla_spmv": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_spmv_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
@@ -100,6 +102,7 @@ SIGNATURES = {
     "cfdla_precond_apply": (C.c_int, [_VP, _VP, _VP]),
     "cfdla_precond_apply_device": (C.c_int, [_VP, C.c_int32, _VP, _VP]),
     "cfdla_amg_update_values": (C.c_int, [_VP, _VP]),
+    "cfdla_amg_set_gather": (C.c_int, [_VP, C.c_int32]),
     "cfdla_amg_update_values_device": (C.c_int, [_VP, _VP]),
     "cfdla_amg_stats": (C.c_int, [_VP, C.POINTER(AmgStatistics)]),
     "cfdla_amg_export_level": (C.c_int, [_VP, C.c_int32, C.c_int32, _I64P, _I64P, _I64P, _VP, _VP, _VP]),

@@ -137,12 +137,12 @@ template <typename T>
 constexpr int rb_cap() {
   return 2048;
 }
-// Rows per block: 512 in FP64, 256 in FP32 (measured on the 128^3 solves:
-// 256-row FP32 blocks 4.79-4.81 against 4.81-4.84 ms over three pairs of
-// runs, 256-row FP64 ones 6.73 against 6.69 ms).
+// Rows per block: kBlock = 256, so every row is summed by at least one whole
+// thread (two rows per thread in 512-row FP64 blocks ran y = A x on the
+// 128^3 7-point operator in 47.8 against 42.1 us).
 template <typename T>
 constexpr int rb_rows() {
-  return sizeof(T) == 8 ? 512 : 256;
+  return 256;
 }
 // Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
 // one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
@@ -171,6 +171,9 @@ struct DevCsr {
   // every block of that partition has columns spanning at most 65535: 2 bytes
   // per nonzero instead of 4.  Each partition decides on its own.
   bool c16_f64 = false, c16_f32 = false;
+  // row-block gather layout per value type: rows (LDS-staged row walk) or quads
+  bool walk64 = false, walk32 = false;
+  int gather_request = -1;  // CFDLA_GATHER_*
   DevBuf<uint16_t> c16_64, c16_32;
   DevBuf<int32_t> cbase64, cbase32;
   DevBuf<double> v64, sdinv64;
@@ -214,6 +217,7 @@ struct DevCsr {
   template <typename T> const int *rb() const { return sizeof(T) == 8 ? rb64.p : rb32.p; }
   template <typename T> int nrb() const { return sizeof(T) == 8 ? nrb64 : nrb32; }
   template <typename T> bool has_cols16() const { return sizeof(T) == 8 ? c16_f64 : c16_f32; }
+  template <typename T> bool rb_walk() const { return sizeof(T) == 8 ? walk64 : walk32; }
   template <typename T> const uint16_t *cols16() const { return sizeof(T) == 8 ? c16_64.p : c16_32.p; }
   template <typename T> const int *cbase() const { return sizeof(T) == 8 ? cbase64.p : cbase32.p; }
 };
@@ -230,6 +234,10 @@ template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32
 // span > 0: a block also closes before its columns would span more than
 // `span` (so 16-bit column offsets fit every block whose rows each do).
 void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span = 0);
+// True when the row walk's gathers cost less than the quad layout's for this
+// partition (value size vbytes): instructions with a live lane plus 0.04 per
+// distinct 128-byte line of x, over a sample of about 64 row blocks.
+bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes);
 
 // Device-resident scalars of one solve.  A reducing kernel writes block
 // partials; the single-block finalize kernel launched after it sums them into

@@ -235,17 +235,40 @@ __device__ __forceinline__ void load_quad_cols(const uint16_t *p, int base, int 
   const v4u16 a = load_vec<NT>(reinterpret_cast<const v4u16 *>(p));
   c[0] = base + (int)a.x; c[1] = base + (int)a.y; c[2] = base + (int)a.z; c[3] = base + (int)a.w;
 }
+// Four entries into LDS at a 4-aligned entry offset (16- or 32-byte stores).
+__device__ __forceinline__ void store_quad(int *p, const int (&c)[4]) {
+  *reinterpret_cast<v4i *>(p) = v4i{c[0], c[1], c[2], c[3]};
+}
+__device__ __forceinline__ void store_quad(float *p, const float (&v)[4]) {
+  *reinterpret_cast<v4f *>(p) = v4f{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
+  reinterpret_cast<v2d *>(p)[0] = v2d{v[0], v[1]};
+  reinterpret_cast<v2d *>(p)[1] = v2d{v[2], v[3]};
+}
 
-// The host packs rows in order into blocks of at most rb_rows rows and rb_cap
-// nonzeros.  A workgroup streams its block's nonzeros in quads from the
-// 4-aligned entry at or below its first one (up to rb_cap / (4 kBlock) = 2
-// quads per thread, the at most 3 entries past them by three lanes; all
-// issued before the x gathers), parks the products in LDS, then sums each
-// row from LDS with 1..64 threads (the largest power of two that fits the
-// block's row count) and a shuffle tree.  Row lengths of 4..700 nonzeros (AMG
-// transfer operators and Galerkin products) all keep every lane busy with
-// 8 loads in flight.  A row longer than rb_cap is a block of its own, summed
-// in steps of rb_cap entries straight from memory.
+// The host packs rows in order into blocks of at most rb_rows = kBlock rows
+// and rb_cap nonzeros.  A workgroup streams its block's columns and values in
+// quads from the 4-aligned entry at or below its first one (up to
+// rb_cap / (4 kBlock) = 2 quads per thread, the at most 3 entries past them by
+// three lanes), all loads in flight together.  Each row is then summed by tpr
+// lanes (the largest power of two up to 64 with tpr * rows <= kBlock), in one
+// of two gather layouts picked per operator at upload (rb_prefers_walk):
+//   * row walk (WALK): the quads are parked in LDS; after the barrier each
+//     row's lanes read their entries back (8 or 4 at a time, all LDS reads
+//     issued before the gathers) and gather x, so lanes of a wavefront hold
+//     consecutive rows (tpr = 1) or consecutive entries of one row and the
+//     gathers of a stencil-like operator fall on a few cache lines per
+//     wavefront (y = A x on the 128^3 7-point operator: 42.1 us FP64,
+//     24.3 us FP32);
+//   * quad: x is gathered for each quad as it arrives (4 consecutive
+//     entries per lane), products are parked in LDS and summed from there;
+//     fewer gather instructions where rows are short and uneven (the
+//     prolongations) or a few lanes share a long row (the first
+//     restriction), but the 7-point operator's gathers spread over 3 grid
+//     planes (47.3 and 29.5 us).
+// A row longer than rb_cap is a block of its own, summed in steps of rb_cap
+// entries straight from memory.
 //
 // Blocks are dealt to the 8 XCDs round-robin; XCD q walks the q-th
 // contiguous range of row blocks, so neighbouring rows share its L2.
@@ -257,7 +280,10 @@ __device__ __forceinline__ void load_quad_cols(const uint16_t *p, int base, int 
 // sweeps (377 against 437 us).
 //
 // C16: columns are 16-bit offsets from the block's base column (cb[q]).
-template <typename T, int EPI, bool DOT, bool C16, bool NT>
+// WALK: the row-walk layout (else the quad layout); separate instantiations,
+// so the quad layout keeps its smaller LDS footprint (8 blocks per CU in FP64
+// instead of 6) and its own register allocation.
+template <typename T, int EPI, bool DOT, bool C16, bool NT, bool WALK>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const uint16_t *__restrict__ c16,
                                                    const int *__restrict__ cb, const T *__restrict__ av,
@@ -268,8 +294,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   constexpr int CAP = rb_cap<T>();
   constexpr int Q = CAP / (4 * kBlock);  // quads per thread
   static_assert(Q * 4 * kBlock == CAP, "the row-block cap is a whole number of quads per thread");
-  constexpr int RPT = rb_rows<T>() / kBlock;  // rows per thread when a block has more rows than threads
-  __shared__ T prod[CAP + 4];
+  static_assert(rb_rows<T>() == kBlock, "one row per thread at most");
+  __shared__ T sval[CAP + 4];  // the walk's values, or the quad layout's products
+  __shared__ int scol[WALK ? CAP + 4 : 1];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -325,14 +352,24 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
             }
           }
         }
+        // every gather of the step in flight before the sums
+        T xv[Q][4];
 #pragma unroll
         for (int u = 0; u < Q; ++u) {
           const int k = base + 4 * (tid + u * kBlock);
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const bool ok = k + j >= e0 && k + j < e1;
-            const T xv = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
-            acc += ok ? v[u][j] * xv : T(0);
+            xv[u][j] = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+          const int k = base + 4 * (tid + u * kBlock);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const bool ok = k + j >= e0 && k + j < e1;
+            acc += ok ? v[u][j] * xv[u][j] : T(0);
           }
         }
       }
@@ -348,105 +385,161 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
       }
       __syncthreads();
     } else {
-      // Reduction layout: with nr <= kBlock rows, tpr threads (a power of two
-      // up to 64) per row; otherwise up to RPT rows per thread.  Row bounds and
-      // epilogue operands are loaded first, then every quad of the block and
-      // the tail entries, then all gathers, so they are in flight together.
-      // Products sit in LDS at their offset from ea, the quad-aligned start.
+      // Row bounds and epilogue operands first, then every quad of the block
+      // and the tail entries into LDS; after the barrier each row's lanes
+      // walk its entries and gather x.
       const int ea = e0 & ~3, e1 = e0 + ne;
-      const bool wide = nr <= kBlock;
       int tpr = 1;
-      if (wide)
-        while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
+      while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
       const int lane = tid & (tpr - 1);
-      int ks[RPT], kr[RPT];
-      EpiIn<T> e[RPT];
+      const int rl = min(tid / tpr, nr - 1);
+      const bool own = tid / tpr < nr;
+      // both bounds loaded unconditionally (rl is clamped into the block), so
+      // the select waits for nothing before the streams are issued
+      const int ks = rp[r0 + rl], kr = rp[r0 + rl + 1];
+      const int ke = own ? kr : ks;
+      const EpiIn<T> e = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, omega, y, dotw);
+      // a column that is valid for any gather: entries outside the row or
+      // the block gather it and drop the product (selected after the
+      // multiply, so a stored zero times a non-finite x still gives NaN)
+      const int safe = C16 ? b0 : 0;
+      const T *sv = sval - ea;
+      T sum = T(0);
+      if constexpr (WALK) {
+        if (ne > 0) {
+          // every quad and the tail loaded first (addresses clamped into the
+          // block, so all loads are issued together), then stored to LDS
+          int c[Q][4];
+          T v[Q][4];
 #pragma unroll
-      for (int j = 0; j < RPT; ++j) {
-        const int rl = wide ? min(tid / tpr, nr - 1) : min(tid + j * kBlock, nr - 1);
-        if (j == 0 || !wide) {
-          ks[j] = rp[r0 + rl] - ea;
-          kr[j] = rp[r0 + rl + 1] - ea;
-          e[j] = epi_load<T, EPI, DOT>(r0 + rl, x, b, dinv, omega, y, dotw);
-        }
-      }
-      if (ne > 0) {
-        // a column that is valid for any gather: entries outside [e0, e1)
-        // (neighbouring blocks, the zero padding) gather it and are dropped
-        const int safe = C16 ? b0 : 0;
-        int c[Q][4];
-        T v[Q][4], xv[Q][4];
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
+            const int kc = k < e1 ? k : ea;
+            if constexpr (C16) load_quad_cols<NT>(c16 + kc, b0, c[u]);
+            else load_quad_cols<NT>(ci + kc, c[u]);
+            load_quad<NT>(av + kc, v[u]);
+          }
+          const int kt = ea + 4 * Q * kBlock + tid;  // entries past the quads (e1 - ea <= CAP + 3)
+          const bool tail = tid < 3 && kt < e1;
+          const int ktc = tail ? kt : ea;
+          const int ct = col(ktc);
+          const T vt = av[ktc];
 #pragma unroll
-        for (int u = 0; u < Q; ++u) {
-          const int k = ea + 4 * (tid + u * kBlock);
-          if (k < e1) {
-            if constexpr (C16) load_quad_cols<NT>(c16 + k, b0, c[u]);
-            else load_quad_cols<NT>(ci + k, c[u]);
-            load_quad<NT>(av + k, v[u]);
-          } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              c[u][j] = safe;
-              v[u][j] = T(0);
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
+            if (k < e1) {
+              store_quad(&scol[k - ea], c[u]);
+              store_quad(&sval[k - ea], v[u]);
             }
           }
-        }
-        const int kt = ea + 4 * Q * kBlock + tid;  // entries past the quads (e1 - ea <= CAP + 3)
-        const bool tail = tid < 3 && kt < e1;
-        const int ct = tail ? col(kt) : safe;
-        const T vt = tail ? av[kt] : T(0);
-#pragma unroll
-        for (int u = 0; u < Q; ++u) {
-          const int k = ea + 4 * (tid + u * kBlock);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const bool ok = k + j >= e0 && k + j < e1;
-            xv[u][j] = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
+          if (tail) {
+            scol[kt - ea] = ct;
+            sval[kt - ea] = vt;
           }
         }
-        const T xt = gather<T, EPI>(ct, x, b, dinv, omega);
+        __syncthreads();
+        const int *sc = scol - ea;
+        if (tpr == 1) {
+          // the row's entries in order, 8 gathers in flight
+          for (int k = ks; k < ke; k += 8) {
+            T vv[8], xv[8];
 #pragma unroll
-        for (int u = 0; u < Q; ++u) {
-          const int k = ea + 4 * (tid + u * kBlock);
-          if (k < e1) {
+            for (int u = 0; u < 8; ++u) {
+              // LDS index clamped to the row's first entry, so the reads
+              // are unconditional and issue back to back
+              const bool ok = k + u < ke;
+              const int kk = ok ? k + u : ks;
+              const int cc = sc[kk];
+              const T vk = sv[kk];
+              vv[u] = ok ? vk : T(0);
+              xv[u] = gather<T, EPI>(ok ? cc : safe, x, b, dinv, omega);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (k + u < ke) sum += vv[u] * xv[u];
+          }
+        } else {
+          // tpr lanes per row, four running sums per lane
+          T s4[4] = {T(0), T(0), T(0), T(0)};
+          for (int k = ks + lane; k < ke; k += 4 * tpr) {
+            T vv[4], xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const bool ok = k + u * tpr < ke;
+              const int kk = ok ? k + u * tpr : k;
+              const int cc = sc[kk];
+              const T vk = sv[kk];
+              vv[u] = ok ? vk : T(0);
+              xv[u] = gather<T, EPI>(ok ? cc : safe, x, b, dinv, omega);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (k + u * tpr < ke) s4[u] += vv[u] * xv[u];
+          }
+          sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        }
+      } else {
+        // gathers in the quad layout, products parked in LDS
+        if (ne > 0) {
+          int c[Q][4];
+          T v[Q][4], xv[Q][4];
+#pragma unroll
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
+            if (k < e1) {
+              if constexpr (C16) load_quad_cols<NT>(c16 + k, b0, c[u]);
+              else load_quad_cols<NT>(ci + k, c[u]);
+              load_quad<NT>(av + k, v[u]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                c[u][j] = safe;
+                v[u][j] = T(0);
+              }
+            }
+          }
+          const int kt = ea + 4 * Q * kBlock + tid;
+          const bool tail = tid < 3 && kt < e1;
+          const int ct = tail ? col(kt) : safe;
+          const T vt = tail ? av[kt] : T(0);
+#pragma unroll
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               const bool ok = k + j >= e0 && k + j < e1;
-              prod[k + j - ea] = ok ? v[u][j] * xv[u][j] : T(0);
+              xv[u][j] = gather<T, EPI>(ok ? c[u][j] : safe, x, b, dinv, omega);
             }
           }
+          const T xt = gather<T, EPI>(ct, x, b, dinv, omega);
+#pragma unroll
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
+            if (k < e1) {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const bool ok = k + j >= e0 && k + j < e1;
+                sval[k + j - ea] = ok ? v[u][j] * xv[u][j] : T(0);
+              }
+            }
+          }
+          if (tail) sval[kt - ea] = vt * xt;
         }
-        if (tail) prod[kt - ea] = vt * xt;
-      }
-      __syncthreads();
-      if (wide) {
-        // four running sums per thread, so four LDS reads are in flight
-        // (one sum waited for every read: on blocks of a few long rows,
-        // 32 dependent reads per thread)
-        const bool own = tid / tpr < nr;
-        const int ke = own ? kr[0] : ks[0];
+        __syncthreads();
+        // four running sums per lane, so four LDS reads are in flight
         T s4[4] = {T(0), T(0), T(0), T(0)};
-        int k = ks[0] + lane;
+        int k = ks + lane;
         for (; k + 3 * tpr < ke; k += 4 * tpr) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) s4[u] += prod[k + u * tpr];
+          for (int u = 0; u < 4; ++u) s4[u] += sv[k + u * tpr];
         }
 #pragma unroll
         for (int u = 0; u < 3; ++u)
-          if (k + u * tpr < ke) s4[u] += prod[k + u * tpr];
-        T sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-        if (own && lane == 0) epi_store<T, EPI, DOT>(e[0], r0 + tid / tpr, sum, omega, y, dotw, dacc, y2);
-      } else {
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-          if (tid + j * kBlock < nr) {
-            T sum = T(0);
-            for (int k = ks[j]; k < kr[j]; ++k) sum += prod[k];
-            epi_store<T, EPI, DOT>(e[j], r0 + tid + j * kBlock, sum, omega, y, dotw, dacc, y2);
-          }
-        }
+          if (k + u * tpr < ke) s4[u] += sv[k + u * tpr];
+        sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
       }
+      for (int o = tpr >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (own && lane == 0) epi_store<T, EPI, DOT>(e, r0 + tid / tpr, sum, omega, y, dotw, dacc, y2);
       __syncthreads();
     }
     d0 = n0;
@@ -470,11 +563,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
 #define CFDLA_RB_NT_MIN_BYTES (32.0 * 1024.0 * 1024.0)
 #endif
 
-template <typename T, int EPI, bool DOT, bool C16, bool NT>
+template <typename T, int EPI, bool DOT, bool C16, bool NT, bool WALK>
 void launch_csr_rb_k(const DevCsr &A, int blocks, const T *x, const T *b, const T *dinv, double omega, T *y,
                      const T *dotw, Scal *s, bool check_done, RedWs &ws, T *y2) {
   const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
-  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT, WALK>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
                      A.rp.p, C16 ? nullptr : A.ci.p, C16 ? A.cols16<T>() : nullptr, C16 ? A.cbase<T>() : nullptr,
                      A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
 }
@@ -486,13 +579,17 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
   const bool c16 = A.has_cols16<T>();
   const bool nt = sizeof(T) == 4 || (double)A.nnz * (double)(sizeof(T) + (c16 ? 2 : 4)) >= CFDLA_RB_NT_MIN_BYTES;
+  const bool walk = A.rb_walk<T>();
+#define CFDLA_RB(C, N, W) \
+  launch_csr_rb_k<T, EPI, DOT, C, N, W>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2)
   if (c16) {
-    if (nt) launch_csr_rb_k<T, EPI, DOT, true, true>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
-    else launch_csr_rb_k<T, EPI, DOT, true, false>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+    if (nt) walk ? CFDLA_RB(true, true, true) : CFDLA_RB(true, true, false);
+    else walk ? CFDLA_RB(true, false, true) : CFDLA_RB(true, false, false);
   } else {
-    if (nt) launch_csr_rb_k<T, EPI, DOT, false, true>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
-    else launch_csr_rb_k<T, EPI, DOT, false, false>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2);
+    if (nt) walk ? CFDLA_RB(false, true, true) : CFDLA_RB(false, true, false);
+    else walk ? CFDLA_RB(false, false, true) : CFDLA_RB(false, false, false);
   }
+#undef CFDLA_RB
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }
 

@@ -179,6 +179,70 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
   }
 }
 
+bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes) {
+  // The two layouts of k_csr_rb's regular path, wavefront by wavefront:
+  //   quad: gather instruction (u, j) of wave w covers entry
+  //         ea + 4 (64 w + lane + 256 u) + j of the block;
+  //   rows: lane t of wave w walks row t / tpr; its p-th gather is entry
+  //         ks + p (tpr = 1) or ks + t % tpr + p tpr (tpr > 1).
+  // An instruction counts when some lane has a live entry; its lines are the
+  // distinct 128-byte lines of x those lanes read.  Measured on the 128^3
+  // hierarchy (FP32 solve, per launch), 0.04 per line orders every operator:
+  // the 7-point operator, R1 and the level-2 operator faster in rows, R0 and
+  // every prolongation faster in quads.
+  constexpr double kLineCost = 0.04;
+  const int64_t nrb = (int64_t)rb.size() / 2 - 1;
+  if (nrb <= 0) return false;
+  const int64_t step = std::max<int64_t>(1, nrb / 64);
+  const int cap = rb_cap<double>();
+  double iq = 0.0, lq = 0.0, iw = 0.0, lw = 0.0;
+  std::vector<int64_t> lines;
+  lines.reserve(64);
+  auto count = [&](double &instr, double &nl) {
+    if (lines.empty()) return;
+    std::sort(lines.begin(), lines.end());
+    instr += 1.0;
+    nl += (double)(std::unique(lines.begin(), lines.end()) - lines.begin());
+    lines.clear();
+  };
+  auto line = [&](int64_t k) { return ((int64_t)h.ci[k] * vbytes) >> 7; };
+  for (int64_t q = 0; q < nrb; q += step) {
+    const int64_t r0 = rb[2 * q], e0 = rb[2 * q + 1], r1 = rb[2 * q + 2], e1 = rb[2 * q + 3];
+    const int64_t nr = r1 - r0, ea = e0 & ~int64_t(3);
+    if (e1 - e0 > cap) continue;  // a long row: the kernel's other path
+    for (int w = 0; w < kBlock / 64; ++w)
+      for (int u = 0; u < cap / (4 * kBlock); ++u)
+        for (int j = 0; j < 4; ++j) {
+          for (int l = 0; l < 64; ++l) {
+            const int64_t k = ea + 4 * (64 * w + l + kBlock * u) + j;
+            if (k >= e0 && k < e1) lines.push_back(line(k));
+          }
+          count(iq, lq);
+        }
+    int64_t tpr = 1;
+    while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      int64_t ks[64], len[64], maxp = 0;
+      for (int l = 0; l < 64; ++l) {
+        const int64_t t = 64 * w + l, row = t / tpr, sub = t % tpr;
+        ks[l] = 0;
+        len[l] = 0;
+        if (row >= nr) continue;
+        const int64_t a = h.rp[r0 + row], b = h.rp[r0 + row + 1];
+        ks[l] = a + sub;
+        len[l] = b - a > sub ? (b - a - sub + tpr - 1) / tpr : 0;
+        maxp = std::max(maxp, len[l]);
+      }
+      for (int64_t p = 0; p < maxp; ++p) {
+        for (int l = 0; l < 64; ++l)
+          if (p < len[l]) lines.push_back(line(ks[l] + p * tpr));
+        count(iw, lw);
+      }
+    }
+  }
+  return iw + kLineCost * lw < iq + kLineCost * lq;
+}
+
 namespace {
 // 16-bit column offsets for a row-block partition: base[q] = the smallest
 // column of block q, c16[k] = ci[k] - base[q].  False when some block has
@@ -369,6 +433,7 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   build_row_blocks(h, rb_cap<double>(), rb_rows<double>(), blocks, span);
   nrb64 = (int)blocks.size() / 2 - 1;
   rb64.upload(blocks.data(), blocks.size());
+  const std::vector<int32_t> blocks64 = blocks;
   const bool allow16 = span > 0;
   c16_f64 = allow16 && build_cols16(h, blocks, c16v, cb);
   if (c16_f64) {
@@ -453,6 +518,14 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
       sync_stream();
     }
   }
+  // gather layout of the row-block product (blocks holds the FP32 partition
+  // when there is one); evaluated only for matrices stored as row blocks
+  auto pick = [&](const std::vector<int32_t> &part, int vbytes) {
+    if (gather_request != CFDLA_GATHER_AUTO) return gather_request == CFDLA_GATHER_ROWS;
+    return !dia && !sell && rb_prefers_walk(h, part, vbytes);
+  };
+  walk64 = pick(blocks64, 8);
+  walk32 = want_fp32 && pick(blocks, 4);
   sync_stream();
 }
 
@@ -733,6 +806,7 @@ void AmgPrecond::upload() {
   for (size_t l = 0; l < L; ++l) {
     HostLevel &h = host.levels[l];
     DevLevel &d = lv[l];
+    d.A.gather_request = d.P.gather_request = d.R.gather_request = gather_request;
     d.A.upload(h.A, true, true);
     if (l + 1 < L) {
       d.P.upload(h.P, true, false);
@@ -837,6 +911,7 @@ bool AmgPrecond::build_fused(size_t l) {
   F.v = S.v;
   F.v.insert(F.v.end(), f.M.v.begin(), f.M.v.end());
   DevLevel &dl = lv[l];
+  dl.F.gather_request = gather_request;
   dl.F.upload(F, true, false, false);
   dl.fx64.alloc((size_t)F.n);
   dl.fx32.alloc((size_t)F.n);

@@ -141,6 +141,7 @@ struct AmgPrecond : Precond {
   int coarse_n = 0;
   double setup_time = 0.0;
   bool on_device = false;
+  int gather_request = -1;  // CFDLA_GATHER_* for every level's row-block products
   bool host_stale = false;  // a device refresh left newer level values than host
   std::vector<GalerkinPlan> gplan;  // per level, made on the first device refresh
   // The level before a dense coarsest one runs its two Jacobi pre-sweeps

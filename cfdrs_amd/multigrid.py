@@ -234,6 +234,15 @@ class AlgebraicMultigrid(_Preconditioner):
             raise N.CfdlaError(N.ERR_INVALID_ARG, "recompute_device needs the matrix the hierarchy was built from")
         N.call("cfdla_amg_update_values_device", self.handle, fine_matrix.handle)
 
+    def set_gather(self, layout: str) -> None:
+        """Gather layout of every level's row-block CSR products: "auto"
+        (chosen per operator), "quad" or "rows" (see CsrMatrix.set_gather)."""
+        from .sparse import CsrMatrix as _Csr
+
+        if layout not in _Csr.GATHER_LAYOUTS:
+            raise ValueError(f"unknown gather layout {layout!r}")
+        N.call("cfdla_amg_set_gather", self.handle, _Csr.GATHER_LAYOUTS[layout])
+
     @property
     def statistics(self) -> AMGStatistics:
         s = N.AmgStatistics()

@@ -199,6 +199,27 @@ class CsrMatrix:
         N.call("cfdla_matrix_set_storage", self.handle, self.STORAGE_FORMATS[fmt])
         return self.storage()
 
+    GATHER_LAYOUTS = {"auto": -1, "quad": 0, "rows": 1}
+
+    def set_gather(self, layout: str) -> dict:
+        """Gather layout of the row-block CSR product: "auto" (chosen per
+        matrix and value type at upload), "quad" (x gathered for 4
+        consecutive entries per thread as they are loaded) or "rows" (the
+        block staged in LDS, each row's lanes gathering as they walk it).
+        Returns gather()."""
+        if layout not in self.GATHER_LAYOUTS:
+            raise ValueError(f"unknown gather layout {layout!r}")
+        N.call("cfdla_matrix_set_gather", self.handle, self.GATHER_LAYOUTS[layout])
+        return self.gather()
+
+    def gather(self) -> dict:
+        """Gather layout in use for FP64 and FP32 products ("quad" for
+        matrices not stored as row-block CSR)."""
+        g64, g32 = C.c_int32(), C.c_int32()
+        N.call("cfdla_matrix_gather", self.handle, C.byref(g64), C.byref(g32))
+        names = {0: "quad", 1: "rows"}
+        return {"fp64": names[g64.value], "fp32": names[g32.value]}
+
     def diagonal(self):
         d = np.zeros(self.nrows)
         for i in range(self.nrows):

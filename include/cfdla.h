@@ -231,6 +231,23 @@ cfdla_status cfdla_matrix_storage(cfdla_matrix m, int32_t *format, int64_t *stor
  * would fit (the CSR layout the reference stores, int32 indices) */
 #define CFDLA_STORAGE_CSR_INT32 3
 cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format);
+/* Gather layout of the row-block CSR product.  CFDLA_GATHER_QUAD: every
+ * thread gathers x for 4 consecutive stored entries of its block as it loads
+ * them; CFDLA_GATHER_ROWS: the block's columns and values are staged in LDS
+ * and each row's lanes gather x while walking the row, so lanes of a
+ * wavefront gather for consecutive rows.  CFDLA_GATHER_AUTO (the default)
+ * picks per matrix and value type, at upload, the layout whose gathers cost
+ * less over a sample of 64 row blocks (instructions with a live lane plus 0.04
+ * per distinct 128-byte line).  Results differ between layouts by rounding
+ * only.  No counterpart in the reference; it exists to time and test each
+ * layout. */
+#define CFDLA_GATHER_AUTO (-1)
+#define CFDLA_GATHER_QUAD 0
+#define CFDLA_GATHER_ROWS 1
+cfdla_status cfdla_matrix_set_gather(cfdla_matrix m, int32_t layout);
+/* Layout in use for FP64 and FP32 products (uploads the matrix if needed);
+ * CFDLA_GATHER_QUAD for matrices not stored as row-block CSR. */
+cfdla_status cfdla_matrix_gather(cfdla_matrix m, int32_t *layout_fp64, int32_t *layout_fp32);
 cfdla_status cfdla_matrix_destroy(cfdla_matrix m);
 
 /* y = A x on host vectors.  Replaces cfd_math::sparse::spmv / try_spmv
@@ -271,6 +288,12 @@ cfdla_status cfdla_precond_apply_device(cfdla_precond p, int32_t dtype, const vo
 /* AlgebraicMultigrid::recompute (amg.rs:179-214): new fine values, same
  * pattern; coarse operators recomputed with the existing P and R. */
 cfdla_status cfdla_amg_update_values(cfdla_precond p, const double *values);
+/* Gather layout (CFDLA_GATHER_*) of the row-block CSR products of every
+ * level of an AMG hierarchy (its operators, transfer operators and fused
+ * coarse product); the device copies are rebuilt if they exist.
+ * CFDLA_GATHER_AUTO (the default) picks per operator.  No counterpart in the
+ * reference; it exists to time and test each layout. */
+cfdla_status cfdla_amg_set_gather(cfdla_precond p, int32_t layout);
 /* The same recompute from the current device values of m (the matrix the
  * preconditioner was built from), on the device: the level-0 copy is
  * refreshed, A_{l+1} = (R_l A_l) P_l is formed by numeric products on the
