@@ -23,12 +23,25 @@
 namespace cfdla {
 namespace {
 
+// EPI_DIR on a stopped solve: the product is skipped, but the new-direction
+// buffer still receives the current direction, so the caller's buffer swap
+// keeps pointing at the latest one when the iteration resumes (after a
+// residual replacement).
+template <typename T>
+__device__ __forceinline__ void keep_direction(int n, const T *__restrict__ p, T *__restrict__ pn) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) pn[i] = p[i];
+}
+
 template <typename T, int EPI, bool DOT, bool SYM, int ND>
 __global__ __launch_bounds__(kBlock) void k_dia(int n, DiaDesc dd, const T *__restrict__ dv,
                                                 const T *__restrict__ x, const T *__restrict__ b,
                                                 const T *__restrict__ dinv, T omega, T *y, const T *dotw, Scal *s,
-                                                int check_done, double *partials) {
-  if (check_done && s->done) return;
+                                                int check_done, double *partials, T *y2) {
+  if (check_done && s->done) {
+    if constexpr (EPI == EPI_DIR) keep_direction(n, x, y2);
+    return;
+  }
+  if constexpr (EPI == EPI_DIR) omega = (T)s->beta;
   // entries per row in column order: SYM lower k = ND-1..1, main, upper k = 1..ND-1
   constexpr int NS = SYM ? 2 * ND - 1 : ND;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -59,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(int n, DiaDesc dd, const T *__re
     T sum = T(0);
 #pragma unroll
     for (int j = 0; j < NS; ++j) sum += (ok[j] && v[j] != T(0)) ? v[j] * xv[j] : T(0);
-    epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc);
+    epi_store<T, EPI, DOT>(e, row, sum, omega, y, dotw, dacc, y2);
   }
   if constexpr (DOT) {
     double d1[1] = {dacc};
@@ -97,6 +110,12 @@ __device__ __forceinline__ void gatherv(int c, const T *__restrict__ x, const T 
     ldv<T, R>(b + c, bb);
 #pragma unroll
     for (int r = 0; r < R; ++r) o[r] = omega * dd[r] * bb[r];
+  } else if constexpr (EPI == EPI_DIR) {
+    T pp[R], zz[R];
+    ldv<T, R>(x + c, pp);
+    ldv<T, R>(b + c, zz);
+#pragma unroll
+    for (int r = 0; r < R; ++r) o[r] = cg_dir(omega, pp[r], zz[r]);
   } else {
     ldv<T, R>(x + c, o);
   }
@@ -106,8 +125,12 @@ template <typename T, int EPI, bool DOT, int ND, int R>
 __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *__restrict__ dv,
                                                     const T *__restrict__ x, const T *__restrict__ b,
                                                     const T *__restrict__ dinv, T omega, T *y, const T *dotw,
-                                                    Scal *s, int check_done, double *partials) {
-  if (check_done && s->done) return;
+                                                    Scal *s, int check_done, double *partials, T *y2) {
+  if (check_done && s->done) {
+    if constexpr (EPI == EPI_DIR) keep_direction(n, x, y2);
+    return;
+  }
+  if constexpr (EPI == EPI_DIR) omega = (T)s->beta;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
   const int groups = n / R;
   const int tiles = (groups + kBlock - 1) / kBlock;
@@ -126,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *
       if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI0) ldv<T, R>(dinv + row, td);
       if constexpr (EPI == EPI_JACOBI) ldv<T, R>(x + row, tx);
       if constexpr (EPI == EPI_ADD) ldv<T, R>(y + row, ty);
-      if constexpr (DOT) {
+      if constexpr (DOT && EPI != EPI_DIR) {
         if (dotw != y) ldv<T, R>(dotw + row, tw);
       }
 #pragma unroll
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *
           e[r].x = omega * td[r] * tb[r];
         }
         if constexpr (EPI == EPI_ADD) e[r].y = ty[r];
-        if constexpr (DOT) {
+        if constexpr (DOT && EPI != EPI_DIR) {
           if (dotw != y) e[r].w = tw[r];
         }
       }
@@ -151,6 +174,10 @@ __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *
     T vm[R], xm[R];
     ldv<T, R>(dv + row, vm);
     gatherv<T, EPI, R>(row, x, b, dinv, omega, xm);
+    if constexpr (EPI == EPI_DIR) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) e[r].x = xm[r];  // the direction at the lane's own rows
+    }
     // off-diagonals: lower (column row + r - off) and upper (row + r + off)
     T vl[ND][R], xl[ND][R], vu[ND][R], xu[ND][R];
     bool okl[ND][R], oku[ND][R];
@@ -199,12 +226,18 @@ __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *
       for (int k = 1; k < ND; ++k)
         sum += (oku[k][r] && vu[k][r] != T(0)) ? vu[k][r] * xu[k][r] : T(0);
       out[r] = epi_out<T, EPI>(e[r], sum, omega);
-      if constexpr (DOT) dacc += (double)(dotw == y ? out[r] : e[r].w) * (double)out[r];
+      if constexpr (DOT) dacc += (double)(EPI == EPI_DIR ? e[r].x : (dotw == y ? out[r] : e[r].w)) * (double)out[r];
     }
     vec_t<T, R> ov;
 #pragma unroll
     for (int r = 0; r < R; ++r) ov[r] = out[r];
     *reinterpret_cast<vec_t<T, R> *>(y + row) = ov;
+    if constexpr (EPI == EPI_DIR) {
+      vec_t<T, R> pv;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pv[r] = e[r].x;
+      *reinterpret_cast<vec_t<T, R> *>(y2 + row) = pv;
+    }
   }
   if constexpr (DOT) {
     double d1[1] = {dacc};
@@ -214,16 +247,16 @@ __global__ __launch_bounds__(kBlock) void k_dia_vec(int n, DiaDesc dd, const T *
 
 template <typename T, int EPI, bool DOT, bool SYM, int ND>
 void launch_shape(const DevCsr &A, int blocks, const T *x, const T *b, const T *dinv, double omega, T *y,
-                  const T *dotw, Scal *s, bool check_done, RedWs &ws) {
+                  const T *dotw, Scal *s, bool check_done, RedWs &ws, T *y2) {
   hipLaunchKernelGGL((k_dia<T, EPI, DOT, SYM, ND>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.dia_desc,
-                     A.dia_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p);
+                     A.dia_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
 }
 
 template <typename T, int EPI, bool DOT, int ND, int R>
 void launch_vec(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
-                Scal *s, bool check_done, RedWs &ws, int blocks) {
+                Scal *s, bool check_done, RedWs &ws, int blocks, T *y2) {
   hipLaunchKernelGGL((k_dia_vec<T, EPI, DOT, ND, R>), dim3(blocks), dim3(kBlock), 0, stream(), A.n, A.dia_desc,
-                     A.dia_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p);
+                     A.dia_vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
 }
 
 bool aligned(const void *p, size_t bytes) { return (reinterpret_cast<uintptr_t>(p) & (bytes - 1)) == 0; }
@@ -238,7 +271,7 @@ constexpr int dia_r() { return sizeof(T) == 8 ? 2 : 1; }
 
 template <typename T>
 int dia_rows_per_lane(const DevCsr &A, const void *x, const void *b, const void *dinv, const void *y,
-                      const void *dotw) {
+                      const void *dotw, const void *y2) {
   constexpr int R = dia_r<T>();
   if (R == 1) return 1;
   const DiaDesc &d = A.dia_desc;
@@ -246,7 +279,7 @@ int dia_rows_per_lane(const DevCsr &A, const void *x, const void *b, const void 
   if (d.off[1] != 1) return 1;
   for (int k = 2; k < d.nd; ++k)
     if (d.off[k] % R != 0) return 1;
-  for (const void *p : {x, b, dinv, y, dotw, static_cast<const void *>(A.dia_vals<T>())})
+  for (const void *p : {x, b, dinv, y, dotw, y2, static_cast<const void *>(A.dia_vals<T>())})
     if (p != nullptr && !aligned(p, R * sizeof(T))) return 1;
   return R;
 }
@@ -262,26 +295,26 @@ bool dia_supported(const DiaDesc &d) {
 
 template <typename T, int EPI, bool DOT>
 void launch_dia(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
-                Scal *s, bool check_done, int op, RedWs &ws) {
+                Scal *s, bool check_done, int op, RedWs &ws, T *y2) {
   // one 256-row tile per block up to the partials capacity: blocks that
   // finish hand their CU to the next tile without a loop-carried wait
   // (26.2 us against 29.1 us with 4 tiles per block, FP64 y = A x, 128^3)
   const DiaDesc &d = A.dia_desc;
   constexpr int R = dia_r<T>();
   if constexpr (R > 1) {
-    if (dia_rows_per_lane<T>(A, x, b, dinv, y, dotw) == R) {
+    if (dia_rows_per_lane<T>(A, x, b, dinv, y, dotw, y2) == R) {
       const int vblocks = xcd_grid((A.n / R + kBlock - 1) / kBlock, 1, kMaxBlocks);
       switch (d.nd) {
-        case 2: launch_vec<T, EPI, DOT, 2, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks); break;
-        case 3: launch_vec<T, EPI, DOT, 3, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks); break;
-        default: launch_vec<T, EPI, DOT, 4, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks); break;
+        case 2: launch_vec<T, EPI, DOT, 2, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks, y2); break;
+        case 3: launch_vec<T, EPI, DOT, 3, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks, y2); break;
+        default: launch_vec<T, EPI, DOT, 4, R>(A, x, b, dinv, omega, y, dotw, s, check_done, ws, vblocks, y2); break;
       }
       if (DOT) finish(1, s, op, vblocks, ws, check_done);
       return;
     }
   }
   const int blocks = xcd_grid((A.n + kBlock - 1) / kBlock, 1, kMaxBlocks);
-#define CFDLA_DIA(SYM, ND) launch_shape<T, EPI, DOT, SYM, ND>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws)
+#define CFDLA_DIA(SYM, ND) launch_shape<T, EPI, DOT, SYM, ND>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2)
   if (d.sym) {
     switch (d.nd) {
       case 2: CFDLA_DIA(true, 2); break;
@@ -305,15 +338,17 @@ void launch_dia(const DevCsr &A, const T *x, const T *b, const T *dinv, double o
 
 #define CFDLA_DIA_INST(T, E)                                                                                   \
   template void launch_dia<T, E, false>(const DevCsr &, const T *, const T *, const T *, double, T *, const T *, \
-                                        Scal *, bool, int, RedWs &);                                           \
+                                        Scal *, bool, int, RedWs &, T *);                                      \
   template void launch_dia<T, E, true>(const DevCsr &, const T *, const T *, const T *, double, T *, const T *,  \
-                                       Scal *, bool, int, RedWs &);
-#define CFDLA_DIA_ALL(T)        \
-  CFDLA_DIA_INST(T, EPI_Y)      \
-  CFDLA_DIA_INST(T, EPI_RESID)  \
-  CFDLA_DIA_INST(T, EPI_JACOBI) \
-  CFDLA_DIA_INST(T, EPI_ADD)    \
-  CFDLA_DIA_INST(T, EPI_JACOBI0)
+                                       Scal *, bool, int, RedWs &, T *);
+#define CFDLA_DIA_ALL(T)         \
+  CFDLA_DIA_INST(T, EPI_Y)       \
+  CFDLA_DIA_INST(T, EPI_RESID)   \
+  CFDLA_DIA_INST(T, EPI_JACOBI)  \
+  CFDLA_DIA_INST(T, EPI_ADD)     \
+  CFDLA_DIA_INST(T, EPI_JACOBI0) \
+  template void launch_dia<T, EPI_DIR, true>(const DevCsr &, const T *, const T *, const T *, double, T *, const T *, \
+                                             Scal *, bool, int, RedWs &, T *);
 CFDLA_DIA_ALL(double)
 CFDLA_DIA_ALL(float)
 

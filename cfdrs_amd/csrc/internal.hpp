@@ -293,7 +293,7 @@ enum FinalizeOp : int {
   OP_BI_RR_RHO = 14,   // BiCGSTAB: OP_BI_RR on red0, then (still running) OP_BI_RHO on red1 = rhat . r
 };
 
-enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4, EPI_RJ0 = 5 };
+enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACOBI0 = 4, EPI_RJ0 = 5, EPI_DIR = 6 };
 
 // ---- kernel launchers (kernels.hip) -------------------------------------
 // Diagonal-format product with epilogue (dia.hip); A.dia must hold and the
@@ -301,7 +301,7 @@ enum Epi : int { EPI_Y = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_ADD = 3, EPI_JACO
 bool dia_supported(const DiaDesc &d);
 template <typename T, int EPI, bool DOT>
 void launch_dia(const DevCsr &A, const T *x, const T *b, const T *dinv, double omega, T *y, const T *dotw,
-                Scal *s, bool check_done, int op, RedWs &ws);
+                Scal *s, bool check_done, int op, RedWs &ws, T *y2 = nullptr);
 // Single-block kernel that sums the block partials of a reducing kernel and
 // runs the finalize op (kernels.hip).
 void finish(int nv, Scal *s, int op, int nblocks, RedWs &ws, bool check_done);
@@ -354,12 +354,22 @@ void dense_inverse_device(int n, const int32_t *rp, const int32_t *ci, const dou
 //   EPI_RJ0     y = A x and y2 = omega * dinv .* y (row-block CSR only: a
 //               restriction that also forms the coarse level's first Jacobi
 //               iterate from zero; dinv and y2 are the coarse level's)
+//   EPI_DIR     the CG direction update folded into the product (diagonal
+//               storage only, fuses_direction): p = b + beta x formed on the
+//               fly at every gathered column, as k_cg_direction forms it
+//               (x = the previous direction, b = z, beta = s->beta; omega
+//               unused); y = A p, y2 = p, and the dot reduces p . y
 // When dot_op != OP_NONE, also reduces sum_i dotw_i * y_i (dotw defaults to
 // x when null) and runs dot_op.  Skipped entirely when *s->done != 0 and
 // check_done is set.
 template <typename T>
 void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, double omega, T *y,
                Scal *s, bool check_done, int dot_op, const T *dotw, RedWs &ws, T *y2 = nullptr);
+// True when csr_apply takes EPI_DIR for A in value type T.
+template <typename T>
+bool fuses_direction(const DevCsr &A) {
+  return A.dia && A.dia_vals<T>() != nullptr;
+}
 
 // Reductions: up to 2 dot products of (a0,b0), (a1,b1) then op.
 template <typename T>

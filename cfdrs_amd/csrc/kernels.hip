@@ -793,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, const T *__restr
                                                          Scal *s) {
   if (s->done) return;
   const T beta = (T)s->beta;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i] + beta * p[i];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = cg_dir(beta, p[i], z[i]);
 }
 
 template <typename T>
@@ -1063,6 +1063,11 @@ void csr_apply(const DevCsr &A, int epi, const T *x, const T *b, const T *dinv, 
       CFDLA_REQUIRE(!dot && y2 != nullptr && dinv != nullptr, CFDLA_ERR_INVALID_ARG,
                     "restriction epilogue takes the coarse dinv and a second output, no dot");
       launch_csr_rb<T, EPI_RJ0, false>(A, x, b, dinv, omega, y, dotw, s, check_done, dot_op, ws, y2);
+      break;
+    case EPI_DIR:
+      CFDLA_REQUIRE(dot && y2 != nullptr && b != nullptr && fuses_direction<T>(A), CFDLA_ERR_INVALID_ARG,
+                    "direction epilogue takes diagonal storage, z, a second output and a dot");
+      launch_dia<T, EPI_DIR, true>(A, x, b, dinv, omega, y, dotw, s, check_done, dot_op, ws, y2);
       break;
     default:
       throw Error(CFDLA_ERR_INVALID_ARG, "unknown CSR epilogue");

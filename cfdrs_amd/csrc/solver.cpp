@@ -1183,6 +1183,11 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
   RedWs &ws = c.ws;
   const int n = A.n;
   T *r = vec<T>(0, n), *z = vec<T>(1, n), *p = vec<T>(2, n), *q = vec<T>(3, n);
+  // Diagonal storage folds p = z + beta p into the next product (EPI_DIR),
+  // which writes the new direction into the other buffer of a pair.
+  const bool fuse = fuses_direction<T>(A);
+  T *pn = fuse ? vec<T>(4, n) : nullptr;
+  bool fresh = true;  // p was just set from z: the next product takes it as is
   Scal *s = c.s(0);
   SolveOut o;
   o.thr = thr;
@@ -1196,6 +1201,7 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
   auto start_direction = [&]() {
     M->apply_dot(n, r, z, s, false, OP_SET_RZ);
     copy_vec<T>(n, z, p, s, false);
+    fresh = true;
   };
   start_direction();
   if (chunk <= 0) chunk = auto_chunk(M, n);
@@ -1208,14 +1214,20 @@ SolveOut pcg(const DevCsr &A, Precond *M, const T *b, T *x, double thr, int64_t 
     if (enq < max_it) {
       int64_t steps = std::min<int64_t>(chunk, max_it - enq);
       for (int64_t k = 0; k < steps; ++k) {
-        csr_apply<T>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        if (fuse && !fresh) {
+          csr_apply<T>(A, EPI_DIR, p, z, nullptr, 0.0, q, s, true, OP_CG_ALPHA, nullptr, ws, pn);
+          std::swap(p, pn);
+        } else {
+          csr_apply<T>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        }
+        fresh = false;
         if (jd) {
           cg_update<T>(n, p, q, x, r, jd, z, s, ws);
         } else {
           cg_update<T>(n, p, q, x, r, nullptr, nullptr, s, ws);
           M->apply_dot(n, r, z, s, true, OP_CG_BETA);
         }
-        cg_direction<T>(n, z, p, s);
+        if (!fuse) cg_direction<T>(n, z, p, s);
       }
       enq += steps;
       poll.post();
@@ -1290,6 +1302,10 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
   const int n = A.n;
   double *r64 = c.dv_at(8, n);
   float *r = c.fv_at(0, n), *z = c.fv_at(1, n), *p = c.fv_at(2, n), *q = c.fv_at(3, n), *xc = c.fv_at(4, n);
+  // Diagonal storage folds p = z + beta p into the next product (EPI_DIR)
+  const bool fuse = fuses_direction<float>(A);
+  float *pn = fuse ? c.fv_at(5, n) : nullptr;
+  bool fresh = true;  // p was just set from z: the next product takes it as is
   Scal *s = c.s(0);
   SolveOut o;
   o.thr = thr;
@@ -1320,7 +1336,7 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
     } else {
       M->apply_dot(n, r, z, s, true, OP_CG_BETA);
     }
-    cg_direction<float>(n, z, p, s);
+    if (!fuse) cg_direction<float>(n, z, p, s);
   };
   int64_t enq = 0;
   int32_t replacements = 0;  // enqueued, including those the device skips after stopping
@@ -1332,14 +1348,20 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
     if (enq < max_it) {
       const int64_t steps = std::min<int64_t>(chunk, max_it - enq);
       for (int64_t k = 0; k < steps; ++k) {
-        csr_apply<float>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        if (fuse && !fresh) {
+          csr_apply<float>(A, EPI_DIR, p, z, nullptr, 0.0, q, s, true, OP_CG_ALPHA, nullptr, ws, pn);
+          std::swap(p, pn);
+        } else {
+          csr_apply<float>(A, EPI_Y, p, nullptr, nullptr, 0.0, q, s, true, OP_CG_ALPHA, p, ws);
+        }
+        fresh = false;
         const bool rep = want || (interval > 0 && (enq + k + 1) % interval == 0);
         if (rep) ++replacements;
         want = false;
         if (jd && !rep) {
           // xc += alpha p, r -= alpha q, z = D^-1 r, rr and rz -> beta in one pass
           cg_update<float>(n, p, q, xc, r, jd, z, s, ws);
-          cg_direction<float>(n, z, p, s);
+          if (!fuse) cg_direction<float>(n, z, p, s);
         } else {
           cg_update<float>(n, p, q, xc, r, nullptr, nullptr, s, ws);  // xc += alpha p, r -= alpha q, rr
           if (rep) replace(true);
