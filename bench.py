@@ -305,7 +305,11 @@ def spmv_roofline(parts, n: int, nnz: int, b_host, args, fmt: str) -> dict:
               "sell64": f"k_sell (64-row slices, int16 offsets, {storage['stored_entries']} stored slots)",
               "csr_rowblock": f"k_csr_rb ({storage['units']} row blocks)",
               "csr_int32": f"k_csr_rb, row-block CSR with int32 row pointers and columns ({storage['units']} row "
-                           f"blocks of at most 2048 nonzeros)"}[storage["format"]]
+                           f"blocks of at most 256 rows and 2048 nonzeros)"}[storage["format"]]
+    if storage["format"] in ("csr_rowblock", "csr_int32"):
+        # gather layout the upload picked for this operator, per value type
+        g = mats[0].gather()
+        kernel += f"; gathers: FP64 {g['fp64']}, FP32 {g['fp32']}"
     fn = N.lib().cfdla_spmv_device
     out = {"kernel": kernel, "format": storage["format"]}
     for dt, s, code in ((torch.float64, 8, N.DT_F64), (torch.float32, 4, N.DT_F32)):
