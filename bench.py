@@ -265,7 +265,8 @@ def main() -> None:
                      "algorithmic_bytes": spmv[8]["bytes"],
                      "same_bytes_copy_gbs": round(spmv[8]["copy_gbs"], 1),
                      "frac_of_same_bytes_copy": round(spmv[8]["gbs"] / spmv[8]["copy_gbs"], 4),
-                     "fp32_us": round(spmv[4]["us"], 3), "fp32_gbs": round(spmv[4]["gbs"], 1)},
+                     "fp32_us": round(spmv[4]["us"], 3), "fp32_gbs": round(spmv[4]["gbs"], 1),
+                     "fp32_frac": round(spmv[4]["gbs"] / HBM_PEAK_GBS, 4)},
         "spmv_device_format": {"kernel": "FP64 y = A x, " + spmv_dev["kernel"], "avg_us": round(spmv_dev[8]["us"], 3),
                                "format_bytes": spmv_dev[8]["bytes"], "achieved_gbs": round(spmv_dev[8]["gbs"], 1),
                                "frac": round(spmv_dev[8]["gbs"] / HBM_PEAK_GBS, 4),
