@@ -419,8 +419,8 @@ cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format) {
 cfdla_status cfdla_matrix_set_gather(cfdla_matrix m, int32_t layout) {
   return guarded([&] {
     require_ptr(m, "matrix");
-    CFDLA_REQUIRE(layout == CFDLA_GATHER_AUTO || layout == CFDLA_GATHER_QUAD || layout == CFDLA_GATHER_ROWS,
-                  CFDLA_ERR_INVALID_ARG, "unknown gather layout");
+    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_STRIDED, CFDLA_ERR_INVALID_ARG,
+                  "unknown gather layout");
     if (m->dev.gather_request == layout) return;
     m->dev.gather_request = layout;
     if (m->on_device) m->dev.upload(host_values(m), true, false);
@@ -432,8 +432,8 @@ cfdla_status cfdla_matrix_gather(cfdla_matrix m, int32_t *layout_fp64, int32_t *
     require_ptr(m, "matrix");
     const DevCsr &d = devmat(m);
     const bool rowblock = !d.dia && !d.sell;
-    if (layout_fp64) *layout_fp64 = rowblock && d.walk64 ? CFDLA_GATHER_ROWS : CFDLA_GATHER_QUAD;
-    if (layout_fp32) *layout_fp32 = rowblock && d.walk32 ? CFDLA_GATHER_ROWS : CFDLA_GATHER_QUAD;
+    if (layout_fp64) *layout_fp64 = rowblock ? d.layout64 : CFDLA_GATHER_QUAD;
+    if (layout_fp32) *layout_fp32 = rowblock ? d.layout32 : CFDLA_GATHER_QUAD;
   });
 }
 
@@ -575,8 +575,8 @@ cfdla_status cfdla_amg_set_gather(cfdla_precond p, int32_t layout) {
     require_ptr(p, "preconditioner");
     auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
     CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
-    CFDLA_REQUIRE(layout == CFDLA_GATHER_AUTO || layout == CFDLA_GATHER_QUAD || layout == CFDLA_GATHER_ROWS,
-                  CFDLA_ERR_INVALID_ARG, "unknown gather layout");
+    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_STRIDED, CFDLA_ERR_INVALID_ARG,
+                  "unknown gather layout");
     if (amg->gather_request == layout) return;
     amg->gather_request = layout;
     if (amg->on_device) amg->upload();

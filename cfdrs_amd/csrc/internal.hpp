@@ -144,6 +144,8 @@ template <typename T>
 constexpr int rb_rows() {
   return 256;
 }
+// Gather layouts of the row-block product (CFDLA_GATHER_QUAD / ROWS / STRIDED).
+constexpr int kRbQuad = 0, kRbRows = 1, kRbStrided = 2;
 // Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
 // one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
 // 90 % nonzeros.  Values are stored diagonal-major, dia_v[k * n + i] =
@@ -171,8 +173,8 @@ struct DevCsr {
   // every block of that partition has columns spanning at most 65535: 2 bytes
   // per nonzero instead of 4.  Each partition decides on its own.
   bool c16_f64 = false, c16_f32 = false;
-  // row-block gather layout per value type: rows (LDS-staged row walk) or quads
-  bool walk64 = false, walk32 = false;
+  // row-block gather layout per value type (kRbQuad, kRbRows, kRbStrided)
+  int layout64 = 0, layout32 = 0;
   int gather_request = -1;  // CFDLA_GATHER_*
   DevBuf<uint16_t> c16_64, c16_32;
   DevBuf<int32_t> cbase64, cbase32;
@@ -217,7 +219,7 @@ struct DevCsr {
   template <typename T> const int *rb() const { return sizeof(T) == 8 ? rb64.p : rb32.p; }
   template <typename T> int nrb() const { return sizeof(T) == 8 ? nrb64 : nrb32; }
   template <typename T> bool has_cols16() const { return sizeof(T) == 8 ? c16_f64 : c16_f32; }
-  template <typename T> bool rb_walk() const { return sizeof(T) == 8 ? walk64 : walk32; }
+  template <typename T> int rb_layout() const { return sizeof(T) == 8 ? layout64 : layout32; }
   template <typename T> const uint16_t *cols16() const { return sizeof(T) == 8 ? c16_64.p : c16_32.p; }
   template <typename T> const int *cbase() const { return sizeof(T) == 8 ? cbase64.p : cbase32.p; }
 };
@@ -234,10 +236,10 @@ template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32
 // span > 0: a block also closes before its columns would span more than
 // `span` (so 16-bit column offsets fit every block whose rows each do).
 void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span = 0);
-// True when the row walk's gathers cost less than the quad layout's for this
-// partition (value size vbytes): instructions with a live lane plus 0.04 per
-// distinct 128-byte line of x, over a sample of about 64 row blocks.
-bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes);
+// The gather layout (kRbQuad or kRbRows) whose gathers cost least
+// for this partition (value size vbytes): instructions with a live lane plus
+// 0.04 per distinct 128-byte line of x, over a sample of about 64 row blocks.
+int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes);
 
 // Device-resident scalars of one solve.  A reducing kernel writes block
 // partials; the single-block finalize kernel launched after it sums them into

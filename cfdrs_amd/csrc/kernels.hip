@@ -253,7 +253,7 @@ __device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
 // rb_cap / (4 kBlock) = 2 quads per thread, the at most 3 entries past them by
 // three lanes), all loads in flight together.  Each row is then summed by tpr
 // lanes (the largest power of two up to 64 with tpr * rows <= kBlock), in one
-// of two gather layouts picked per operator at upload (rb_prefers_walk):
+// of two gather layouts picked per operator at upload (rb_pick_layout):
 //   * row walk (WALK): the quads are parked in LDS; after the barrier each
 //     row's lanes read their entries back (8 or 4 at a time, all LDS reads
 //     issued before the gathers) and gather x, so lanes of a wavefront hold
@@ -267,6 +267,11 @@ __device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
 //     prolongations) or a few lanes share a long row (the first
 //     restriction), but the 7-point operator's gathers spread over 3 grid
 //     planes (47.3 and 29.5 us).
+//   * strided (forced only, CFDLA_GATHER_STRIDED): staged as for the row
+//     walk, entries dealt to threads round robin so a wavefront gathers for
+//     64 consecutive entries, products written back over the staged values
+//     and summed as in the quad layout (a second barrier); the first
+//     restriction in FP64 took 31.7 against 36.2 us in the quad layout.
 // A row longer than rb_cap is a block of its own, summed in steps of rb_cap
 // entries straight from memory.
 //
@@ -280,10 +285,10 @@ __device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
 // sweeps (377 against 437 us).
 //
 // C16: columns are 16-bit offsets from the block's base column (cb[q]).
-// WALK: the row-walk layout (else the quad layout); separate instantiations,
-// so the quad layout keeps its smaller LDS footprint (8 blocks per CU in FP64
-// instead of 6) and its own register allocation.
-template <typename T, int EPI, bool DOT, bool C16, bool NT, bool WALK>
+// LAYOUT: kRbQuad, kRbRows (row walk) or kRbStrided; separate
+// instantiations, so the quad layout keeps its smaller LDS footprint (8
+// blocks per CU in FP64 instead of 6) and its own register allocation.
+template <typename T, int EPI, bool DOT, bool C16, bool NT, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restrict__ rb, const int *__restrict__ rp,
                                                    const int *__restrict__ ci, const uint16_t *__restrict__ c16,
                                                    const int *__restrict__ cb, const T *__restrict__ av,
@@ -295,8 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   constexpr int Q = CAP / (4 * kBlock);  // quads per thread
   static_assert(Q * 4 * kBlock == CAP, "the row-block cap is a whole number of quads per thread");
   static_assert(rb_rows<T>() == kBlock, "one row per thread at most");
-  __shared__ T sval[CAP + 4];  // the walk's values, or the quad layout's products
-  __shared__ int scol[WALK ? CAP + 4 : 1];
+  __shared__ T sval[CAP + 4];  // staged values (row walk, strided), or products
+  __shared__ int scol[LAYOUT != kRbQuad ? CAP + 4 : 1];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -405,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
       const int safe = C16 ? b0 : 0;
       const T *sv = sval - ea;
       T sum = T(0);
-      if constexpr (WALK) {
+      if constexpr (LAYOUT != kRbQuad) {
         if (ne > 0) {
           // every quad and the tail loaded first (addresses clamped into the
           // block, so all loads are issued together), then stored to LDS
@@ -439,7 +444,39 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
         }
         __syncthreads();
         const int *sc = scol - ea;
-        if (tpr == 1) {
+        if constexpr (LAYOUT == kRbStrided) {
+          // entries dealt to threads round robin (thread tid takes
+          // e0 + tid + kBlock u), so a wavefront gathers for 64 consecutive
+          // entries; each product overwrites its own staged value, then rows
+          // are summed from LDS as in the quad layout.  A thread without an
+          // entry reads slot CAP + 3, which no block stages.
+          constexpr int U = CAP / kBlock;
+          int kk[U];
+          T vk[U], xv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int k = e0 + tid + kBlock * u;
+            const bool ok = k < e1;
+            kk[u] = ok ? k - ea : CAP + 3;
+            const int cc = scol[kk[u]];
+            vk[u] = sval[kk[u]];
+            xv[u] = gather<T, EPI>(ok ? cc : safe, x, b, dinv, omega);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (kk[u] != CAP + 3) sval[kk[u]] = vk[u] * xv[u];
+          __syncthreads();
+          T s4[4] = {T(0), T(0), T(0), T(0)};
+          int k = ks + lane;
+          for (; k + 3 * tpr < ke; k += 4 * tpr) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s4[u] += sv[k + u * tpr];
+          }
+#pragma unroll
+          for (int u = 0; u < 3; ++u)
+            if (k + u * tpr < ke) s4[u] += sv[k + u * tpr];
+          sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        } else if (tpr == 1) {
           // the row's entries in order, 8 gathers in flight
           for (int k = ks; k < ke; k += 8) {
             T vv[8], xv[8];
@@ -563,11 +600,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
 #define CFDLA_RB_NT_MIN_BYTES (32.0 * 1024.0 * 1024.0)
 #endif
 
-template <typename T, int EPI, bool DOT, bool C16, bool NT, bool WALK>
+template <typename T, int EPI, bool DOT, bool C16, bool NT, int LAYOUT>
 void launch_csr_rb_k(const DevCsr &A, int blocks, const T *x, const T *b, const T *dinv, double omega, T *y,
                      const T *dotw, Scal *s, bool check_done, RedWs &ws, T *y2) {
   const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
-  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT, WALK>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
+  hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT, LAYOUT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
                      A.rp.p, C16 ? nullptr : A.ci.p, C16 ? A.cols16<T>() : nullptr, C16 ? A.cbase<T>() : nullptr,
                      A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
 }
@@ -579,16 +616,21 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
   const int blocks = xcd_grid(nrb, 1, kMaxBlocks);
   const bool c16 = A.has_cols16<T>();
   const bool nt = sizeof(T) == 4 || (double)A.nnz * (double)(sizeof(T) + (c16 ? 2 : 4)) >= CFDLA_RB_NT_MIN_BYTES;
-  const bool walk = A.rb_walk<T>();
-#define CFDLA_RB(C, N, W) \
-  launch_csr_rb_k<T, EPI, DOT, C, N, W>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2)
+  const int layout = A.rb_layout<T>();
+#define CFDLA_RB(C, N, L) \
+  launch_csr_rb_k<T, EPI, DOT, C, N, L>(A, blocks, x, b, dinv, omega, y, dotw, s, check_done, ws, y2)
+#define CFDLA_RB3(C, N)                                  \
+  if (layout == kRbRows) CFDLA_RB(C, N, kRbRows);        \
+  else if (layout == kRbStrided) CFDLA_RB(C, N, kRbStrided); \
+  else CFDLA_RB(C, N, kRbQuad);
   if (c16) {
-    if (nt) walk ? CFDLA_RB(true, true, true) : CFDLA_RB(true, true, false);
-    else walk ? CFDLA_RB(true, false, true) : CFDLA_RB(true, false, false);
+    if (nt) { CFDLA_RB3(true, true) }
+    else { CFDLA_RB3(true, false) }
   } else {
-    if (nt) walk ? CFDLA_RB(false, true, true) : CFDLA_RB(false, true, false);
-    else walk ? CFDLA_RB(false, false, true) : CFDLA_RB(false, false, false);
+    if (nt) { CFDLA_RB3(false, true) }
+    else { CFDLA_RB3(false, false) }
   }
+#undef CFDLA_RB3
 #undef CFDLA_RB
   if (DOT) finish(1, s, op, blocks, ws, check_done);
 }

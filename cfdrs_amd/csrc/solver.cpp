@@ -179,30 +179,35 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
   }
 }
 
-bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes) {
-  // The two layouts of k_csr_rb's regular path, wavefront by wavefront:
-  //   quad: gather instruction (u, j) of wave w covers entry
-  //         ea + 4 (64 w + lane + 256 u) + j of the block;
-  //   rows: lane t of wave w walks row t / tpr; its p-th gather is entry
-  //         ks + p (tpr = 1) or ks + t % tpr + p tpr (tpr > 1).
+int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes) {
+  // The layouts of k_csr_rb's regular path, wavefront by wavefront:
+  //   quad:    gather instruction (u, j) of wave w covers entry
+  //            ea + 4 (64 w + lane + 256 u) + j of the block;
+  //   rows:    lane t of wave w walks row t / tpr; its p-th gather is entry
+  //            ks + p (tpr = 1) or ks + t % tpr + p tpr (tpr > 1).
+  // (The strided layout is not chosen here: on this count it looks best for
+  // R0 and the prolongations, but its second barrier and pass of products
+  // through LDS, which the count leaves out, made it slower on P0 (+7-12 %)
+  // and on R0 in FP32 (+3 %); it won on R0 in FP64 (36.2 -> 31.7 us).  It is
+  // available through CFDLA_GATHER_STRIDED.)
   // An instruction counts when some lane has a live entry; its lines are the
   // distinct 128-byte lines of x those lanes read.  Measured on the 128^3
-  // hierarchy (FP32 solve, per launch), 0.04 per line orders every operator:
-  // the 7-point operator, R1 and the level-2 operator faster in rows, R0 and
-  // every prolongation faster in quads.
+  // hierarchy (FP32 solve, per launch), 0.04 per line orders quad and rows
+  // for every operator: the 7-point operator, R1 and the level-2 operator
+  // faster in rows, R0 and every prolongation faster in quads.
   constexpr double kLineCost = 0.04;
   const int64_t nrb = (int64_t)rb.size() / 2 - 1;
-  if (nrb <= 0) return false;
+  if (nrb <= 0) return kRbQuad;
   const int64_t step = std::max<int64_t>(1, nrb / 64);
   const int cap = rb_cap<double>();
-  double iq = 0.0, lq = 0.0, iw = 0.0, lw = 0.0;
+  double instr[3] = {0.0, 0.0, 0.0}, nl[3] = {0.0, 0.0, 0.0};
   std::vector<int64_t> lines;
   lines.reserve(64);
-  auto count = [&](double &instr, double &nl) {
+  auto count = [&](int layout) {
     if (lines.empty()) return;
     std::sort(lines.begin(), lines.end());
-    instr += 1.0;
-    nl += (double)(std::unique(lines.begin(), lines.end()) - lines.begin());
+    instr[layout] += 1.0;
+    nl[layout] += (double)(std::unique(lines.begin(), lines.end()) - lines.begin());
     lines.clear();
   };
   auto line = [&](int64_t k) { return ((int64_t)h.ci[k] * vbytes) >> 7; };
@@ -217,7 +222,7 @@ bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbyte
             const int64_t k = ea + 4 * (64 * w + l + kBlock * u) + j;
             if (k >= e0 && k < e1) lines.push_back(line(k));
           }
-          count(iq, lq);
+          count(kRbQuad);
         }
     int64_t tpr = 1;
     while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
@@ -236,11 +241,12 @@ bool rb_prefers_walk(const HostCsr &h, const std::vector<int32_t> &rb, int vbyte
       for (int64_t p = 0; p < maxp; ++p) {
         for (int l = 0; l < 64; ++l)
           if (p < len[l]) lines.push_back(line(ks[l] + p * tpr));
-        count(iw, lw);
+        count(kRbRows);
       }
     }
   }
-  return iw + kLineCost * lw < iq + kLineCost * lq;
+  const double cq = instr[kRbQuad] + kLineCost * nl[kRbQuad], cr = instr[kRbRows] + kLineCost * nl[kRbRows];
+  return cr < cq ? kRbRows : kRbQuad;
 }
 
 namespace {
@@ -520,12 +526,14 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   }
   // gather layout of the row-block product (blocks holds the FP32 partition
   // when there is one); evaluated only for matrices stored as row blocks
+  static_assert(CFDLA_GATHER_QUAD == kRbQuad && CFDLA_GATHER_ROWS == kRbRows && CFDLA_GATHER_STRIDED == kRbStrided,
+                "gather layout codes");
   auto pick = [&](const std::vector<int32_t> &part, int vbytes) {
-    if (gather_request != CFDLA_GATHER_AUTO) return gather_request == CFDLA_GATHER_ROWS;
-    return !dia && !sell && rb_prefers_walk(h, part, vbytes);
+    if (gather_request != CFDLA_GATHER_AUTO) return gather_request;
+    return !dia && !sell ? rb_pick_layout(h, part, vbytes) : kRbQuad;
   };
-  walk64 = pick(blocks64, 8);
-  walk32 = want_fp32 && pick(blocks, 4);
+  layout64 = pick(blocks64, 8);
+  layout32 = want_fp32 ? pick(blocks, 4) : kRbQuad;
   sync_stream();
 }
 

@@ -199,7 +199,7 @@ class CsrMatrix:
         N.call("cfdla_matrix_set_storage", self.handle, self.STORAGE_FORMATS[fmt])
         return self.storage()
 
-    GATHER_LAYOUTS = {"auto": -1, "quad": 0, "rows": 1}
+    GATHER_LAYOUTS = {"auto": -1, "quad": 0, "rows": 1, "strided": 2}
 
     def set_gather(self, layout: str) -> dict:
         """Gather layout of the row-block CSR product: "auto" (chosen per
@@ -217,7 +217,7 @@ class CsrMatrix:
         matrices not stored as row-block CSR)."""
         g64, g32 = C.c_int32(), C.c_int32()
         N.call("cfdla_matrix_gather", self.handle, C.byref(g64), C.byref(g32))
-        names = {0: "quad", 1: "rows"}
+        names = {0: "quad", 1: "rows", 2: "strided"}
         return {"fp64": names[g64.value], "fp32": names[g32.value]}
 
     def diagonal(self):

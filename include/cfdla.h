@@ -244,6 +244,12 @@ cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format);
 #define CFDLA_GATHER_AUTO (-1)
 #define CFDLA_GATHER_QUAD 0
 #define CFDLA_GATHER_ROWS 1
+/* CFDLA_GATHER_STRIDED: the block staged in LDS as for ROWS, entries dealt
+ * to threads round robin (a wavefront gathers for 64 consecutive stored
+ * entries), products summed per row as for QUAD.  Never picked by
+ * CFDLA_GATHER_AUTO (which chooses QUAD or ROWS); it suits restrictions with
+ * long rows in FP64. */
+#define CFDLA_GATHER_STRIDED 2
 cfdla_status cfdla_matrix_set_gather(cfdla_matrix m, int32_t layout);
 /* Layout in use for FP64 and FP32 products (uploads the matrix if needed);
  * CFDLA_GATHER_QUAD for matrices not stored as row-block CSR. */

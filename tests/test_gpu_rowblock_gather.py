@@ -1,14 +1,16 @@
-"""GPU parity of the row-block CSR product in both gather layouts.
+"""GPU parity of the row-block CSR product in every gather layout.
 
 quad: every thread gathers x for 4 consecutive stored entries of its block as
 it loads them; rows: the block's columns and values are staged in LDS and
-each row's lanes gather x while walking the row.  Tolerances:
+each row's lanes gather x while walking the row; strided: staged as for rows,
+entries dealt to threads round robin, products summed per row as for quad.
+Tolerances:
 
 * integer-valued products: bit-exact in FP64 and FP32 (every product and
   partial sum is an exact integer below 2^24);
 * a stored zero times a non-finite x gives NaN, as the oracle's CSR product;
-* V-cycles: 1e-11 against the oracle (FP64), 1e-5 (FP32), and the two
-  layouts within 1e-13 of each other (they differ by rounding only);
+* V-cycles: 1e-11 against the oracle (FP64), 1e-5 (FP32), and the layouts
+  within 1e-13 of each other (they differ by rounding only);
 * solves: explicit FP64 relative residual <= 1e-10.
 """
 import numpy as np
@@ -20,7 +22,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = ["quad", "rows"]
+LAYOUTS = ["quad", "rows", "strided"]
 
 
 def _rows(lens, m, seed):
@@ -76,7 +78,7 @@ def _case(name):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("case", ["stencil7", "short", "long", "mixed"])
-def test_spmv_exact_in_both_layouts(case, layout, dtype, storage):
+def test_spmv_exact_in_every_layout(case, layout, dtype, storage):
     import torch
 
     rp, ci, v, n, m = _case(case)
@@ -123,12 +125,13 @@ def test_gather_auto_choice():
     assert a.gather() == {"fp64": "quad", "fp32": "quad"}
     a.set_gather("auto")
     assert a.gather() == {"fp64": "rows", "fp32": "rows"}
+    assert a.set_gather("strided") == {"fp64": "strided", "fp32": "strided"}
     with pytest.raises(ValueError):
         a.set_gather("columns")
 
 
 @pytest.mark.parametrize("nn", [20, 23])
-def test_amg_vcycle_in_both_layouts(nn):
+def test_amg_vcycle_in_every_layout(nn):
     # every row-block product of the hierarchy (restrictions, prolongations,
     # row-block levels, the fused coarse product) in each layout
     import torch
@@ -161,12 +164,13 @@ def test_amg_vcycle_in_both_layouts(nn):
         amg.apply_to(r, z2)
         assert np.linalg.norm(z2 - zo) <= 1e-11 * np.linalg.norm(zo), layout
         out[layout] = z
-    assert np.linalg.norm(out["quad"] - out["rows"]) <= 1e-13 * np.linalg.norm(out["quad"])
+    for layout in ("rows", "strided"):
+        assert np.linalg.norm(out["quad"] - out[layout]) <= 1e-13 * np.linalg.norm(out["quad"])
 
 
 @pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_krylov_solves_in_both_layouts(layout, precision):
+def test_krylov_solves_in_every_layout(layout, precision):
     # 2D variable-coefficient operator, Dirichlet west and east walls
     nn = 96
     n = nn * nn
