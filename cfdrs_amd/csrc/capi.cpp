@@ -419,7 +419,7 @@ cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format) {
 cfdla_status cfdla_matrix_set_gather(cfdla_matrix m, int32_t layout) {
   return guarded([&] {
     require_ptr(m, "matrix");
-    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_STRIDED, CFDLA_ERR_INVALID_ARG,
+    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_SORTED, CFDLA_ERR_INVALID_ARG,
                   "unknown gather layout");
     if (m->dev.gather_request == layout) return;
     m->dev.gather_request = layout;
@@ -575,7 +575,7 @@ cfdla_status cfdla_amg_set_gather(cfdla_precond p, int32_t layout) {
     require_ptr(p, "preconditioner");
     auto *amg = dynamic_cast<AmgPrecond *>(p->p.get());
     CFDLA_REQUIRE(amg != nullptr, CFDLA_ERR_INVALID_ARG, "preconditioner is not AMG");
-    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_STRIDED, CFDLA_ERR_INVALID_ARG,
+    CFDLA_REQUIRE(layout >= CFDLA_GATHER_AUTO && layout <= CFDLA_GATHER_SORTED, CFDLA_ERR_INVALID_ARG,
                   "unknown gather layout");
     if (amg->gather_request == layout) return;
     amg->gather_request = layout;

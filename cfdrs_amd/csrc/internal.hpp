@@ -144,8 +144,15 @@ template <typename T>
 constexpr int rb_rows() {
   return 256;
 }
-// Gather layouts of the row-block product (CFDLA_GATHER_QUAD / ROWS / STRIDED).
-constexpr int kRbQuad = 0, kRbRows = 1, kRbStrided = 2;
+// Gather layouts of the row-block product (CFDLA_GATHER_QUAD / ROWS / STRIDED
+// / SORTED).
+constexpr int kRbQuad = 0, kRbRows = 1, kRbStrided = 2, kRbSorted = 3;
+// Column-sorted blocks store each entry as col | (position in its row-ordered
+// block) << kRbPosShift: columns below 2^21, positions below 2^11 (rb_cap).
+constexpr int kRbPosShift = 21;
+constexpr int32_t kRbColMask = (1 << kRbPosShift) - 1;
+static_assert(rb_cap<double>() <= (1 << (32 - kRbPosShift)) && rb_cap<float>() <= (1 << (32 - kRbPosShift)),
+              "a block position fits the packed bits");
 // Diagonal (DIA) layout for square stencil matrices: every nonzero lies on
 // one of nd <= kDiaMax diagonals (offsets col - row), which hold at least
 // 90 % nonzeros.  Values are stored diagonal-major, dia_v[k * n + i] =
@@ -173,11 +180,20 @@ struct DevCsr {
   // every block of that partition has columns spanning at most 65535: 2 bytes
   // per nonzero instead of 4.  Each partition decides on its own.
   bool c16_f64 = false, c16_f32 = false;
-  // row-block gather layout per value type (kRbQuad, kRbRows, kRbStrided)
+  // row-block gather layout per value type (kRbQuad, kRbRows, kRbStrided,
+  // kRbSorted)
   int layout64 = 0, layout32 = 0;
   int gather_request = -1;  // CFDLA_GATHER_*
   DevBuf<uint16_t> c16_64, c16_32;
   DevBuf<int32_t> cbase64, cbase32;
+  // kRbSorted: every block of at most rb_cap entries stored in column order
+  // (ties in row order), each entry packed with its position in the
+  // row-ordered block (kRbPosShift); longer rows stay in order with position
+  // 0.  One copy for both partitions (they are cut alike); sperm holds the
+  // CSR slot of every stored entry, for the device value refresh.
+  DevBuf<int32_t> sci, sperm;
+  DevBuf<double> sv64;
+  DevBuf<float> sv32;
   DevBuf<double> v64, sdinv64;
   DevBuf<float> v32, sdinv32;
   bool sell = false;
@@ -222,6 +238,9 @@ struct DevCsr {
   template <typename T> int rb_layout() const { return sizeof(T) == 8 ? layout64 : layout32; }
   template <typename T> const uint16_t *cols16() const { return sizeof(T) == 8 ? c16_64.p : c16_32.p; }
   template <typename T> const int *cbase() const { return sizeof(T) == 8 ? cbase64.p : cbase32.p; }
+  // columns and values the row-block product streams
+  template <typename T> const int *rb_cols() const { return rb_layout<T>() == kRbSorted ? sci.p : ci.p; }
+  template <typename T> const T *rb_vals() const;
 };
 template <> inline const double *DevCsr::vals<double>() const { return v64.p; }
 template <> inline const float *DevCsr::vals<float>() const { return v32.p; }
@@ -231,12 +250,20 @@ template <> inline const double *DevCsr::sell_vals<double>() const { return sell
 template <> inline const float *DevCsr::sell_vals<float>() const { return sell_v32.p; }
 template <> inline const double *DevCsr::dia_vals<double>() const { return dia_v64.p; }
 template <> inline const float *DevCsr::dia_vals<float>() const { return dia_v32.p; }
+template <> inline const double *DevCsr::rb_vals<double>() const { return layout64 == kRbSorted ? sv64.p : v64.p; }
+template <> inline const float *DevCsr::rb_vals<float>() const { return layout32 == kRbSorted ? sv32.p : v32.p; }
 
 
 // span > 0: a block also closes before its columns would span more than
 // `span` (so 16-bit column offsets fit every block whose rows each do).
 void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span = 0);
-// The gather layout (kRbQuad or kRbRows) whose gathers cost least
+// Column-sorted copy of a row-block partition (kRbSorted): packed columns,
+// FP64 values and the CSR slot of every stored entry; false (nothing built)
+// when a column reaches 2^21.
+bool build_sorted_blocks(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<int32_t> &packed,
+                         std::vector<double> &vals, std::vector<int32_t> &perm);
+// The gather layout (kRbQuad, kRbRows or, below 2^21 columns, kRbSorted)
+// whose gathers cost least
 // for this partition (value size vbytes): instructions with a live lane plus
 // 0.04 per distinct 128-byte line of x, over a sample of about 64 row blocks.
 int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes);
@@ -316,6 +343,8 @@ void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *
 bool symmetric_on_device(const DevCsr &d);
 // FP32 copy, diagonal / sliced-ELL arrays and smoother diagonal from v64.
 void scatter_formats(DevCsr &d);
+// Column-sorted row-block values from v64 through sperm (kRbSorted).
+void sorted_values(DevCsr &d);
 // S (into sv) and E (into ev) values of a level's fused operator from A's
 // values and smoother diagonal (fused_jacobi.hpp); both follow A's slots.
 void fused_values(const DevCsr &A, double w, double *sv, double *ev);

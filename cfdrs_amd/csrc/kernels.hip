@@ -272,6 +272,11 @@ __device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
 //     64 consecutive entries, products written back over the staged values
 //     and summed as in the quad layout (a second barrier); the first
 //     restriction in FP64 took 31.7 against 36.2 us in the quad layout.
+//   * sorted (CFDLA_GATHER_SORTED): the quad layout over blocks stored in
+//     column order, every entry packed with its row-ordered position
+//     (kRbPosShift), so a wavefront's gathers fall on adjacent columns; each
+//     product is parked at its position and the rows are summed exactly as
+//     in the quad layout, bit for bit the same result.
 // A row longer than rb_cap is a block of its own, summed in steps of rb_cap
 // entries straight from memory.
 //
@@ -285,7 +290,7 @@ __device__ __forceinline__ void store_quad(double *p, const double (&v)[4]) {
 // sweeps (377 against 437 us).
 //
 // C16: columns are 16-bit offsets from the block's base column (cb[q]).
-// LAYOUT: kRbQuad, kRbRows (row walk) or kRbStrided; separate
+// LAYOUT: kRbQuad, kRbRows (row walk), kRbStrided or kRbSorted; separate
 // instantiations, so the quad layout keeps its smaller LDS footprint (8
 // blocks per CU in FP64 instead of 6) and its own register allocation.
 template <typename T, int EPI, bool DOT, bool C16, bool NT, int LAYOUT>
@@ -300,8 +305,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
   constexpr int Q = CAP / (4 * kBlock);  // quads per thread
   static_assert(Q * 4 * kBlock == CAP, "the row-block cap is a whole number of quads per thread");
   static_assert(rb_rows<T>() == kBlock, "one row per thread at most");
+  constexpr bool STAGED = LAYOUT == kRbRows || LAYOUT == kRbStrided;
+  static_assert(LAYOUT != kRbSorted || !C16, "column-sorted blocks keep packed 32-bit columns");
   __shared__ T sval[CAP + 4];  // staged values (row walk, strided), or products
-  __shared__ int scol[LAYOUT != kRbQuad ? CAP + 4 : 1];
+  __shared__ int scol[STAGED ? CAP + 4 : 1];
   __shared__ T wsum[kBlock / 64];
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -410,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
       const int safe = C16 ? b0 : 0;
       const T *sv = sval - ea;
       T sum = T(0);
-      if constexpr (LAYOUT != kRbQuad) {
+      if constexpr (STAGED) {
         if (ne > 0) {
           // every quad and the tail loaded first (addresses clamped into the
           // block, so all loads are issued together), then stored to LDS
@@ -516,7 +523,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
           sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         }
       } else {
-        // gathers in the quad layout, products parked in LDS
+        // gathers in the quad layout, products parked in LDS; column-sorted
+        // blocks park each product at its entry's row-ordered position
         if (ne > 0) {
           int c[Q][4];
           T v[Q][4], xv[Q][4];
@@ -537,8 +545,28 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
           }
           const int kt = ea + 4 * Q * kBlock + tid;
           const bool tail = tid < 3 && kt < e1;
-          const int ct = tail ? col(kt) : safe;
+          int ct = tail ? col(kt) : safe;
           const T vt = tail ? av[kt] : T(0);
+          // LDS slot of each product (relative to ea)
+          int slot[Q][4];
+          int st = kt - ea;
+#pragma unroll
+          for (int u = 0; u < Q; ++u) {
+            const int k = ea + 4 * (tid + u * kBlock);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              if constexpr (LAYOUT == kRbSorted) {
+                slot[u][j] = (int)((uint32_t)c[u][j] >> kRbPosShift) + (e0 - ea);
+                c[u][j] &= kRbColMask;
+              } else {
+                slot[u][j] = k + j - ea;
+              }
+            }
+          }
+          if constexpr (LAYOUT == kRbSorted) {
+            st = (int)((uint32_t)ct >> kRbPosShift) + (e0 - ea);
+            ct &= kRbColMask;
+          }
 #pragma unroll
           for (int u = 0; u < Q; ++u) {
             const int k = ea + 4 * (tid + u * kBlock);
@@ -556,11 +584,16 @@ __global__ __launch_bounds__(kBlock) void k_csr_rb(int nrb, const int2 *__restri
 #pragma unroll
               for (int j = 0; j < 4; ++j) {
                 const bool ok = k + j >= e0 && k + j < e1;
-                sval[k + j - ea] = ok ? v[u][j] * xv[u][j] : T(0);
+                if constexpr (LAYOUT == kRbSorted) {
+                  // entries outside the block belong to its neighbours
+                  if (ok) sval[slot[u][j]] = v[u][j] * xv[u][j];
+                } else {
+                  sval[slot[u][j]] = ok ? v[u][j] * xv[u][j] : T(0);
+                }
               }
             }
           }
-          if (tail) sval[kt - ea] = vt * xt;
+          if (tail) sval[st] = vt * xt;
         }
         __syncthreads();
         // four running sums per lane, so four LDS reads are in flight
@@ -605,8 +638,9 @@ void launch_csr_rb_k(const DevCsr &A, int blocks, const T *x, const T *b, const 
                      const T *dotw, Scal *s, bool check_done, RedWs &ws, T *y2) {
   const int2 *rb = reinterpret_cast<const int2 *>(A.rb<T>());
   hipLaunchKernelGGL((k_csr_rb<T, EPI, DOT, C16, NT, LAYOUT>), dim3(blocks), dim3(kBlock), 0, stream(), A.nrb<T>(), rb,
-                     A.rp.p, C16 ? nullptr : A.ci.p, C16 ? A.cols16<T>() : nullptr, C16 ? A.cbase<T>() : nullptr,
-                     A.vals<T>(), x, b, dinv, (T)omega, y, dotw, s, check_done ? 1 : 0, ws.partials.p, y2);
+                     A.rp.p, C16 ? nullptr : A.rb_cols<T>(), C16 ? A.cols16<T>() : nullptr,
+                     C16 ? A.cbase<T>() : nullptr, A.rb_vals<T>(), x, b, dinv, (T)omega, y, dotw, s,
+                     check_done ? 1 : 0, ws.partials.p, y2);
 }
 
 template <typename T, int EPI, bool DOT>
@@ -623,13 +657,18 @@ void launch_csr_rb(const DevCsr &A, const T *x, const T *b, const T *dinv, doubl
   if (layout == kRbRows) CFDLA_RB(C, N, kRbRows);        \
   else if (layout == kRbStrided) CFDLA_RB(C, N, kRbStrided); \
   else CFDLA_RB(C, N, kRbQuad);
+  // column-sorted blocks never have 16-bit columns
+#define CFDLA_RB4(N)                                      \
+  if (layout == kRbSorted) CFDLA_RB(false, N, kRbSorted); \
+  else { CFDLA_RB3(false, N) }
   if (c16) {
     if (nt) { CFDLA_RB3(true, true) }
     else { CFDLA_RB3(true, false) }
   } else {
-    if (nt) { CFDLA_RB3(false, true) }
-    else { CFDLA_RB3(false, false) }
+    if (nt) { CFDLA_RB4(true) }
+    else { CFDLA_RB4(false) }
   }
+#undef CFDLA_RB4
 #undef CFDLA_RB3
 #undef CFDLA_RB
   if (DOT) finish(1, s, op, blocks, ws, check_done);

@@ -610,6 +610,24 @@ void scatter_formats(DevCsr &d) {
                        d.sdinv64.p, d.sdinv32.p);
 }
 
+namespace {
+__global__ __launch_bounds__(kBlock) void k_sorted_values(int64_t nnz, const double *__restrict__ v64,
+                                                          const int32_t *__restrict__ perm, double *sv64,
+                                                          float *sv32) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock) {
+    const double a = v64[perm[k]];
+    sv64[k] = a;
+    if (sv32) sv32[k] = (float)a;
+  }
+}
+}  // namespace
+
+void sorted_values(DevCsr &d) {
+  if (d.nnz > 0)
+    hipLaunchKernelGGL(k_sorted_values, dim3(grid_for(d.nnz)), dim3(kBlock), 0, stream(), d.nnz, d.v64.p, d.sperm.p,
+                       d.sv64.p, d.sv32.p);
+}
+
 void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, const double *u, const double *v,
                     const double *p, const double *du, const double *dv, const double *rho, const int32_t *rp,
                     double *v64, double *rhs, bool want_max) {

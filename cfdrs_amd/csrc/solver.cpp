@@ -179,12 +179,41 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
   }
 }
 
+bool build_sorted_blocks(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<int32_t> &packed,
+                         std::vector<double> &vals, std::vector<int32_t> &perm) {
+  if (h.m > (int64_t)kRbColMask + 1) return false;
+  const int64_t nnz = h.nnz(), nrb = (int64_t)rb.size() / 2 - 1;
+  packed.resize((size_t)nnz);
+  vals.resize((size_t)nnz);
+  perm.resize((size_t)nnz);
+  std::vector<int32_t> idx;
+  for (int64_t q = 0; q < nrb; ++q) {
+    const int64_t e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
+    idx.resize((size_t)(e1 - e0));
+    for (int64_t k = e0; k < e1; ++k) idx[(size_t)(k - e0)] = (int32_t)k;
+    const bool regular = e1 - e0 <= rb_cap<double>();
+    // a long row keeps its order (the kernel sums it straight from memory)
+    if (regular)
+      std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return h.ci[a] < h.ci[b]; });
+    for (int64_t j = e0; j < e1; ++j) {
+      const int32_t k = idx[(size_t)(j - e0)];
+      const uint32_t pos = regular ? (uint32_t)(k - e0) : 0u;
+      packed[(size_t)j] = (int32_t)((uint32_t)h.ci[k] | (pos << kRbPosShift));
+      vals[(size_t)j] = h.v[k];
+      perm[(size_t)j] = k;
+    }
+  }
+  return true;
+}
+
 int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes) {
   // The layouts of k_csr_rb's regular path, wavefront by wavefront:
   //   quad:    gather instruction (u, j) of wave w covers entry
   //            ea + 4 (64 w + lane + 256 u) + j of the block;
   //   rows:    lane t of wave w walks row t / tpr; its p-th gather is entry
-  //            ks + p (tpr = 1) or ks + t % tpr + p tpr (tpr > 1).
+  //            ks + p (tpr = 1) or ks + t % tpr + p tpr (tpr > 1);
+  //   sorted:  as quad, over the block's columns in ascending order (only
+  //            where every column is below 2^21).
   // (The strided layout is not chosen here: on this count it looks best for
   // R0 and the prolongations, but its second barrier and pass of products
   // through LDS, which the count leaves out, made it slower on P0 (+7-12 %)
@@ -192,16 +221,21 @@ int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes)
   // available through CFDLA_GATHER_STRIDED.)
   // An instruction counts when some lane has a live entry; its lines are the
   // distinct 128-byte lines of x those lanes read.  Measured on the 128^3
-  // hierarchy (FP32 solve, per launch), 0.04 per line orders quad and rows
-  // for every operator: the 7-point operator, R1 and the level-2 operator
-  // faster in rows, R0 and every prolongation faster in quads.
+  // hierarchy (per launch, FP64 / FP32 us), 0.04 per line orders the three
+  // layouts for every operator: the 7-point operator in rows (42.7 / 24.2;
+  // sorted 42.8 / 25.3, quad 47.3 / 29.3), R0, P0, R1, P1 and the level-2
+  // operator sorted (R0 28.4 / 16.5 against quad 31.8 / 18.8, P1 6.5 / 6.1
+  // against 7.3 / 6.9; the others 2-10 % ahead of the better of quad and
+  // rows).  The count leaves out what sorted blocks give up, 16-bit columns
+  // and a contiguous LDS store; they won regardless.
   constexpr double kLineCost = 0.04;
+  const bool sortable = h.m <= (int64_t)kRbColMask + 1;
   const int64_t nrb = (int64_t)rb.size() / 2 - 1;
   if (nrb <= 0) return kRbQuad;
   const int64_t step = std::max<int64_t>(1, nrb / 64);
   const int cap = rb_cap<double>();
-  double instr[3] = {0.0, 0.0, 0.0}, nl[3] = {0.0, 0.0, 0.0};
-  std::vector<int64_t> lines;
+  double instr[4] = {0.0, 0.0, 0.0, 0.0}, nl[4] = {0.0, 0.0, 0.0, 0.0};
+  std::vector<int64_t> lines, sorted_cols;
   lines.reserve(64);
   auto count = [&](int layout) {
     if (lines.empty()) return;
@@ -224,6 +258,19 @@ int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes)
           }
           count(kRbQuad);
         }
+    if (sortable) {
+      sorted_cols.assign(h.ci.begin() + e0, h.ci.begin() + e1);
+      std::sort(sorted_cols.begin(), sorted_cols.end());
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int u = 0; u < cap / (4 * kBlock); ++u)
+          for (int j = 0; j < 4; ++j) {
+            for (int l = 0; l < 64; ++l) {
+              const int64_t k = ea + 4 * (64 * w + l + kBlock * u) + j;
+              if (k >= e0 && k < e1) lines.push_back((sorted_cols[(size_t)(k - e0)] * vbytes) >> 7);
+            }
+            count(kRbSorted);
+          }
+    }
     int64_t tpr = 1;
     while (tpr * 2 * nr <= kBlock && tpr < 64) tpr *= 2;
     for (int w = 0; w < kBlock / 64; ++w) {
@@ -245,8 +292,10 @@ int rb_pick_layout(const HostCsr &h, const std::vector<int32_t> &rb, int vbytes)
       }
     }
   }
-  const double cq = instr[kRbQuad] + kLineCost * nl[kRbQuad], cr = instr[kRbRows] + kLineCost * nl[kRbRows];
-  return cr < cq ? kRbRows : kRbQuad;
+  auto cost = [&](int layout) { return instr[layout] + kLineCost * nl[layout]; };
+  int best = cost(kRbRows) < cost(kRbQuad) ? kRbRows : kRbQuad;
+  if (sortable && cost(kRbSorted) < cost(best)) best = kRbSorted;
+  return best;
 }
 
 namespace {
@@ -526,7 +575,8 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   }
   // gather layout of the row-block product (blocks holds the FP32 partition
   // when there is one); evaluated only for matrices stored as row blocks
-  static_assert(CFDLA_GATHER_QUAD == kRbQuad && CFDLA_GATHER_ROWS == kRbRows && CFDLA_GATHER_STRIDED == kRbStrided,
+  static_assert(CFDLA_GATHER_QUAD == kRbQuad && CFDLA_GATHER_ROWS == kRbRows && CFDLA_GATHER_STRIDED == kRbStrided &&
+                    CFDLA_GATHER_SORTED == kRbSorted,
                 "gather layout codes");
   auto pick = [&](const std::vector<int32_t> &part, int vbytes) {
     if (gather_request != CFDLA_GATHER_AUTO) return gather_request;
@@ -534,6 +584,41 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   };
   layout64 = pick(blocks64, 8);
   layout32 = want_fp32 ? pick(blocks, 4) : kRbQuad;
+  // the column-sorted copy: row-block storage only, one copy for both
+  // partitions (cut alike), 32-bit packed columns in place of 16-bit ones
+  std::vector<int32_t> packed, perm;
+  std::vector<double> svals;
+  const bool want_sorted = !dia && !sell && (layout64 == kRbSorted || layout32 == kRbSorted) &&
+                           (!want_fp32 || blocks == blocks64) && build_sorted_blocks(h, blocks64, packed, svals, perm);
+  if (!want_sorted) {
+    if (layout64 == kRbSorted) layout64 = kRbQuad;
+    if (layout32 == kRbSorted) layout32 = kRbQuad;
+    sci.release();
+    sperm.release();
+    sv64.release();
+    sv32.release();
+  } else {
+    sci.upload_padded(packed.data(), packed.size(), kRbPad);
+    sperm.upload(perm.data(), perm.size());
+    sv64.upload_padded(svals.data(), svals.size(), kRbPad);
+    if (layout32 == kRbSorted) {
+      std::vector<float> sf(svals.size());
+      for (size_t k = 0; k < sf.size(); ++k) sf[k] = (float)svals[k];
+      sv32.upload_padded(sf.data(), sf.size(), kRbPad);
+    } else {
+      sv32.release();
+    }
+    if (layout64 == kRbSorted && c16_f64) {
+      c16_f64 = false;
+      c16_64.release();
+      cbase64.release();
+    }
+    if (layout32 == kRbSorted && c16_f32) {
+      c16_f32 = false;
+      c16_32.release();
+      cbase32.release();
+    }
+  }
   sync_stream();
 }
 
@@ -576,6 +661,7 @@ bool DevCsr::refresh_from_v64() {
   CFDLA_REQUIRE(maps_ready, CFDLA_ERR_INVALID_ARG, "device refresh before its maps were built");
   if (dia && dia_desc.sym && !symmetric_on_device(*this)) return false;
   scatter_formats(*this);
+  if (sci.p) sorted_values(*this);
   return true;
 }
 

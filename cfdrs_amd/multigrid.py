@@ -236,7 +236,8 @@ class AlgebraicMultigrid(_Preconditioner):
 
     def set_gather(self, layout: str) -> None:
         """Gather layout of every level's row-block CSR products: "auto"
-        (chosen per operator), "quad" or "rows" (see CsrMatrix.set_gather)."""
+        (chosen per operator), "quad", "rows", "strided" or "sorted" (see
+        CsrMatrix.set_gather)."""
         from .sparse import CsrMatrix as _Csr
 
         if layout not in _Csr.GATHER_LAYOUTS:

@@ -199,14 +199,16 @@ class CsrMatrix:
         N.call("cfdla_matrix_set_storage", self.handle, self.STORAGE_FORMATS[fmt])
         return self.storage()
 
-    GATHER_LAYOUTS = {"auto": -1, "quad": 0, "rows": 1, "strided": 2}
+    GATHER_LAYOUTS = {"auto": -1, "quad": 0, "rows": 1, "strided": 2, "sorted": 3}
 
     def set_gather(self, layout: str) -> dict:
         """Gather layout of the row-block CSR product: "auto" (chosen per
         matrix and value type at upload), "quad" (x gathered for 4
-        consecutive entries per thread as they are loaded) or "rows" (the
-        block staged in LDS, each row's lanes gathering as they walk it).
-        Returns gather()."""
+        consecutive entries per thread as they are loaded), "rows" (the
+        block staged in LDS, each row's lanes gathering as they walk it),
+        "strided" (staged, entries dealt round robin) or "sorted" (each
+        block stored in column order, products placed back in row order;
+        quad where a column reaches 2^21).  Returns gather()."""
         if layout not in self.GATHER_LAYOUTS:
             raise ValueError(f"unknown gather layout {layout!r}")
         N.call("cfdla_matrix_set_gather", self.handle, self.GATHER_LAYOUTS[layout])
@@ -217,7 +219,7 @@ class CsrMatrix:
         matrices not stored as row-block CSR)."""
         g64, g32 = C.c_int32(), C.c_int32()
         N.call("cfdla_matrix_gather", self.handle, C.byref(g64), C.byref(g32))
-        names = {0: "quad", 1: "rows", 2: "strided"}
+        names = {0: "quad", 1: "rows", 2: "strided", 3: "sorted"}
         return {"fp64": names[g64.value], "fp32": names[g32.value]}
 
     def diagonal(self):

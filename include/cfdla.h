@@ -236,20 +236,26 @@ cfdla_status cfdla_matrix_set_storage(cfdla_matrix m, int32_t format);
  * them; CFDLA_GATHER_ROWS: the block's columns and values are staged in LDS
  * and each row's lanes gather x while walking the row, so lanes of a
  * wavefront gather for consecutive rows.  CFDLA_GATHER_AUTO (the default)
- * picks per matrix and value type, at upload, the layout whose gathers cost
- * less over a sample of 64 row blocks (instructions with a live lane plus 0.04
- * per distinct 128-byte line).  Results differ between layouts by rounding
- * only.  No counterpart in the reference; it exists to time and test each
- * layout. */
+ * picks per matrix and value type, at upload, among QUAD, ROWS and SORTED
+ * the layout whose gathers cost least over a sample of 64 row blocks
+ * (instructions with a live lane plus 0.04 per distinct 128-byte line).
+ * Results differ between layouts by rounding only.  No counterpart in the
+ * reference; it exists to time and test each layout. */
 #define CFDLA_GATHER_AUTO (-1)
 #define CFDLA_GATHER_QUAD 0
 #define CFDLA_GATHER_ROWS 1
 /* CFDLA_GATHER_STRIDED: the block staged in LDS as for ROWS, entries dealt
  * to threads round robin (a wavefront gathers for 64 consecutive stored
  * entries), products summed per row as for QUAD.  Never picked by
- * CFDLA_GATHER_AUTO (which chooses QUAD or ROWS); it suits restrictions with
- * long rows in FP64. */
+ * CFDLA_GATHER_AUTO. */
 #define CFDLA_GATHER_STRIDED 2
+/* CFDLA_GATHER_SORTED: every block's entries stored in column order (each
+ * packed with its position in the row-ordered block), so a wavefront gathers
+ * for 64 stored entries of adjacent columns as it loads them; the products
+ * are placed in row order in LDS and summed as for QUAD, which they equal bit
+ * for bit.  Needs fewer than 2^21 columns (otherwise QUAD is used) and keeps
+ * 32-bit columns. */
+#define CFDLA_GATHER_SORTED 3
 cfdla_status cfdla_matrix_set_gather(cfdla_matrix m, int32_t layout);
 /* Layout in use for FP64 and FP32 products (uploads the matrix if needed);
  * CFDLA_GATHER_QUAD for matrices not stored as row-block CSR. */

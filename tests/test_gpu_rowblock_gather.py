@@ -3,8 +3,10 @@
 quad: every thread gathers x for 4 consecutive stored entries of its block as
 it loads them; rows: the block's columns and values are staged in LDS and
 each row's lanes gather x while walking the row; strided: staged as for rows,
-entries dealt to threads round robin, products summed per row as for quad.
-Tolerances:
+entries dealt to threads round robin, products summed per row as for quad;
+sorted: the quad layout over blocks stored in column order, each product
+placed back at its row-ordered position, so its sums are the quad layout's
+bit for bit.  Tolerances:
 
 * integer-valued products: bit-exact in FP64 and FP32 (every product and
   partial sum is an exact integer below 2^24);
@@ -22,7 +24,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = ["quad", "rows", "strided"]
+LAYOUTS = ["quad", "rows", "strided", "sorted"]
 
 
 def _rows(lens, m, seed):
@@ -126,6 +128,7 @@ def test_gather_auto_choice():
     a.set_gather("auto")
     assert a.gather() == {"fp64": "rows", "fp32": "rows"}
     assert a.set_gather("strided") == {"fp64": "strided", "fp32": "strided"}
+    assert a.set_gather("sorted") == {"fp64": "sorted", "fp32": "sorted"}
     with pytest.raises(ValueError):
         a.set_gather("columns")
 
@@ -166,6 +169,47 @@ def test_amg_vcycle_in_every_layout(nn):
         out[layout] = z
     for layout in ("rows", "strided"):
         assert np.linalg.norm(out["quad"] - out[layout]) <= 1e-13 * np.linalg.norm(out["quad"])
+    # column-sorted blocks sum their products exactly as the quad layout does
+    assert np.array_equal(out["quad"], out["sorted"])
+
+
+@pytest.mark.parametrize("case", ["short", "mixed", "stencil7"])
+def test_sorted_layout_matches_quad_bit_for_bit(case):
+    # real values: the column-sorted blocks must place every product where
+    # the quad layout sums it, so the two agree in every bit
+    import torch
+
+    rp, ci, v, n, m = _case(case)
+    v = v + np.random.default_rng(4).uniform(-0.5, 0.5, size=v.size)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
+    a.set_storage("csr_rowblock")
+    x = np.random.default_rng(5).standard_normal(m)
+    for dt in (torch.float64, torch.float32):
+        xt = torch.tensor(x, dtype=dt, device="cuda")
+        a.set_gather("quad")
+        yq = a.spmv(xt).cpu().numpy()
+        assert a.set_gather("sorted") == {"fp64": "sorted", "fp32": "sorted"}
+        ys = a.spmv(xt).cpu().numpy()
+        assert np.array_equal(yq, ys), dt
+
+
+def test_sorted_layout_device_value_refresh():
+    # a device value refresh rewrites the column-sorted copies through their
+    # slot map; integer values keep the products exact
+    import torch
+
+    rp, ci, v, n, m = _case("mixed")
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
+    a.set_storage("csr_rowblock")
+    a.set_gather("sorted")
+    x = np.random.default_rng(8).integers(-20, 21, size=m).astype(float)
+    assert np.array_equal(a.spmv(x), O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
+    v2 = np.random.default_rng(9).integers(-9, 10, size=v.size).astype(float)
+    a.update_values_device(torch.tensor(v2, dtype=torch.float64, device="cuda"))
+    yo = O.Csr.from_arrays(rp, ci, v2, n, m).spmv(x)
+    assert np.array_equal(a.spmv(x), yo)
+    y32 = a.spmv(torch.tensor(x, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+    assert np.array_equal(y32, yo)
 
 
 @pytest.mark.parametrize("precision", [ca.Precision.FP64, ca.Precision.FP32_REFINED])
