@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # 1.0 KiB, the factors that make the counters read 512 MiB each for a 512 MiB
 # device copy.  Keyed by the device format measured (128^3 operator).
 SPMV_FP64_TRAFFIC_BYTES = {"dia": round((42329.487 * 2.0 + 16384.0) * 1024),
-                           "csr_int32": round((99715.755 * 2.0 + 16384.0) * 1024)}
+                           "csr_int32": round((99720.64 * 2.0 + 16384.0) * 1024)}
 
 
 def log(msg: str) -> None:
