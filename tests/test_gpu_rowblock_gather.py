@@ -193,6 +193,26 @@ def test_sorted_layout_matches_quad_bit_for_bit(case):
         assert np.array_equal(yq, ys), dt
 
 
+@pytest.mark.parametrize("ncols", [1 << 21, (1 << 21) + 1])
+def test_sorted_layout_column_limit(ncols):
+    # columns are packed in 21 bits: 2^21 columns take the sorted layout,
+    # one more falls back to quad; both stay exact
+    rng = np.random.default_rng(10)
+    lens = rng.integers(0, 40, size=300)
+    rp, ci, v = _rows(lens, ncols, 11)
+    ci[rp[5]] = ncols - 1 if lens[5] > 0 else ci[rp[5]]
+    ci = np.concatenate([ci, [0, ncols - 1]])
+    v = np.concatenate([v, [3.0, -2.0]])
+    rp = np.concatenate([rp, [rp[-1] + 2]])
+    n = rp.size - 1
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, ncols)
+    a.set_storage("csr_rowblock")
+    want = "sorted" if ncols <= (1 << 21) else "quad"
+    assert a.set_gather("sorted") == {"fp64": want, "fp32": want}
+    x = rng.integers(-20, 21, size=ncols).astype(float)
+    assert np.array_equal(a.spmv(x), O.Csr.from_arrays(rp, ci, v, n, ncols).spmv(x))
+
+
 def test_sorted_layout_device_value_refresh():
     # a device value refresh rewrites the column-sorted copies through their
     # slot map; integer values keep the products exact
