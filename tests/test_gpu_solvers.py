@@ -37,31 +37,35 @@ def random_csr(n, m, row_len_max, seed, integer=False, empty_every=0):
 
 
 # m = 2777 keeps every row block's columns within 65536 (16-bit column
-# offsets from a per-block base); m = 90001 does not (32-bit columns)
+# offsets from a per-block base, read by the quad layout); m = 90001 does not
+# (32-bit columns); "auto" takes the layout the upload rule picks
+@pytest.mark.parametrize("layout", ["auto", "quad"])
 @pytest.mark.parametrize("m", [2777, 90001])
 @pytest.mark.parametrize("row_len", [1, 3, 7, 16, 40, 130])
-def test_spmv_integer_bit_exact(row_len, m):
+def test_spmv_integer_bit_exact(row_len, m, layout):
     n = 3001
     rp, ci, v = random_csr(n, m, row_len, seed=row_len, integer=True, empty_every=97)
     x = np.random.default_rng(1).integers(-50, 51, size=m).astype(float)
-    y = ca.CsrMatrix.from_parts(v, ci, rp, n, m).spmv(x)
+    a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
+    a.set_gather(layout)
+    y = a.spmv(x)
     assert np.array_equal(y, O.Csr.from_arrays(rp, ci, v, n, m).spmv(x))
 
 
-# Row blocks hold at most 2048 nonzeros and 512 (FP64) / 256 (FP32) rows, so
-# with 5 entries per row the FP64 blocks are rows [0, 409), [409, 818), ...
-# and the FP32 blocks rows [0, 256), [256, 512), ...  A block also closes
+# Row blocks hold at most 2048 nonzeros and 256 rows, so with 5 entries per
+# row the blocks are rows [0, 256), [256, 512), ...  A block also closes
 # before its columns would span more than 65535, unless some row alone does,
 # and a partition stores 16-bit column offsets when every block's columns
-# satisfy max - min <= 65535.
-#   span65535: every row holds columns {0, 7, 1000, 30000, 65535}: 16-bit in
-#     both partitions, with the largest offset that fits;
-#   span65536: the last column is 65536: no cut helps, 32-bit in both;
+# satisfy max - min <= 65535.  The quad and row-walk layouts read them (the
+# column-sorted one keeps packed 32-bit columns), so both are forced here.
+#   span65535: every row holds columns {0, 7, 1000, 30000, 65535}: 16-bit,
+#     with the largest offset that fits;
+#   span65536: the last column is 65536: no cut helps, 32-bit;
 #   fp32_straddles: rows below 409 hold columns 0..4 and rows from 409 on
-#     70000..70004, so every FP64 block spans 4 while the FP32 block from 256
-#     closes at row 409: 16-bit in both partitions.
+#     70000..70004, so the block from row 256 closes at row 409: 16-bit.
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("case", ["span65535", "span65536", "fp32_straddles"])
-def test_spmv_16bit_column_span_boundary(case):
+def test_spmv_16bit_column_span_boundary(case, layout):
     import torch
 
     n = 818 if case == "fp32_straddles" else 1500
@@ -79,6 +83,7 @@ def test_spmv_16bit_column_span_boundary(case):
     x = rng.integers(-50, 51, size=m).astype(float)
     a = ca.CsrMatrix.from_parts(v, ci, rp, n, m)
     assert a.storage()["format"] == "csr_rowblock"
+    assert a.set_gather(layout) == {"fp64": layout, "fp32": layout}
     yo = O.Csr.from_arrays(rp, ci, v, n, m).spmv(x)
     assert np.array_equal(a.spmv(x), yo)
     y32 = a.spmv(torch.from_numpy(x.astype(np.float32)).cuda()).cpu().numpy()
