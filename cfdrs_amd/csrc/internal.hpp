@@ -433,6 +433,9 @@ void axpy_f32_to_f64(int n, double a, const float *x, double *y);
 // Residual replacement of the reliable-update CG, skipped when check_done
 // and s->done != 0:  x += (double) xc, xc = 0  /  dst = (float) src.
 void fold_correction(int n, float *xc, double *x, Scal *s, bool check_done);
+// cg_update<float> (no Jacobi) and fold_correction in one pass, for a step
+// that replaces the residual: x64 += xc + alpha p, xc = 0, r -= alpha q, r.r.
+void cg_update_fold(int n, const float *p, const float *q, float *xc, double *x64, float *r, Scal *s, RedWs &ws);
 void f64_to_f32(int n, const double *src, float *dst, Scal *s, bool check_done);
 
 // x = Ainv * b (dense row-major FP64 inverse), b/x of type T.

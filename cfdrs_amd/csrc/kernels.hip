@@ -869,6 +869,30 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, const T *__restrict
   reduce_and_finalize<JAC ? 2 : 1>(v, s, JAC ? OP_CG_RR_RZ : OP_CG_RR, partials, counter);
 }
 
+// The FP32 update of a step that replaces the residual, with the correction
+// folded into the FP64 iterate in the same pass: xc + alpha p goes to x64 and
+// xc restarts at zero, r -= alpha q, r.r for the finalize op.  The same
+// operations, in the same forms, as k_cg_update<float, false> followed by
+// k_fold_correction.
+__global__ __launch_bounds__(kBlock) void k_cg_update_fold(int n, const float *__restrict__ p,
+                                                           const float *__restrict__ q, float *__restrict__ xc,
+                                                           double *__restrict__ x64, float *__restrict__ r, Scal *s,
+                                                           double *partials, unsigned *counter) {
+  if (s->done) return;
+  const float alpha = (float)s->alpha;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    float ri = r[i] - alpha * q[i];
+    float ci = xc[i];
+    ci += alpha * p[i];
+    x64[i] += (double)ci;
+    xc[i] = 0.0f;
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  reduce_and_finalize<1>(v, s, OP_CG_RR, partials, counter);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, const T *__restrict__ z, T *__restrict__ p,
                                                          Scal *s) {
@@ -1188,6 +1212,14 @@ void cg_update(int n, const T *p, const T *q, T *x, T *r, const T *jdinv, T *z, 
     hipLaunchKernelGGL((k_cg_update<T, false>), dim3(blocks), dim3(kBlock), 0, stream(), n, p, q, x, r, jdinv,
                        z, s, ws.partials.p, ws.counter.p);
   finish(jdinv ? 2 : 1, s, jdinv ? OP_CG_RR_RZ : OP_CG_RR, blocks, ws, true);
+  CFDLA_HIP(hipGetLastError());
+}
+
+void cg_update_fold(int n, const float *p, const float *q, float *xc, double *x64, float *r, Scal *s, RedWs &ws) {
+  const int blocks = blocks_for(n, kBlock * 4);
+  hipLaunchKernelGGL(k_cg_update_fold, dim3(blocks), dim3(kBlock), 0, stream(), n, p, q, xc, x64, r, s,
+                     ws.partials.p, ws.counter.p);
+  finish(1, s, OP_CG_RR, blocks, ws, true);
   CFDLA_HIP(hipGetLastError());
 }
 

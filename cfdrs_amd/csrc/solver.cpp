@@ -1416,9 +1416,10 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
   M->apply_dot(n, r, z, s, false, OP_SET_RZ);
   copy_vec<float>(n, z, p, s, false);
   if (chunk <= 0) chunk = auto_chunk(M, n);
-  // FP64 residual replacement; check_done skips it once the solve stopped
-  auto replace = [&](bool check_done) {
-    fold_correction(n, xc, x, s, check_done);
+  // FP64 residual replacement; check_done skips it once the solve stopped.
+  // folded: the step's update has already moved the correction into x.
+  auto replace = [&](bool check_done, bool folded = false) {
+    if (!folded) fold_correction(n, xc, x, s, check_done);
     csr_apply<double>(A, EPI_RESID, x, b, nullptr, 0.0, r64, s, check_done, OP_RELIABLE, r64, ws);
     f64_to_f32(n, r64, r, s, check_done);
   };
@@ -1456,9 +1457,15 @@ SolveOut pcg_reliable(const DevCsr &A, Precond *M, const double *b, double *x, d
           // xc += alpha p, r -= alpha q, z = D^-1 r, rr and rz -> beta in one pass
           cg_update<float>(n, p, q, xc, r, jd, z, s, ws);
           if (!fuse) cg_direction<float>(n, z, p, s);
+        } else if (rep) {
+          // xc + alpha p into x (xc = 0), r -= alpha q, rr; then the FP64
+          // residual.  A step whose rr stops the solve skips the residual;
+          // the fold it already did leaves the final fold nothing to add.
+          cg_update_fold(n, p, q, xc, x, r, s, ws);
+          replace(true, true);
+          direction();
         } else {
           cg_update<float>(n, p, q, xc, r, nullptr, nullptr, s, ws);  // xc += alpha p, r -= alpha q, rr
-          if (rep) replace(true);
           direction();
         }
       }
