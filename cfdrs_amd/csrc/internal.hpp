@@ -189,9 +189,9 @@ struct DevCsr {
   // kRbSorted: every block of at most rb_cap entries stored in column order
   // (ties in row order), each entry packed with its position in the
   // row-ordered block (kRbPosShift); longer rows stay in order with position
-  // 0.  One copy for both partitions (they are cut alike); sperm holds the
-  // CSR slot of every stored entry, for the device value refresh.
-  DevBuf<int32_t> sci, sperm;
+  // 0.  One copy for both partitions (they are cut alike); a device value
+  // refresh finds each entry's CSR slot from its block and position.
+  DevBuf<int32_t> sci;
   DevBuf<double> sv64;
   DevBuf<float> sv32;
   DevBuf<double> v64, sdinv64;
@@ -257,11 +257,10 @@ template <> inline const float *DevCsr::rb_vals<float>() const { return layout32
 // span > 0: a block also closes before its columns would span more than
 // `span` (so 16-bit column offsets fit every block whose rows each do).
 void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32_t> &rb, int64_t span = 0);
-// Column-sorted copy of a row-block partition (kRbSorted): packed columns,
-// FP64 values and the CSR slot of every stored entry; false (nothing built)
-// when a column reaches 2^21.
+// Column-sorted copy of a row-block partition (kRbSorted): packed columns
+// and FP64 values; false (nothing built) when a column reaches 2^21.
 bool build_sorted_blocks(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<int32_t> &packed,
-                         std::vector<double> &vals, std::vector<int32_t> &perm);
+                         std::vector<double> &vals);
 // The gather layout (kRbQuad, kRbRows or, below 2^21 columns, kRbSorted)
 // whose gathers cost least
 // for this partition (value size vbytes): instructions with a live lane plus
@@ -343,7 +342,8 @@ void gather_values(int64_t nnz, const double *src, const int32_t *perm, double *
 bool symmetric_on_device(const DevCsr &d);
 // FP32 copy, diagonal / sliced-ELL arrays and smoother diagonal from v64.
 void scatter_formats(DevCsr &d);
-// Column-sorted row-block values from v64 through sperm (kRbSorted).
+// Column-sorted row-block values from v64 (kRbSorted): entry j of block q
+// takes CSR slot e0 + its packed position (a row longer than rb_cap: j).
 void sorted_values(DevCsr &d);
 // S (into sv) and E (into ev) values of a level's fused operator from A's
 // values and smoother diagonal (fused_jacobi.hpp); both follow A's slots.

@@ -611,21 +611,29 @@ void scatter_formats(DevCsr &d) {
 }
 
 namespace {
-__global__ __launch_bounds__(kBlock) void k_sorted_values(int64_t nnz, const double *__restrict__ v64,
-                                                          const int32_t *__restrict__ perm, double *sv64,
+// One workgroup per row block at a time; the FP64 partition (both are cut
+// alike when the sorted copy exists).
+__global__ __launch_bounds__(kBlock) void k_sorted_values(int nrb, const int *__restrict__ rb,
+                                                          const int32_t *__restrict__ sci,
+                                                          const double *__restrict__ v64, double *sv64,
                                                           float *sv32) {
-  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBlock) {
-    const double a = v64[perm[k]];
-    sv64[k] = a;
-    if (sv32) sv32[k] = (float)a;
+  for (int q = blockIdx.x; q < nrb; q += gridDim.x) {
+    const int e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
+    const bool regular = e1 - e0 <= rb_cap<double>();
+    for (int j = e0 + (int)threadIdx.x; j < e1; j += kBlock) {
+      const int k = regular ? e0 + (int)((uint32_t)sci[j] >> kRbPosShift) : j;
+      const double a = v64[k];
+      sv64[j] = a;
+      if (sv32) sv32[j] = (float)a;
+    }
   }
 }
 }  // namespace
 
 void sorted_values(DevCsr &d) {
-  if (d.nnz > 0)
-    hipLaunchKernelGGL(k_sorted_values, dim3(grid_for(d.nnz)), dim3(kBlock), 0, stream(), d.nnz, d.v64.p, d.sperm.p,
-                       d.sv64.p, d.sv32.p);
+  if (d.nnz > 0 && d.nrb64 > 0)
+    hipLaunchKernelGGL(k_sorted_values, dim3(std::min(d.nrb64, kMaxBlocks)), dim3(kBlock), 0, stream(), d.nrb64,
+                       d.rb64.p, d.sci.p, d.v64.p, d.sv64.p, d.sv32.p);
 }
 
 void simple_refresh(int64_t nx, int64_t ny, double dx, double dy, bool negate, const double *u, const double *v,

@@ -180,28 +180,24 @@ void build_row_blocks(const HostCsr &h, int cap, int max_rows, std::vector<int32
 }
 
 bool build_sorted_blocks(const HostCsr &h, const std::vector<int32_t> &rb, std::vector<int32_t> &packed,
-                         std::vector<double> &vals, std::vector<int32_t> &perm) {
+                         std::vector<double> &vals) {
   if (h.m > (int64_t)kRbColMask + 1) return false;
   const int64_t nnz = h.nnz(), nrb = (int64_t)rb.size() / 2 - 1;
   packed.resize((size_t)nnz);
   vals.resize((size_t)nnz);
-  perm.resize((size_t)nnz);
-  std::vector<int32_t> idx;
   for (int64_t q = 0; q < nrb; ++q) {
     const int64_t e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
-    idx.resize((size_t)(e1 - e0));
-    for (int64_t k = e0; k < e1; ++k) idx[(size_t)(k - e0)] = (int32_t)k;
     const bool regular = e1 - e0 <= rb_cap<double>();
-    // a long row keeps its order (the kernel sums it straight from memory)
-    if (regular)
-      std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return h.ci[a] < h.ci[b]; });
-    for (int64_t j = e0; j < e1; ++j) {
-      const int32_t k = idx[(size_t)(j - e0)];
-      const uint32_t pos = regular ? (uint32_t)(k - e0) : 0u;
-      packed[(size_t)j] = (int32_t)((uint32_t)h.ci[k] | (pos << kRbPosShift));
-      vals[(size_t)j] = h.v[k];
-      perm[(size_t)j] = k;
+    // the packed word itself is the sort key: column, then position, so
+    // ties keep row order (a long row keeps its order, with position 0)
+    for (int64_t k = e0; k < e1; ++k)
+      packed[(size_t)k] = (int32_t)((uint32_t)h.ci[(size_t)k] | (regular ? (uint32_t)(k - e0) << kRbPosShift : 0u));
+    if (regular) {
+      const auto key = [](int32_t w) { return ((uint32_t)w & (uint32_t)kRbColMask) << 11 | (uint32_t)w >> kRbPosShift; };
+      std::sort(packed.begin() + e0, packed.begin() + e1, [&](int32_t a, int32_t b) { return key(a) < key(b); });
     }
+    for (int64_t j = e0; j < e1; ++j)
+      vals[(size_t)j] = h.v[(size_t)(regular ? e0 + ((uint32_t)packed[(size_t)j] >> kRbPosShift) : j)];
   }
   return true;
 }
@@ -586,25 +582,24 @@ void DevCsr::upload(const HostCsr &h, bool want_fp32, bool want_dinv, bool allow
   layout32 = want_fp32 ? pick(blocks, 4) : kRbQuad;
   // the column-sorted copy: row-block storage only, one copy for both
   // partitions (cut alike), 32-bit packed columns in place of 16-bit ones
-  std::vector<int32_t> packed, perm;
+  std::vector<int32_t> packed;
   std::vector<double> svals;
   const bool want_sorted = !dia && !sell && (layout64 == kRbSorted || layout32 == kRbSorted) &&
-                           (!want_fp32 || blocks == blocks64) && build_sorted_blocks(h, blocks64, packed, svals, perm);
+                           (!want_fp32 || blocks == blocks64) && build_sorted_blocks(h, blocks64, packed, svals);
   if (!want_sorted) {
     if (layout64 == kRbSorted) layout64 = kRbQuad;
     if (layout32 == kRbSorted) layout32 = kRbQuad;
     sci.release();
-    sperm.release();
     sv64.release();
     sv32.release();
   } else {
     sci.upload_padded(packed.data(), packed.size(), kRbPad);
-    sperm.upload(perm.data(), perm.size());
     sv64.upload_padded(svals.data(), svals.size(), kRbPad);
     if (layout32 == kRbSorted) {
       std::vector<float> sf(svals.size());
       for (size_t k = 0; k < sf.size(); ++k) sf[k] = (float)svals[k];
       sv32.upload_padded(sf.data(), sf.size(), kRbPad);
+      sync_stream();  // sf is freed at the end of this block
     } else {
       sv32.release();
     }
