@@ -11,6 +11,7 @@
 #include "fused_jacobi.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -185,20 +186,33 @@ bool build_sorted_blocks(const HostCsr &h, const std::vector<int32_t> &rb, std::
   const int64_t nnz = h.nnz(), nrb = (int64_t)rb.size() / 2 - 1;
   packed.resize((size_t)nnz);
   vals.resize((size_t)nnz);
-  for (int64_t q = 0; q < nrb; ++q) {
-    const int64_t e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
-    const bool regular = e1 - e0 <= rb_cap<double>();
-    // the packed word itself is the sort key: column, then position, so
-    // ties keep row order (a long row keeps its order, with position 0)
-    for (int64_t k = e0; k < e1; ++k)
-      packed[(size_t)k] = (int32_t)((uint32_t)h.ci[(size_t)k] | (regular ? (uint32_t)(k - e0) << kRbPosShift : 0u));
-    if (regular) {
-      const auto key = [](int32_t w) { return ((uint32_t)w & (uint32_t)kRbColMask) << 11 | (uint32_t)w >> kRbPosShift; };
-      std::sort(packed.begin() + e0, packed.begin() + e1, [&](int32_t a, int32_t b) { return key(a) < key(b); });
+  // blocks are independent: contiguous ranges of them go to up to 16 host
+  // threads (sorting the 24M entries of the 128^3 hierarchy's transfer and
+  // level-2 operators on one thread took most of the hierarchy's upload)
+  auto work = [&](int64_t q0, int64_t q1) {
+    for (int64_t q = q0; q < q1; ++q) {
+      const int64_t e0 = rb[2 * q + 1], e1 = rb[2 * q + 3];
+      const bool regular = e1 - e0 <= rb_cap<double>();
+      // the packed word itself is the sort key: column, then position, so
+      // ties keep row order (a long row keeps its order, with position 0)
+      for (int64_t k = e0; k < e1; ++k)
+        packed[(size_t)k] =
+            (int32_t)((uint32_t)h.ci[(size_t)k] | (regular ? (uint32_t)(k - e0) << kRbPosShift : 0u));
+      if (regular) {
+        const auto key = [](int32_t w) {
+          return ((uint32_t)w & (uint32_t)kRbColMask) << 11 | (uint32_t)w >> kRbPosShift;
+        };
+        std::sort(packed.begin() + e0, packed.begin() + e1, [&](int32_t a, int32_t b) { return key(a) < key(b); });
+      }
+      for (int64_t j = e0; j < e1; ++j)
+        vals[(size_t)j] = h.v[(size_t)(regular ? e0 + ((uint32_t)packed[(size_t)j] >> kRbPosShift) : j)];
     }
-    for (int64_t j = e0; j < e1; ++j)
-      vals[(size_t)j] = h.v[(size_t)(regular ? e0 + ((uint32_t)packed[(size_t)j] >> kRbPosShift) : j)];
-  }
+  };
+  const int64_t nt = std::min<int64_t>({16, std::max(1u, std::thread::hardware_concurrency()), nnz / (1 << 16) + 1});
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work, nrb * t / nt, nrb * (t + 1) / nt);
+  work(0, nrb / nt);
+  for (std::thread &th : pool) th.join();
   return true;
 }
 
